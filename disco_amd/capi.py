@@ -1,0 +1,226 @@
+"""ctypes view of the C ABI in include/disco_hip.h, and the Python driver of
+libdisco_hip.so.
+
+`DiscoHip` is a thin handle over the dh_* entry points; it is what the tests
+and bench.py call.  The structures below mirror dh_grid / dh_params field for
+field.  The library is loaded from the package directory (built in-tree by
+`python -m disco_amd.build`); a missing library is an error, never a fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from . import configs as cfgmod
+from .state_io import MeshState
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdisco_hip.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class DhGrid(C.Structure):
+    _fields_ = [("Nr", C.c_int), ("Nz", C.c_int), ("NgRa", C.c_int), ("NgRb", C.c_int),
+                ("NgZa", C.c_int), ("NgZb", C.c_int), ("Np", _ip), ("r_jph", _dp),
+                ("z_kph", _dp), ("phi_max", C.c_double)]
+
+
+_PARAM_INTS = ["num_c", "num_n", "ct_mode", "hydro", "riemann_solver", "mesh_motion",
+               "exact_mesh_omega", "energy_omega", "timestep_mode", "ct", "isothermal",
+               "cs2_profile", "visc_flag", "alpha_flag", "rot_frame", "grav2d",
+               "nobc_rmin", "nobc_rmax", "nobc_zmin", "nobc_zmax", "r_periodic",
+               "z_periodic", "bc_rmin", "bc_rmax", "bc_zmin", "bc_zmax", "nozzle_type"]
+_PARAM_DOUBLES = ["gamma_law", "plm", "cfl", "max_dt", "rho_floor", "p_floor",
+                  "viscosity", "mach", "metric_par2", "rot_omega", "rot_d"]
+
+
+class DhParams(C.Structure):
+    _fields_ = ([(n, C.c_int) for n in _PARAM_INTS] + [("pad_", C.c_int * 3)]
+                + [(n, C.c_double) for n in _PARAM_DOUBLES]
+                + [("init_par", C.c_double * 9)])
+
+
+PLANET_STRIDE = 8
+MAX_PLANETS = 8
+
+# in.par key -> dh_params field (readpar.c:70-155 names)
+_PAR_INT_KEYS = {
+    "Riemann_Solver": "riemann_solver", "Mesh_Motion": "mesh_motion",
+    "Exact_Mesh_Omega": "exact_mesh_omega", "Energy_Omega": "energy_omega",
+    "Timestep": "timestep_mode", "Constrained_Transport": "ct",
+    "Isothermal": "isothermal", "Cs2_Profile": "cs2_profile",
+    "Use_Viscosity": "visc_flag", "Use_As_Alpha": "alpha_flag", "RotFrame": "rot_frame",
+    "Grav2D": "grav2d", "NoBC_Rmin": "nobc_rmin", "NoBC_Rmax": "nobc_rmax",
+    "NoBC_Zmin": "nobc_zmin", "NoBC_Zmax": "nobc_zmax", "R_Periodic": "r_periodic",
+    "Z_Periodic": "z_periodic", "Nozzle_Type": "nozzle_type",
+}
+_PAR_DBL_KEYS = {
+    "Adiabatic_Index": "gamma_law", "PLM": "plm", "CFL": "cfl", "Max_DT": "max_dt",
+    "Density_Floor": "rho_floor", "Pressure_Floor": "p_floor", "Viscosity": "viscosity",
+    "Mach_Number": "mach", "Metric_Par2": "metric_par2", "RotOmega": "rot_omega",
+    "RotD": "rot_d",
+}
+
+
+def make_params(cfg: "cfgmod.Config", par: Dict[str, object]) -> DhParams:
+    from .state_io import _DEFAULTS
+    full = dict(_DEFAULTS)
+    full.update(par)
+    p = DhParams()
+    p.num_c, p.num_n, p.ct_mode = cfg.num_c, cfg.num_n, cfg.ct_mode
+    p.hydro = cfgmod.HYDRO_MHD if cfg.hydro == "mhd" else cfgmod.HYDRO_EULER
+    for k, f in _PAR_INT_KEYS.items():
+        # readpar.c:31-36 parses integers through %lf and a cast.
+        setattr(p, f, int(float(full[k])))
+    for k, f in _PAR_DBL_KEYS.items():
+        setattr(p, f, float(full[k]))
+    for i in range(9):
+        p.init_par[i] = float(full[f"Init_Par{i}"])
+    p.bc_rmin, p.bc_rmax, p.bc_zmin, p.bc_zmax = cfg.dh_bc(full)
+    return p
+
+
+class GridView:
+    """dh_grid plus the numpy arrays it points into (kept alive here)."""
+
+    def __init__(self, s: MeshState):
+        self.np_ = np.ascontiguousarray(s.np_, dtype=np.int32)
+        self.r = np.ascontiguousarray(s.r_jph, dtype=np.float64)
+        self.z = np.ascontiguousarray(s.z_kph, dtype=np.float64)
+        self.g = DhGrid(s.nr, s.nz, *s.ng, self.np_.ctypes.data_as(_ip),
+                        self.r.ctypes.data_as(_dp), self.z.ctypes.data_as(_dp), s.phi_max)
+
+
+def _ptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("state arrays must be C-contiguous float64")
+    return a.ctypes.data_as(_dp)
+
+
+class DhError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    if not os.path.exists(path):
+        raise DhError(f"{path} is missing: build it with `python -m disco_amd.build`")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    sig = {
+        "dh_create": ([C.POINTER(DhGrid), C.POINTER(DhParams), C.POINTER(vp)], C.c_int),
+        "dh_destroy": ([vp], C.c_int),
+        "dh_sizes": ([vp, C.POINTER(C.c_long), _ip, _ip], C.c_int),
+        "dh_upload_state": ([vp, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int),
+        "dh_download_state": ([vp, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int),
+        "dh_set_planets": ([vp, C.c_int, _dp, C.c_int], C.c_int),
+        "dh_getmindt": ([vp, _dp], C.c_int),
+        "dh_timestep": ([vp, C.c_double], C.c_int),
+        "dh_sync": ([vp], C.c_int),
+        "dh_enable_timers": ([vp, C.c_int], C.c_int),
+        "dh_read_timers": ([vp, C.c_int, C.c_void_p, _dp, C.POINTER(C.c_long), _ip], C.c_int),
+        "dh_last_error": ([], C.c_char_p),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib
+
+
+EXPORTED_SYMBOLS = ["dh_create", "dh_destroy", "dh_sizes", "dh_upload_state",
+                    "dh_download_state", "dh_set_planets", "dh_getmindt", "dh_timestep",
+                    "dh_sync", "dh_enable_timers", "dh_read_timers", "dh_last_error"]
+
+_LIB: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        _LIB = load_library()
+    return _LIB
+
+
+class DiscoHip:
+    """One mesh on the GPU, driven through the dh_* C ABI."""
+
+    def __init__(self, cfg: "cfgmod.Config", par: Dict[str, object], state: MeshState,
+                 planets: Optional[np.ndarray] = None):
+        self.lib = lib()
+        self.cfg = cfg
+        self.grid = GridView(state)
+        self.params = make_params(cfg, par)
+        self.ctx = C.c_void_p()
+        self._check(self.lib.dh_create(C.byref(self.grid.g), C.byref(self.params),
+                                       C.byref(self.ctx)))
+        self.ntot = state.ntot
+        self.nq = cfg.nq
+        self.nf = cfg.nf
+        self.upload(state)
+        pl = state.planets if planets is None else planets
+        self.set_planets(0, pl)
+        self.set_planets(1, pl)
+
+    def _check(self, rc: int):
+        if rc != 0:
+            msg = self.lib.dh_last_error()
+            raise DhError(f"dh call failed ({rc}): {msg.decode() if msg else ''}")
+
+    def upload(self, s: MeshState):
+        phi = s.phi_flux if self.nf else None
+        self._check(self.lib.dh_upload_state(
+            self.ctx, _ptr(np.ascontiguousarray(s.prim)), _ptr(np.ascontiguousarray(s.cons)),
+            _ptr(s.piph), _ptr(s.dphi), _ptr(s.wiph),
+            _ptr(np.ascontiguousarray(phi)) if phi is not None else None))
+
+    def download(self, s: MeshState) -> MeshState:
+        out = s.copy()
+        self._check(self.lib.dh_download_state(
+            self.ctx, _ptr(out.prim), _ptr(out.cons), _ptr(out.piph), _ptr(out.dphi),
+            _ptr(out.wiph), _ptr(out.phi_flux) if self.nf else None))
+        return out
+
+    def set_planets(self, substep: int, planets: np.ndarray):
+        pl = np.ascontiguousarray(planets, dtype=np.float64).reshape(-1, PLANET_STRIDE)
+        self._check(self.lib.dh_set_planets(self.ctx, substep, _ptr(pl), pl.shape[0]))
+
+    def getmindt(self) -> float:
+        v = C.c_double()
+        self._check(self.lib.dh_getmindt(self.ctx, C.byref(v)))
+        return v.value
+
+    def timestep(self, dt: float):
+        self._check(self.lib.dh_timestep(self.ctx, dt))
+
+    def sync(self):
+        self._check(self.lib.dh_sync(self.ctx))
+
+    def enable_timers(self, on: bool = True):
+        self._check(self.lib.dh_enable_timers(self.ctx, 1 if on else 0))
+
+    def read_timers(self):
+        cap = 64
+        names = (C.c_char * 48 * cap)()
+        ms = (C.c_double * cap)()
+        calls = (C.c_long * cap)()
+        n = C.c_int()
+        self._check(self.lib.dh_read_timers(self.ctx, cap, names, ms, calls, C.byref(n)))
+        return [(bytes(names[i]).split(b"\0")[0].decode(), ms[i], calls[i]) for i in range(n.value)]
+
+    def close(self):
+        if self.ctx:
+            self.lib.dh_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

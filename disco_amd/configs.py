@@ -1,0 +1,96 @@
+"""Module selections and in.par presets of the supported workloads.
+
+A Disco configuration is a link-time module selection (the reference's
+Makefile_opt.in: HYDRO, BOUNDARY, INITIAL, PLANETS, NUM_C, NUM_N, CT_MODE)
+plus an in.par file.  `Config` holds both; `dh_bc` says how each side of the
+mesh is treated on the device (see DH_BC_* in include/disco_hip.h).
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import Dict, Optional, Tuple
+
+# DH_BC_* (include/disco_hip.h)
+BC_NONE, BC_HELD, BC_ZEROGRAD, BC_DIODE = 0, 1, 2, 3
+HYDRO_EULER, HYDRO_MHD = 0, 1
+
+
+@dataclasses.dataclass(frozen=True)
+class Config:
+    name: str
+    hydro: str                 # "euler" | "mhd"
+    initial: str               # plugin name (disco_amd/plugins/Initial or reference Initial/)
+    boundary: str              # "polar" | "fixed" | "outflow3d" | "fixed_r_outflow_z"
+    planets: str               # "none" | "sun" | ...
+    num_c: int
+    num_n: int
+    ct_mode: int
+    par: Tuple[Tuple[str, object], ...]
+
+    def par_dict(self, **overrides) -> Dict[str, object]:
+        d = dict(self.par)
+        d.update(overrides)
+        return d
+
+    @property
+    def nq(self) -> int:
+        return self.num_c + self.num_n
+
+    @property
+    def nf(self) -> int:
+        return {0: 0, 1: 3, 2: 5}[self.ct_mode]
+
+    def dh_bc(self, par: Dict[str, object]) -> Tuple[int, int, int, int]:
+        """Per-side device boundary kind (rmin, rmax, zmin, zmax)."""
+        nobc = [par.get("NoBC_Rmin", 0), par.get("NoBC_Rmax", 0),
+                par.get("NoBC_Zmin", 0), par.get("NoBC_Zmax", 0)]
+        if self.boundary in ("polar", "fixed"):
+            sides = [BC_HELD] * 4
+        elif self.boundary == "outflow3d":
+            sides = [BC_DIODE] * 4
+        elif self.boundary == "zerograd":
+            sides = [BC_ZEROGRAD] * 4
+        elif self.boundary == "fixed_r_outflow_z":
+            sides = [BC_HELD, BC_HELD, BC_DIODE, BC_DIODE]
+        else:
+            raise ValueError(f"no device boundary for module {self.boundary!r}")
+        if int(par.get("Num_Z", 1)) == 1:
+            nobc[2] = nobc[3] = 1
+        return tuple(BC_NONE if nobc[i] else sides[i] for i in range(4))
+
+
+def _cfg(name, hydro, initial, boundary, planets, num_c, num_n, ct_mode, **par):
+    return Config(name, hydro, initial, boundary, planets, num_c, num_n, ct_mode,
+                  tuple(par.items()))
+
+
+CONFIGS: Dict[str, Config] = {}
+
+# 2-D Newtonian Euler isentropic vortex: r in [0,1] with the polar axis,
+# Nr=128 (51,809 cells), gamma=5/3, eps=5, PLM 1.5, HLLC, RK2, CFL 0.5,
+# rigid mesh rotation (Mesh_Motion 4, Exact_Mesh_Omega 1).
+CONFIGS["euler_yee"] = _cfg(
+    "euler_yee", "euler", "yee_vortex", "polar", "none", 5, 0, 0,
+    Num_R=128, Num_Z=1, R_Min=0.0, R_Max=1.0, Z_Min=-0.5, Z_Max=0.5,
+    NoBC_Rmin=1, CFL=0.5, PLM=1.5, Riemann_Solver=1, Mesh_Motion=4,
+    Exact_Mesh_Omega=1, Adiabatic_Index=5.0 / 3.0, Init_Par1=5.0)
+
+CONFIGS["euler_yee_n1"] = dataclasses.replace(
+    CONFIGS["euler_yee"], name="euler_yee_n1", num_n=1)
+
+# The reference's own Gaussian vortex (Initial/vortex.c), HLLC, polar axis.
+CONFIGS["euler_vortex"] = _cfg(
+    "euler_vortex", "euler", "vortex", "polar", "none", 5, 0, 0,
+    Num_R=32, Num_Z=1, R_Min=0.0, R_Max=3.0, Z_Min=-0.5, Z_Max=0.5,
+    NoBC_Rmin=1, CFL=0.5, PLM=1.5, Riemann_Solver=1, Mesh_Motion=1,
+    Adiabatic_Index=5.0 / 3.0)
+
+
+def get(name: str) -> Config:
+    return CONFIGS[name]
+
+
+def ref_binary(cfg: Config, repo: Optional[str] = None) -> str:
+    import os
+    repo = repo or os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    return os.path.join(repo, "oracle", "_ref", f"disco_ref_{cfg.name}")

@@ -1,0 +1,156 @@
+/*
+ * disco_hip.h -- C ABI of the MI355X timestep library (libdisco_hip.so).
+ *
+ * The library advances a Disco mesh (ragged annuli of cells, cylindrical
+ * r/phi/z) by full RK2 timesteps on one GPU.  Every entry point takes plain
+ * pointers and sizes; no C++ or framework types cross this boundary.
+ *
+ * Correspondence with the reference (HangChelseaSu/Disco):
+ *   dh_timestep   replaces  void timestep(struct domain*, double dt)
+ *                           (timestep.c:7-45: RK snapshot + onestep x2,
+ *                            onestep.c:84-228)
+ *   dh_getmindt   replaces  double getmindt(struct domain*)   (misc.c:39-84)
+ *   dh_create / dh_upload_state / dh_download_state
+ *                 replace the state ownership of struct domain
+ *                 (theCells[jk][i], paul.h:134-203; domain.c:26-127): the
+ *                 device holds the state, the host struct is a mirror that is
+ *                 filled on demand.
+ *   dh_set_planets  passes the planet list that planet_src reads
+ *                 (planet.c:152-196) for each RK substep.
+ * The reference-signature wrappers `timestep` and `getmindt` that a Disco
+ * driver links against are in disco_amd/host/dh_timestep.c (disco_host.h).
+ *
+ * Layout of state arrays (all double precision, structure of arrays):
+ *   cell c = off[jk] + i, with jk = j + Nr*k the annulus index (ghost annuli
+ *   included, as theCells[jk] of the reference), i = 0..Np[jk]-1 and
+ *   off[jk] = sum_{m<jk} Np[m].  Ntot = off[Nr*Nz].
+ *   prim[q*Ntot + c], cons[q*Ntot + c]        q < NQ = num_c + num_n
+ *   piph[c], dphi[c], wiph[c]
+ *   Phi[f*Ntot + c]                           f < NF (0, 3 or 5 by ct_mode)
+ *
+ * Every function returns 0 on success and a nonzero DH_E* code otherwise;
+ * dh_last_error() returns a message for the last failure on this thread.
+ */
+#ifndef DISCO_HIP_H
+#define DISCO_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+   DH_OK = 0,
+   DH_EINVAL = 1,      /* bad argument / unsupported configuration */
+   DH_EHIP = 2,        /* HIP runtime error */
+   DH_ENOMEM = 3,      /* device allocation failed */
+   DH_ENODEV = 4       /* no GPU visible */
+};
+
+/* Hydro modules (reference Hydro/<name>.c). */
+enum { DH_HYDRO_EULER = 0, DH_HYDRO_MHD = 1 };
+
+/* Boundary treatment of one side of the mesh, applied to ghost annuli after
+ * every substep (reference onestep.c:194-210 -> Boundary/<name>.c).
+ *   DH_BC_HELD:    ghost prims are left as uploaded.  This is exactly the
+ *                  reference's fixed/polar boundaries (Boundary/fixed.c:24-60,
+ *                  Boundary/polar.c:26-75) when initial() does not depend on
+ *                  phi, which the host checks before choosing it.
+ *   DH_BC_ZEROGRAD / DH_BC_DIODE: the reference's zero-gradient copy from the
+ *                  adjacent annulus, optionally with the outflow diode
+ *                  (boundary_functions.c:269-441, 90-159).
+ *   DH_BC_NONE:    the side has no ghost annuli (NoBC_* = 1). */
+enum { DH_BC_NONE = 0, DH_BC_HELD = 1, DH_BC_ZEROGRAD = 2, DH_BC_DIODE = 3 };
+
+typedef struct dh_grid {
+   int Nr, Nz;                  /* annuli including ghosts (paul.h:142) */
+   int NgRa, NgRb, NgZa, NgZb;  /* ghost annuli on each side */
+   const int *Np;               /* [Nr*Nz] cells per annulus */
+   const double *r_jph;         /* [Nr+1]: radial edges, r_jph[0] is edge -1 */
+   const double *z_kph;         /* [Nz+1]: vertical edges, z_kph[0] is edge -1 */
+   double phi_max;              /* azimuthal period (2 pi for full disks) */
+} dh_grid;
+
+/* Run-time parameters: the in.par keys (readpar.c:70-155) and the compile-time
+ * sizes (NUM_C, NUM_N, CT_MODE; paul.h:29-49) that the timestep reads. */
+typedef struct dh_params {
+   int num_c, num_n, ct_mode;   /* NUM_C, NUM_N, CT_MODE */
+   int hydro;                   /* DH_HYDRO_* */
+   int riemann_solver;          /* Riemann_Solver: 0 HLL, 1 HLLC, 2 HLLD */
+   int mesh_motion;             /* Mesh_Motion: 0 fixed, 1 by cell, 3 avg, 4 exact */
+   int exact_mesh_omega;        /* Exact_Mesh_Omega (omega.c:43-60) */
+   int energy_omega;            /* Energy_Omega (omega.c:62-114) */
+   int timestep_mode;           /* Timestep: 0 RK2, 1 forward Euler */
+   int ct;                      /* Constrained_Transport */
+   int isothermal;              /* Isothermal */
+   int cs2_profile;             /* Cs2_Profile (omega.c:116-140) */
+   int visc_flag, alpha_flag;   /* Use_Viscosity, Use_As_Alpha */
+   int rot_frame;               /* RotFrame */
+   int grav2d;                  /* Grav2D */
+   int nobc_rmin, nobc_rmax, nobc_zmin, nobc_zmax;
+   int r_periodic, z_periodic;
+   int bc_rmin, bc_rmax, bc_zmin, bc_zmax;  /* DH_BC_* per side */
+   int nozzle_type;             /* Nozzle_Type (only 0 is supported) */
+   int pad_[3];
+   double gamma_law;            /* Adiabatic_Index */
+   double plm;                  /* PLM */
+   double cfl;                  /* CFL */
+   double max_dt;               /* Max_DT */
+   double rho_floor, p_floor;   /* Density_Floor, Pressure_Floor */
+   double viscosity;            /* Viscosity */
+   double mach;                 /* Mach_Number */
+   double metric_par2;          /* Metric_Par2 (M in the Cs2 profiles) */
+   double rot_omega, rot_d;     /* RotOmega, RotD */
+   double init_par[9];          /* Init_Par0..8 (Init_Par0 as a double) */
+} dh_params;
+
+/* Planet record passed to dh_set_planets: 8 doubles per planet,
+ * {r, phi, M, omega, vr, eps, type, 0} (struct planet, paul.h:233-251). */
+#define DH_PLANET_STRIDE 8
+#define DH_MAX_PLANETS 8
+
+typedef struct dh_ctx dh_ctx;
+
+/* Allocates the device state for one mesh and builds the static tables
+ * (annulus offsets, face-list offsets).  The grid arrays are copied. */
+int dh_create(const dh_grid *grid, const dh_params *par, dh_ctx **out);
+int dh_destroy(dh_ctx *ctx);
+
+/* Number of cells (Ntot) and of conserved/face fields of a context. */
+int dh_sizes(const dh_ctx *ctx, long *ntot, int *nq, int *nf);
+
+/* Host <-> device copies of the whole state (layout above).  Any pointer may
+ * be NULL to skip that field.  dphi is recomputed from piph on upload when
+ * the pointer is NULL. */
+int dh_upload_state(dh_ctx *ctx, const double *prim, const double *cons,
+                    const double *piph, const double *dphi, const double *wiph,
+                    const double *Phi);
+int dh_download_state(dh_ctx *ctx, double *prim, double *cons, double *piph,
+                      double *dphi, double *wiph, double *Phi);
+
+/* Planets seen by the sources of RK substep `substep` (0 or 1) of the next
+ * dh_timestep.  npl <= DH_MAX_PLANETS. */
+int dh_set_planets(dh_ctx *ctx, int substep, const double *planets, int npl);
+
+/* CFL timestep: min over real cells of Hydro mindt, times CFL (misc.c:39-84). */
+int dh_getmindt(dh_ctx *ctx, double *dt);
+
+/* One full timestep (RK2 or forward Euler per timestep_mode). Asynchronous
+ * with respect to the host; dh_getmindt / dh_download_state synchronise. */
+int dh_timestep(dh_ctx *ctx, double dt);
+
+/* Waits for all work queued on the context's stream. */
+int dh_sync(dh_ctx *ctx);
+
+/* Average device time (ms) of the kernels of the last `dh_timestep` calls,
+ * accumulated per kernel family since dh_reset_timers; used by the benchmark
+ * to report the dominant kernel.  names/ms/calls receive up to `cap` rows. */
+int dh_enable_timers(dh_ctx *ctx, int on);
+int dh_read_timers(dh_ctx *ctx, int cap, char (*names)[48], double *ms,
+                   long *calls, int *nrows);
+
+const char *dh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DISCO_HIP_H */

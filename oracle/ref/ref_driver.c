@@ -1,0 +1,168 @@
+/*
+ * Test harness for the reference build (oracle/_ref): replaces the reference's
+ * main() (main.c:26-97) so that the reference timestep can be run for a fixed
+ * number of steps and its full state dumped without HDF5.
+ *
+ * This file is test infrastructure.  It is compiled together with the
+ * reference's own sources (paul.h layout) by oracle/ref/Makefile, and is never
+ * part of the product path.
+ *
+ * Flow (mirrors main.c:35-82 without output/report):
+ *   read_par_file -> mpiSetup -> setupGrid -> setupDomain -> setupCells
+ *   -> initial exchange/boundary_trans (main.c:53-62)
+ *   -> dump state_0000.bin
+ *   -> N x { dt = getmindt ; timestep(dt) }   (main.c:69-82; check_dt is
+ *      skipped: T_End is never reached in these runs)
+ *   -> dump state_final.bin, dt.txt
+ *
+ * Usage: disco_ref_<cfg> <outdir> <nsteps> [dump_every]
+ *   in.par is read from the current directory (readpar.c:60).
+ */
+#include "paul.h"
+#include "profiler.h"
+#include <sys/time.h>
+
+int mpiSetup(struct domain *, int, char *[]);
+void setupGrid(struct domain *);
+void setupDomain(struct domain *);
+void setupCells(struct domain *);
+void freeDomain(struct domain *);
+void timestep(struct domain *, double);
+double getmindt(struct domain *);
+void boundary_trans(struct domain *, int);
+void exchangeData(struct domain *, int);
+int read_par_file(struct domain *);
+
+/* Raw state dump, layout "DSTATE01" (shared with disco_amd/state_io.py):
+ *   char magic[8]; int32 hdr[16]; double dhdr[8];
+ *   int32 Np[Nr*Nz]; double r_jph[Nr+1]; double z_kph[Nz+1];
+ *   double prim[NQ][Ntot]; double cons[NQ][Ntot];
+ *   double piph[Ntot]; double dphi[Ntot]; double wiph[Ntot];
+ *   double Phi[NF][Ntot]; double planets[Npl][8]
+ * Cells are ordered annulus by annulus (jk = j + Nr*k), i = 0..Np[jk]-1. */
+static void dump_state(struct domain *d, const char *path, int step)
+{
+   int Nr = d->Nr, Nz = d->Nz;
+   int *Np = d->Np;
+   int Ntot = 0, jk, i, q;
+   for (jk = 0; jk < Nr * Nz; ++jk) Ntot += Np[jk];
+
+   FILE *f = fopen(path, "wb");
+   if (!f) { perror(path); exit(2); }
+   char magic[8] = {'D', 'S', 'T', 'A', 'T', 'E', '0', '1'};
+   int hdr[16] = {0};
+   hdr[0] = NUM_Q; hdr[1] = NUM_FACES; hdr[2] = Nr; hdr[3] = Nz;
+   hdr[4] = d->NgRa; hdr[5] = d->NgRb; hdr[6] = d->NgZa; hdr[7] = d->NgZb;
+   hdr[8] = Ntot; hdr[9] = d->Npl; hdr[10] = step; hdr[11] = NUM_C;
+   double dhdr[8] = {0};
+   dhdr[0] = d->t; dhdr[1] = d->phi_max;
+   fwrite(magic, 1, 8, f);
+   fwrite(hdr, sizeof(int), 16, f);
+   fwrite(dhdr, sizeof(double), 8, f);
+   fwrite(Np, sizeof(int), Nr * Nz, f);
+   fwrite(d->r_jph - 1, sizeof(double), Nr + 1, f);
+   fwrite(d->z_kph - 1, sizeof(double), Nz + 1, f);
+
+   double *buf = (double *)malloc(sizeof(double) * (size_t)Ntot);
+#define DUMP_FIELD(expr)                                         \
+   do {                                                          \
+      int n_ = 0;                                                \
+      for (jk = 0; jk < Nr * Nz; ++jk)                           \
+         for (i = 0; i < Np[jk]; ++i) {                          \
+            struct cell *c = &(d->theCells[jk][i]);              \
+            buf[n_++] = (expr);                                  \
+         }                                                       \
+      fwrite(buf, sizeof(double), Ntot, f);                      \
+   } while (0)
+   for (q = 0; q < NUM_Q; ++q) DUMP_FIELD(c->prim[q]);
+   for (q = 0; q < NUM_Q; ++q) DUMP_FIELD(c->cons[q]);
+   DUMP_FIELD(c->piph);
+   DUMP_FIELD(c->dphi);
+   DUMP_FIELD(c->wiph);
+#if NUM_FACES > 0
+   for (q = 0; q < NUM_FACES; ++q) DUMP_FIELD(c->Phi[q]);
+#endif
+#undef DUMP_FIELD
+   free(buf);
+   int p;
+   for (p = 0; p < d->Npl; ++p) {
+      struct planet *pl = d->thePlanets + p;
+      double pv[8] = {pl->r, pl->phi, pl->M, pl->omega, pl->vr, pl->eps,
+                      (double)pl->type, 0.0};
+      fwrite(pv, sizeof(double), 8, f);
+   }
+   fclose(f);
+}
+
+static double wall(void)
+{
+   struct timeval tv;
+   gettimeofday(&tv, NULL);
+   return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+int main(int argc, char *argv[])
+{
+   if (argc < 3) {
+      fprintf(stderr, "usage: %s outdir nsteps [dump_every]\n", argv[0]);
+      return 1;
+   }
+   const char *outdir = argv[1];
+   int nsteps = atoi(argv[2]);
+   int dump_every = argc > 3 ? atoi(argv[3]) : 0;
+
+   struct domain theDomain = {0};
+   struct profiler prof;
+   theDomain.prof = &prof;
+   start_clock(&theDomain);
+   read_par_file(&theDomain);
+   if (mpiSetup(&theDomain, argc, argv) == 1) return 0;
+   setupGrid(&theDomain);
+   setupDomain(&theDomain);
+   setupCells(&theDomain);
+
+   if (theDomain.Nr > 1) {
+      exchangeData(&theDomain, 0);
+      if (!theDomain.theParList.R_Periodic) boundary_trans(&theDomain, 1);
+   }
+   if (theDomain.Nz > 1) {
+      exchangeData(&theDomain, 1);
+      if (!theDomain.theParList.Z_Periodic) boundary_trans(&theDomain, 2);
+   }
+
+   char path[1024];
+   snprintf(path, sizeof path, "%s/state_0000.bin", outdir);
+   dump_state(&theDomain, path, 0);
+
+   snprintf(path, sizeof path, "%s/dt.txt", outdir);
+   FILE *fdt = fopen(path, "w");
+   double t0 = wall();
+   int n;
+   for (n = 0; n < nsteps; ++n) {
+      double dt = getmindt(&theDomain);
+      fprintf(fdt, "%.17g\n", dt);
+      timestep(&theDomain, dt);
+      if (dump_every > 0 && (n + 1) % dump_every == 0 && n + 1 < nsteps) {
+         snprintf(path, sizeof path, "%s/state_%04d.bin", outdir, n + 1);
+         dump_state(&theDomain, path, n + 1);
+      }
+   }
+   double t1 = wall();
+   fclose(fdt);
+
+   snprintf(path, sizeof path, "%s/state_final.bin", outdir);
+   dump_state(&theDomain, path, nsteps);
+
+   /* Real cells, as profiler.c:12-37 counts them. */
+   long ncells = 0;
+   int j, k;
+   for (k = theDomain.NgZa; k < theDomain.Nz - theDomain.NgZb; ++k)
+      for (j = theDomain.NgRa; j < theDomain.Nr - theDomain.NgRb; ++j)
+         ncells += theDomain.Np[j + theDomain.Nr * k];
+   double secs = t1 - t0;
+   printf("{\"cells\": %ld, \"steps\": %d, \"seconds\": %.6f, "
+          "\"zone_updates_per_s\": %.6e}\n",
+          ncells, nsteps, secs, secs > 0 ? (double)ncells * nsteps / secs : 0.0);
+   freeDomain(&theDomain);
+   return 0;
+}
