@@ -1,0 +1,105 @@
+"""Builds the native parts in-tree.
+
+    python -m disco_amd.build            # libdisco_hip.so (+ host library)
+    python -m disco_amd.build --all      # also the oracle and, when the
+                                         # reference sources are mounted,
+                                         # oracle/_ref
+
+libdisco_hip.so is compiled for gfx950 only, with packed-FP32 instructions
+disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(REPO, "include")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+REFERENCE = "/root/reference"
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+
+def _run(cmd, **kw):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, **kw)
+
+
+def build_hip(extra=()) -> str:
+    out = os.path.join(HERE, "libdisco_hip.so")
+    tmp = out + ".tmp"
+    srcs = [os.path.join(CSRC, "dh_engine.hip")]
+    _run([HIPCC, *HIP_FLAGS, *extra, f"-I{INCLUDE}", f"-I{CSRC}", *srcs, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+# Module selections that have host plugins in disco_amd/plugins: one host
+# library and one `disco` driver per selection, as the reference links one
+# binary per Makefile_opt.in.
+HOST_CONFIGS = {
+    "euler_yee": dict(initial="yee_vortex", planets="none", boundary="polar",
+                      num_c=5, num_n=0, ct_mode=0),
+    "euler_yee_n1": dict(initial="yee_vortex", planets="none", boundary="polar",
+                         num_c=5, num_n=1, ct_mode=0),
+}
+
+
+def build_host(configs=None) -> None:
+    """libdisco_host_<cfg>.so and disco_<cfg> (host C + plugins, linked to
+    libdisco_hip.so)."""
+    host = os.path.join(HERE, "host")
+    plug = os.path.join(HERE, "plugins")
+    for name, c in HOST_CONFIGS.items():
+        if configs and name not in configs:
+            continue
+        defs = [f"-DNUM_C={c['num_c']}", f"-DNUM_N={c['num_n']}", f"-DCT_MODE={c['ct_mode']}",
+                f'-DBOUNDARY="{c["boundary"]}"', f'-DINITIAL="{c["initial"]}"']
+        srcs = [os.path.join(host, "disco_host.c"),
+                os.path.join(plug, "Initial", c["initial"] + ".c"),
+                os.path.join(plug, "Planets", c["planets"] + ".c")]
+        common = ["gcc", "-O2", "-fPIC", "-Wall", "-Wno-unused-parameter", *defs,
+                  f"-I{os.path.join(INCLUDE, 'disco')}", f"-I{INCLUDE}", f"-I{host}"]
+        link = [f"-L{HERE}", "-ldisco_hip", "-lm", "-Wl,-rpath,$ORIGIN/.."]
+        _run(common + ["-shared", *srcs, "-o", os.path.join(host, f"libdisco_host_{name}.so"),
+                       *link])
+        _run(common + [os.path.join(host, "main.c"), *srcs, "-o",
+                       os.path.join(host, f"disco_{name}"), *link])
+
+
+def build_oracle() -> None:
+    _run(["make", "-s", "-f", os.path.join(REPO, "oracle", "Makefile")])
+
+
+def build_ref() -> bool:
+    if not os.path.isdir(REFERENCE):
+        print("reference sources not mounted: oracle/_ref not rebuilt", flush=True)
+        return False
+    _run(["make", "-s", "-j4", "-f", os.path.join(REPO, "oracle", "ref", "Makefile"),
+          f"REF={REFERENCE}"])
+    return True
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--all", action="store_true")
+    args = ap.parse_args(argv)
+    build_hip()
+    build_host()
+    if args.all:
+        build_oracle()
+        build_ref()
+
+
+if __name__ == "__main__":
+    main()

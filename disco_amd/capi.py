@@ -119,6 +119,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "dh_sizes": ([vp, C.POINTER(C.c_long), _ip, _ip], C.c_int),
         "dh_upload_state": ([vp, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int),
         "dh_download_state": ([vp, _dp, _dp, _dp, _dp, _dp, _dp], C.c_int),
+        "dh_init_cells": ([vp], C.c_int),
         "dh_set_planets": ([vp, C.c_int, _dp, C.c_int], C.c_int),
         "dh_getmindt": ([vp, _dp], C.c_int),
         "dh_timestep": ([vp, C.c_double], C.c_int),
@@ -135,7 +136,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
 
 
 EXPORTED_SYMBOLS = ["dh_create", "dh_destroy", "dh_sizes", "dh_upload_state",
-                    "dh_download_state", "dh_set_planets", "dh_getmindt", "dh_timestep",
+                    "dh_download_state", "dh_init_cells", "dh_set_planets", "dh_getmindt", "dh_timestep",
                     "dh_sync", "dh_enable_timers", "dh_read_timers", "dh_last_error"]
 
 _LIB: Optional[C.CDLL] = None
@@ -190,6 +191,9 @@ class DiscoHip:
     def set_planets(self, substep: int, planets: np.ndarray):
         pl = np.ascontiguousarray(planets, dtype=np.float64).reshape(-1, PLANET_STRIDE)
         self._check(self.lib.dh_set_planets(self.ctx, substep, _ptr(pl), pl.shape[0]))
+
+    def init_cells(self):
+        self._check(self.lib.dh_init_cells(self.ctx))
 
     def getmindt(self) -> float:
         v = C.c_double()

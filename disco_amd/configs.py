@@ -86,6 +86,20 @@ CONFIGS["euler_vortex"] = _cfg(
     Adiabatic_Index=5.0 / 3.0)
 
 
+# 3-D Newtonian MHD Keplerian disk (MRI seed): r in [0.5,2], z in [-0.25,0.25],
+# Nr=256, Nz=64 (22,004,544 cells), rho=1, cs = v_K/20 (isothermal), vertical
+# field at beta=1000, velocity noise 1e-3 cs (Noise_Type 3), HLLD, PLM 1.5,
+# constrained transport, Keplerian mesh motion and energy frame.
+CONFIGS["mhd_kep"] = _cfg(
+    "mhd_kep", "mhd", "kep_mhd_disk", "fixed_r_outflow_z", "central", 8, 0, 2,
+    Num_R=256, Num_Z=64, R_Min=0.5, R_Max=2.0, Z_Min=-0.25, Z_Max=0.25,
+    NoBC_Rmin=0, NoBC_Rmax=0, NoBC_Zmin=0, NoBC_Zmax=0, CFL=0.5, PLM=1.5,
+    Riemann_Solver=2, Mesh_Motion=4, Exact_Mesh_Omega=2, Energy_Omega=2,
+    Constrained_Transport=1, Isothermal=1, Cs2_Profile=4, Mach_Number=20.0,
+    Metric_Par2=1.0, Grav2D=1, Adiabatic_Index=5.0 / 3.0, Init_Par1=1000.0,
+    Noise_Type=3, Noise_Rel=5e-5, Noise_Abs=0.0)
+
+
 def get(name: str) -> Config:
     return CONFIGS[name]
 

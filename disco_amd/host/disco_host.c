@@ -1,0 +1,691 @@
+/*
+ * disco_host.c -- host side of the drop-in: in.par reading, grid and domain
+ * set-up, and the reference-signature entry points
+ *
+ *     double getmindt(struct domain *);          (misc.c:39-84)
+ *     void   timestep(struct domain *, double);  (timestep.c:7-45)
+ *
+ * that a Disco driver calls.  The state lives on the GPU (libdisco_hip.so,
+ * include/disco_hip.h); theCells is a host mirror that disco_sync_to_host()
+ * fills.  User plugins are linked in exactly as in the reference: INITIAL
+ * (setICparams, initial), PLANETS (setPlanetParams, planet_motion_analytic,
+ * initializePlanets, movePlanets) -- see include/disco/paul.h.
+ *
+ * The boundary module is named at compile time (-DBOUNDARY="polar", as the
+ * reference's Makefile passes it) and mapped to a device boundary kind; sides
+ * whose ghost values come from initial() are held on the device after the
+ * host has checked that initial() does not depend on phi there.
+ */
+#include "disco_host.h"
+
+#include <stdarg.h>
+
+#include "disco_hip.h"
+
+#ifndef BOUNDARY
+#define BOUNDARY "polar"
+#endif
+
+/* Plugin symbols (reference plugin ABI). */
+void setICparams(struct domain *);
+void initial(double *prim, double *x);
+void setPlanetParams(struct domain *);
+int planet_motion_analytic(void);
+void initializePlanets(struct planet *);
+void movePlanets(struct planet *, double, double);
+
+struct disco_accel {
+   dh_ctx *ctx;
+   long ntot;
+   long *off;     /* [Nr*Nz+1] */
+   int bc[4];     /* DH_BC_* per side */
+   int host_dirty; /* mirror older than the device state */
+};
+
+static char g_host_err[512];
+
+static int host_fail(const char *fmt, ...)
+{
+   va_list ap;
+   va_start(ap, fmt);
+   vsnprintf(g_host_err, sizeof g_host_err, fmt, ap);
+   va_end(ap);
+   fprintf(stderr, "disco: %s\n", g_host_err);
+   return 1;
+}
+
+const char *disco_last_error(void) { return g_host_err; }
+
+/* ------------------------------------------------------------------------ */
+/* in.par (readpar.c:6-188): "key value" lines; the value starts after the    */
+/* key and any of "\t :=>_"; integers are parsed as doubles then truncated.   */
+/* A missing key is reported and leaves the field unchanged.                  */
+/* ------------------------------------------------------------------------ */
+
+enum { V_INT, V_DBL };
+
+static int par_lookup(const char *file, const char *key, int type, void *out)
+{
+   FILE *f = fopen(file, "r");
+   if (!f) return 1;
+   char line[512], name[512], hit[512];
+   int found = 0;
+   while (!found && fgets(line, sizeof line, f)) {
+      name[0] = 0;
+      if (sscanf(line, "%511s", name) == 1 && strcmp(name, key) == 0) {
+         strcpy(hit, line);
+         found = 1;
+      }
+   }
+   fclose(f);
+   if (!found) return 1;
+   const char *v = hit + strlen(key);
+   v += strspn(v, "\t :=>_");
+   double d = 0.0;
+   if (sscanf(v, "%lf", &d) != 1) return 1;
+   if (type == V_INT) *(int *)out = (int)d;
+   else *(double *)out = d;
+   return 0;
+}
+
+int disco_read_par(struct domain *d, const char *file)
+{
+   struct param_list *L = &d->theParList;
+   int missing = 0;
+   int t2pi = 0, p2pi = 0, zpi = 0;
+   d->rank = 0;
+   d->size = 1;
+#define RD(key, type, ptr) missing += par_lookup(file, key, type, ptr)
+   RD("Num_R", V_INT, &L->Num_R);
+   RD("Num_Z", V_INT, &L->Num_Z);
+   RD("aspect", V_DBL, &L->aspect);
+   RD("Num_Reports", V_INT, &L->NumRepts);
+   RD("Num_Snapshots", V_INT, &L->NumSnaps);
+   RD("Num_Checkpoints", V_INT, &L->NumChecks);
+   RD("T_Start", V_DBL, &L->t_min);
+   RD("T_End", V_DBL, &L->t_max);
+   RD("R_Min", V_DBL, &L->rmin);
+   RD("R_Max", V_DBL, &L->rmax);
+   RD("Z_Min", V_DBL, &L->zmin);
+   RD("Z_Max", V_DBL, &L->zmax);
+   RD("Phi_Max", V_DBL, &L->phimax);
+   RD("Use_Logtime", V_INT, &L->Out_LogTime);
+   RD("Log_Zoning", V_INT, &L->LogZoning);
+   RD("R_Periodic", V_INT, &L->R_Periodic);
+   RD("Z_Periodic", V_INT, &L->Z_Periodic);
+   RD("NoBC_Rmin", V_INT, &L->NoBC_Rmin);
+   RD("NoBC_Rmax", V_INT, &L->NoBC_Rmax);
+   RD("NoBC_Zmin", V_INT, &L->NoBC_Zmin);
+   RD("NoBC_Zmax", V_INT, &L->NoBC_Zmax);
+   RD("Log_Radius", V_DBL, &L->LogRadius);
+   RD("Max_Aspect_Short", V_DBL, &L->MaxShort);
+   RD("Max_Aspect_Long", V_DBL, &L->MaxLong);
+   RD("Timestep", V_INT, &L->Timestep);
+   RD("CFL", V_DBL, &L->CFL);
+   RD("Max_DT", V_DBL, &L->maxDT);
+   RD("PLM", V_DBL, &L->PLM);
+   RD("Adiabatic_Index", V_DBL, &L->Adiabatic_Index);
+   RD("Isothermal", V_INT, &L->isothermal_flag);
+   RD("Cs2_Profile", V_INT, &L->Cs2_Profile);
+   RD("Cs2_Par", V_DBL, &L->Cs2_Par);
+   RD("Density_Floor", V_DBL, &L->Density_Floor);
+   RD("Pressure_Floor", V_DBL, &L->Pressure_Floor);
+   RD("Mesh_Motion", V_INT, &L->Mesh_Motion);
+   RD("Exact_Mesh_Omega", V_INT, &L->Exact_Mesh_Omega);
+   RD("Exact_Mesh_Omega_Par", V_DBL, &L->Exact_Mesh_Omega_Par);
+   RD("Energy_Omega", V_INT, &L->Energy_Omega);
+   RD("Energy_Omega_Par", V_DBL, &L->Energy_Omega_Par);
+   RD("RotFrame", V_INT, &L->RotFrame);
+   RD("RotOmega", V_DBL, &L->RotOmega);
+   RD("RotD", V_DBL, &L->RotD);
+   RD("Riemann_Solver", V_INT, &L->Riemann_Solver);
+   RD("Initial_Regrid", V_INT, &L->Initial_Regrid);
+   RD("Restart", V_INT, &L->restart_flag);
+   RD("Use_Viscosity", V_INT, &L->visc_flag);
+   RD("Viscosity", V_DBL, &L->viscosity);
+   RD("Use_As_Alpha", V_INT, &L->alpha_flag);
+   RD("Include_Atmos", V_INT, &L->include_atmos);
+   RD("T_Times_2pi", V_INT, &t2pi);
+   RD("P_Times_2pi", V_INT, &p2pi);
+   RD("Z_Times_pi", V_INT, &zpi);
+   RD("Mach_Number", V_DBL, &L->Disk_Mach);
+   RD("Mass_Ratio", V_DBL, &L->Mass_Ratio);
+   RD("Eccentricity", V_DBL, &L->Eccentricity);
+   RD("Drift_Rate", V_DBL, &L->Drift_Rate);
+   RD("Drift_Exp", V_DBL, &L->Drift_Exp);
+   RD("Grav2D", V_INT, &L->grav2D);
+   RD("Constrained_Transport", V_INT, &L->CT);
+   RD("Metric_Par0", V_INT, &L->metricPar0);
+   RD("Metric_Par1", V_DBL, &L->metricPar1);
+   RD("Metric_Par2", V_DBL, &L->metricPar2);
+   RD("Metric_Par3", V_DBL, &L->metricPar3);
+   RD("Metric_Par4", V_DBL, &L->metricPar4);
+   RD("Init_Par0", V_INT, &L->initPar0);
+   RD("Init_Par1", V_DBL, &L->initPar1);
+   RD("Init_Par2", V_DBL, &L->initPar2);
+   RD("Init_Par3", V_DBL, &L->initPar3);
+   RD("Init_Par4", V_DBL, &L->initPar4);
+   RD("Init_Par5", V_DBL, &L->initPar5);
+   RD("Init_Par6", V_DBL, &L->initPar6);
+   RD("Init_Par7", V_DBL, &L->initPar7);
+   RD("Init_Par8", V_DBL, &L->initPar8);
+   RD("Noise_Type", V_INT, &L->noiseType);
+   RD("Noise_Abs", V_DBL, &L->noiseAbs);
+   RD("Noise_Rel", V_DBL, &L->noiseRel);
+   RD("Sink_Type", V_INT, &L->sinkType);
+   RD("Sink_Par1", V_DBL, &L->sinkPar1);
+   RD("Sink_Par2", V_DBL, &L->sinkPar2);
+   RD("Sink_Par3", V_DBL, &L->sinkPar3);
+   RD("Sink_Par4", V_DBL, &L->sinkPar4);
+   RD("Nozzle_Type", V_INT, &L->nozzleType);
+   RD("Nozzle_Par1", V_DBL, &L->nozzlePar1);
+   RD("Nozzle_Par2", V_DBL, &L->nozzlePar2);
+   RD("Nozzle_Par3", V_DBL, &L->nozzlePar3);
+   RD("Nozzle_Par4", V_DBL, &L->nozzlePar4);
+#undef RD
+   if (t2pi) { L->t_min *= 2. * M_PI; L->t_max *= 2. * M_PI; }
+   if (p2pi) L->phimax *= 2. * M_PI;
+   if (zpi) { L->zmin *= M_PI; L->zmax *= M_PI; }
+   if (missing) fprintf(stderr, "disco: %d in.par keys missing (left at 0)\n", missing);
+   return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Grid (gridsetup.c:15-144, serial): annulus edges and Np per annulus.      */
+/* ------------------------------------------------------------------------ */
+
+static double centroid_r(double xp, double xm)
+{
+   return 2.0 * (xp * xp + xp * xm + xm * xm) / (3.0 * (xp + xm));
+}
+
+void disco_setup_grid(struct domain *d)
+{
+   struct param_list *L = &d->theParList;
+   int Ng = NUM_G;
+   d->Ng = Ng;
+   d->dim_rank[0] = d->dim_rank[1] = 0;
+   d->dim_size[0] = d->dim_size[1] = 1;
+   int Num_R = L->Num_R, Num_Z = L->Num_Z;
+   int NgRa = L->NoBC_Rmin ? 0 : Ng;
+   int NgRb = L->NoBC_Rmax ? 0 : Ng;
+   int NgZa = (Num_Z == 1 || L->NoBC_Zmin) ? 0 : Ng;
+   int NgZb = (Num_Z == 1 || L->NoBC_Zmax) ? 0 : Ng;
+   int N0r = -NgRa, N0z = -NgZa;
+   int Nr = Num_R + NgRa + NgRb, Nz = Num_Z + NgZa + NgZb;
+   d->Nr = Nr; d->Nz = Nz;
+   d->NgRa = NgRa; d->NgRb = NgRb; d->NgZa = NgZa; d->NgZb = NgZb;
+   d->N0r = N0r; d->N0z = N0z;
+   d->Nr_glob = Num_R + (L->NoBC_Rmin ? 0 : Ng) + (L->NoBC_Rmax ? 0 : Ng);
+   d->Nz_glob = Num_Z + (L->NoBC_Zmin ? 0 : Ng) + (L->NoBC_Zmax ? 0 : Ng);
+   d->N0r_glob = L->NoBC_Rmin ? 0 : -Ng;
+   d->N0z_glob = L->NoBC_Zmin ? 0 : -Ng;
+   d->Np = (int *)malloc(sizeof(int) * Nr * Nz);
+   d->r_jph = (double *)malloc(sizeof(double) * (Nr + 1)) + 1;
+   d->z_kph = (double *)malloc(sizeof(double) * (Nz + 1)) + 1;
+   for (int j = -1; j < Nr; ++j) {
+      double x = (N0r + j + 1) / (double)Num_R;
+      if (L->LogZoning == 0) d->r_jph[j] = L->rmin + x * (L->rmax - L->rmin);
+      else if (L->LogZoning == 1) d->r_jph[j] = L->rmin * pow(L->rmax / L->rmin, x);
+      else d->r_jph[j] = L->LogRadius * (pow(L->rmax / L->LogRadius, x) - 1) + L->rmin +
+                         (L->LogRadius - L->rmin) * x;
+   }
+   double dz = (L->zmax - L->zmin) / (double)Num_Z;
+   double z0 = L->zmin + (double)N0z * dz;
+   for (int k = -1; k < Nz; ++k) d->z_kph[k] = z0 + ((double)k + 1.) * dz;
+   d->phi_max = L->phimax;
+   for (int k = 0; k < Nz; ++k)
+      for (int j = 0; j < Nr; ++j) {
+         double dr = d->r_jph[j] - d->r_jph[j - 1];
+         double hp = d->r_jph[j];  /* get_scale_factor(xp, 0) */
+         double dp = L->aspect * dr / hp;
+         int np = (int)(d->phi_max / dp);
+         if (np < 4) np = 4;
+         d->Np[j + Nr * k] = np;
+      }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Domain and cells (domain.c:26-242)                                       */
+/* ------------------------------------------------------------------------ */
+
+static int bc_kind_of_module(const char *mod, int side)
+{
+   if (!strcmp(mod, "polar") || !strcmp(mod, "fixed")) return DH_BC_HELD;
+   if (!strcmp(mod, "outflow3d")) return DH_BC_DIODE;
+   if (!strcmp(mod, "zerograd")) return DH_BC_ZEROGRAD;
+   if (!strcmp(mod, "fixed_r_outflow_z")) return side < 2 ? DH_BC_HELD : DH_BC_DIODE;
+   return -1;
+}
+
+/* Value of initial() that the boundary module writes into ghost cell c of
+ * annulus (j,k) (Boundary/polar.c:26-75: centroids; Boundary/fixed.c:24-60
+ * and boundary_functions.c:21-31: midpoints / centroid array). */
+static void ghost_initial(struct domain *d, struct cell *c, int j, int k, double phi_shift,
+                          double *prim)
+{
+   double x[3];
+   double phi = c->piph - .5 * c->dphi + phi_shift;
+   if (!strcmp(BOUNDARY, "fixed")) {
+      x[0] = .5 * (d->r_jph[j] + d->r_jph[j - 1]);
+      x[2] = .5 * (d->z_kph[k] + d->z_kph[k - 1]);
+   } else {
+      x[0] = centroid_r(d->r_jph[j], d->r_jph[j - 1]);
+      x[2] = .5 * (d->z_kph[k] + d->z_kph[k - 1]);
+   }
+   x[1] = phi;
+   initial(prim, x);
+}
+
+static double noise_u(void) { return 2 * ((double)rand() / (double)(RAND_MAX)) - 1; }
+
+/* addNoise, noise.c:15-48 (types 1-3; draws rand() in the reference's order) */
+static void add_noise(struct domain *d, double *prim, const double *x)
+{
+   struct param_list *L = &d->theParList;
+   if (L->noiseType == 1) {
+      double x1 = noise_u(), x2 = noise_u(), x3 = noise_u(), x4 = noise_u();
+      prim[URR] = (1.0 + x1 * L->noiseRel) * prim[URR] + x2 * L->noiseAbs;
+      prim[UPP] = (1.0 + x3 * L->noiseRel) * prim[UPP] + x4 * L->noiseAbs;
+   } else if (L->noiseType == 2) {
+      double x1 = noise_u(), x2 = noise_u();
+      prim[PPP] = (1.0 + x1 * L->noiseRel) * prim[PPP] + x2 * L->noiseAbs;
+   } else if (L->noiseType == 3) {
+      double x1 = noise_u(), x2 = noise_u(), x3 = noise_u(), x4 = noise_u();
+      double vphi = prim[UPP] * x[0];
+      prim[URR] = x1 * L->noiseRel * vphi + x2 * L->noiseAbs;
+      prim[UZZ] = x3 * L->noiseRel * vphi + x4 * L->noiseAbs;
+   }
+}
+
+static void fill_params(struct domain *d, dh_params *P, const int *bc)
+{
+   struct param_list *L = &d->theParList;
+   memset(P, 0, sizeof *P);
+   P->num_c = NUM_C;
+   P->num_n = NUM_N;
+   P->ct_mode = CT_MODE;
+   P->hydro = NUM_C > 5 ? DH_HYDRO_MHD : DH_HYDRO_EULER;
+   P->riemann_solver = L->Riemann_Solver;
+   P->mesh_motion = L->Mesh_Motion;
+   P->exact_mesh_omega = L->Exact_Mesh_Omega;
+   P->energy_omega = L->Energy_Omega;
+   P->timestep_mode = L->Timestep;
+   P->ct = L->CT;
+   P->isothermal = L->isothermal_flag;
+   P->cs2_profile = L->Cs2_Profile;
+   P->visc_flag = L->visc_flag;
+   P->alpha_flag = L->alpha_flag;
+   P->rot_frame = L->RotFrame;
+   P->grav2d = L->grav2D;
+   P->nobc_rmin = L->NoBC_Rmin;
+   P->nobc_rmax = L->NoBC_Rmax;
+   P->nobc_zmin = L->NoBC_Zmin;
+   P->nobc_zmax = L->NoBC_Zmax;
+   P->r_periodic = L->R_Periodic;
+   P->z_periodic = L->Z_Periodic;
+   P->bc_rmin = bc[0];
+   P->bc_rmax = bc[1];
+   P->bc_zmin = bc[2];
+   P->bc_zmax = bc[3];
+   P->nozzle_type = L->nozzleType;
+   P->gamma_law = L->Adiabatic_Index;
+   P->plm = L->PLM;
+   P->cfl = L->CFL;
+   P->max_dt = L->maxDT;
+   P->rho_floor = L->Density_Floor;
+   P->p_floor = L->Pressure_Floor;
+   P->viscosity = L->viscosity;
+   P->mach = L->Disk_Mach;
+   P->metric_par2 = L->metricPar2;
+   P->rot_omega = L->RotOmega;
+   P->rot_d = L->RotD;
+   double ip[9] = {(double)L->initPar0, L->initPar1, L->initPar2, L->initPar3, L->initPar4,
+                   L->initPar5, L->initPar6, L->initPar7, L->initPar8};
+   memcpy(P->init_par, ip, sizeof ip);
+}
+
+static void planets_pack(const struct domain *d, double *out)
+{
+   for (int p = 0; p < d->Npl; ++p) {
+      const struct planet *pl = d->thePlanets + p;
+      double *v = out + p * DH_PLANET_STRIDE;
+      v[0] = pl->r; v[1] = pl->phi; v[2] = pl->M; v[3] = pl->omega;
+      v[4] = pl->vr; v[5] = pl->eps; v[6] = (double)pl->type; v[7] = 0.0;
+   }
+}
+
+int disco_setup(struct domain *d)
+{
+   struct param_list *L = &d->theParList;
+   if (L->restart_flag) return host_fail("Restart is not supported by this host layer");
+   if (L->RotFrame) return host_fail("RotFrame is not supported");
+   if (L->include_atmos) return host_fail("Include_Atmos is not supported");
+   if (L->R_Periodic || L->Z_Periodic) return host_fail("periodic boundaries are not supported");
+   disco_setup_grid(d);
+   int Nr = d->Nr, Nz = d->Nz;
+   int *Np = d->Np;
+   struct disco_accel *A = (struct disco_accel *)calloc(1, sizeof *A);
+   d->accel = A;
+   A->off = (long *)calloc(Nr * Nz + 1, sizeof(long));
+   for (int a = 0; a < Nr * Nz; ++a) A->off[a + 1] = A->off[a] + Np[a];
+   A->ntot = A->off[Nr * Nz];
+   int sides_ng[4] = {d->NgRa, d->NgRb, d->NgZa, d->NgZb};
+   for (int s = 0; s < 4; ++s) {
+      int kind = bc_kind_of_module(BOUNDARY, s);
+      if (kind < 0) return host_fail("boundary module '%s' has no device implementation", BOUNDARY);
+      A->bc[s] = sides_ng[s] > 0 ? kind : DH_BC_NONE;
+   }
+
+   /* setupDomain (domain.c:26-127): cells, planets, random phi offsets. */
+   srand(314159);
+   rand();
+   d->theCells = (struct cell **)malloc(sizeof(struct cell *) * Nr * Nz);
+   for (int a = 0; a < Nr * Nz; ++a)
+      d->theCells[a] = (struct cell *)calloc(Np[a], sizeof(struct cell));
+   setPlanetParams(d);
+   d->thePlanets = (struct planet *)calloc(d->Npl > 0 ? d->Npl : 1, sizeof(struct planet));
+   initializePlanets(d->thePlanets);
+   if (d->Npl > DH_MAX_PLANETS) return host_fail("at most %d planets", DH_MAX_PLANETS);
+   double Pmax = d->phi_max;
+   for (int k = 0; k < Nz; ++k)
+      for (int j = 0; j < Nr; ++j) {
+         int jk = k * Nr + j;
+         double p0 = Pmax * (double)rand() / (double)RAND_MAX;
+         double dp = Pmax / (double)Np[jk];
+         for (int i = 0; i < Np[jk]; ++i) {
+            double phi = p0 + dp * (double)i;
+            if (phi > Pmax) phi -= Pmax;
+            d->theCells[jk][i].piph = phi;
+            d->theCells[jk][i].dphi = dp;
+         }
+      }
+   d->t = L->t_min;
+   d->t_init = L->t_min;
+   d->t_fin = L->t_max;
+   d->count_steps = 0;
+   d->final_step = 0;
+   setICparams(d);
+
+   /* setupCells (domain.c:139-242) */
+   for (int a = 0; a < Nr * Nz; ++a)   /* calc_dp */
+      for (int i = 0; i < Np[a]; ++i) {
+         int im = i == 0 ? Np[a] - 1 : i - 1;
+         double v = d->theCells[a][i].piph - d->theCells[a][im].piph;
+         while (v < 0.0) v += Pmax;
+         while (v > Pmax) v -= Pmax;
+         d->theCells[a][i].dphi = v;
+      }
+   for (int k = d->NgZa; k < Nz - d->NgZb; ++k) {
+      double z = .5 * (d->z_kph[k] + d->z_kph[k - 1]);
+      for (int j = d->NgRa; j < Nr - d->NgRb; ++j) {
+         double r = centroid_r(d->r_jph[j], d->r_jph[j - 1]);
+         int jk = j + Nr * k;
+         for (int i = 0; i < Np[jk]; ++i) {
+            struct cell *c = &d->theCells[jk][i];
+            double x[3] = {r, c->piph - .5 * c->dphi, z};
+            initial(c->prim, x);
+            if (L->noiseType != 0) add_noise(d, c->prim, x);
+            c->real = 1;
+         }
+      }
+   }
+   /* Ghost annuli of held sides: the values boundary_trans writes, checked
+    * for independence of phi (otherwise they would change as cells move). */
+   for (int k = 0; k < Nz; ++k)
+      for (int j = 0; j < Nr; ++j) {
+         int side = -1;
+         if (j < d->NgRa) side = 0;
+         else if (j >= Nr - d->NgRb) side = 1;
+         else if (k < d->NgZa) side = 2;
+         else if (k >= Nz - d->NgZb) side = 3;
+         if (side < 0) continue;
+         int jk = j + Nr * k;
+         for (int i = 0; i < Np[jk]; ++i) {
+            struct cell *c = &d->theCells[jk][i];
+            double probe[NUM_Q];
+            ghost_initial(d, c, j, k, 0.0, c->prim);
+            ghost_initial(d, c, j, k, 1.2345, probe);
+            int rside = side < 2 ? side : (side == 2 ? 2 : 3);
+            if (A->bc[rside] == DH_BC_HELD && memcmp(probe, c->prim, sizeof probe) != 0)
+               return host_fail("initial() depends on phi in the ghost annuli of side %d: "
+                                "a held boundary cannot represent it", rside);
+         }
+      }
+
+   /* Device state. */
+   dh_grid g;
+   g.Nr = Nr; g.Nz = Nz;
+   g.NgRa = d->NgRa; g.NgRb = d->NgRb; g.NgZa = d->NgZa; g.NgZb = d->NgZb;
+   g.Np = Np;
+   g.r_jph = d->r_jph - 1;
+   g.z_kph = d->z_kph - 1;
+   g.phi_max = d->phi_max;
+   dh_params P;
+   fill_params(d, &P, A->bc);
+   if (dh_create(&g, &P, &A->ctx)) return host_fail("dh_create: %s", dh_last_error());
+   long nt = A->ntot;
+   double *prim = (double *)malloc(sizeof(double) * nt * NUM_Q);
+   double *piph = (double *)malloc(sizeof(double) * nt);
+   double *dphi = (double *)malloc(sizeof(double) * nt);
+   for (int a = 0; a < Nr * Nz; ++a)
+      for (int i = 0; i < Np[a]; ++i) {
+         long c = A->off[a] + i;
+         for (int q = 0; q < NUM_Q; ++q) prim[q * nt + c] = d->theCells[a][i].prim[q];
+         piph[c] = d->theCells[a][i].piph;
+         dphi[c] = d->theCells[a][i].dphi;
+      }
+   int rc = dh_upload_state(A->ctx, prim, NULL, piph, dphi, NULL, NULL);
+   free(prim); free(piph); free(dphi);
+   if (rc) return host_fail("dh_upload_state: %s", dh_last_error());
+   if (dh_init_cells(A->ctx)) return host_fail("dh_init_cells: %s", dh_last_error());
+   A->host_dirty = 1;
+   return disco_sync_to_host(d);
+}
+
+int disco_sync_to_host(struct domain *d)
+{
+   struct disco_accel *A = (struct disco_accel *)d->accel;
+   if (!A || !A->host_dirty) return 0;
+   long nt = A->ntot;
+   double *prim = (double *)malloc(sizeof(double) * nt * NUM_Q);
+   double *cons = (double *)malloc(sizeof(double) * nt * NUM_Q);
+   double *piph = (double *)malloc(sizeof(double) * nt);
+   double *dphi = (double *)malloc(sizeof(double) * nt);
+   double *wiph = (double *)malloc(sizeof(double) * nt);
+   int rc = dh_download_state(A->ctx, prim, cons, piph, dphi, wiph, NULL);
+   if (rc == 0) {
+      for (int a = 0; a < d->Nr * d->Nz; ++a)
+         for (int i = 0; i < d->Np[a]; ++i) {
+            long c = A->off[a] + i;
+            struct cell *cl = &d->theCells[a][i];
+            for (int q = 0; q < NUM_Q; ++q) {
+               cl->prim[q] = prim[q * nt + c];
+               cl->cons[q] = cons[q * nt + c];
+            }
+            cl->piph = piph[c];
+            cl->dphi = dphi[c];
+            cl->wiph = wiph[c];
+         }
+      A->host_dirty = 0;
+   }
+   free(prim); free(cons); free(piph); free(dphi); free(wiph);
+   return rc ? host_fail("dh_download_state: %s", dh_last_error()) : 0;
+}
+
+int disco_export_state(struct domain *d, double *prim, double *cons, double *piph,
+                       double *dphi, double *wiph)
+{
+   struct disco_accel *A = (struct disco_accel *)d->accel;
+   if (dh_download_state(A->ctx, prim, cons, piph, dphi, wiph, NULL))
+      return host_fail("dh_download_state: %s", dh_last_error());
+   return 0;
+}
+
+long disco_ntot(const struct domain *d) { return ((struct disco_accel *)d->accel)->ntot; }
+
+void *disco_device_context(struct domain *d) { return ((struct disco_accel *)d->accel)->ctx; }
+
+/* ------------------------------------------------------------------------ */
+/* Reference-signature entry points                                          */
+/* ------------------------------------------------------------------------ */
+
+/* getmindt, misc.c:39-84 */
+double getmindt(struct domain *d)
+{
+   struct disco_accel *A = (struct disco_accel *)d->accel;
+   double dt = 0.0;
+   if (dh_getmindt(A->ctx, &dt)) {
+      host_fail("dh_getmindt: %s", dh_last_error());
+      exit(1);
+   }
+   return dt;
+}
+
+/* planet_RK_adjust (planet.c:204-212) and the planet part of clean_pi
+ * (misc.c:26-35). */
+static void planets_rk_adjust(struct domain *d, double RK)
+{
+   for (int p = 0; p < d->Npl; ++p) {
+      struct planet *pl = d->thePlanets + p;
+      pl->r = (1. - RK) * pl->r + RK * pl->RK_r;
+      pl->phi = (1. - RK) * pl->phi + RK * pl->RK_phi;
+      pl->M = (1. - RK) * pl->M + RK * pl->RK_M;
+      pl->omega = (1. - RK) * pl->omega + RK * pl->RK_omega;
+      pl->vr = (1. - RK) * pl->vr + RK * pl->RK_vr;
+   }
+}
+
+static void planets_clean(struct domain *d)
+{
+   double pm = d->phi_max;
+   for (int p = 0; p < d->Npl; ++p) {
+      struct planet *pl = d->thePlanets + p;
+      double phi = pl->phi;
+      while (phi > pm) { phi -= pm; pl->RK_phi -= pm; }
+      while (phi < 0.0) { phi += pm; pl->RK_phi += pm; }
+      pl->phi = phi;
+   }
+}
+
+/* timestep, timestep.c:7-45.  The planets each substep's sources see are
+ * computed on the host first (onestep.c:164-168 moves them after the
+ * sources), then the whole step runs on the device. */
+void timestep(struct domain *d, double dt)
+{
+   struct disco_accel *A = (struct disco_accel *)d->accel;
+   double pl0[DH_MAX_PLANETS * DH_PLANET_STRIDE], pl1[DH_MAX_PLANETS * DH_PLANET_STRIDE];
+   for (int p = 0; p < d->Npl; ++p) {   /* planet_RK_copy */
+      struct planet *pl = d->thePlanets + p;
+      pl->RK_r = pl->r; pl->RK_phi = pl->phi; pl->RK_M = pl->M;
+      pl->RK_omega = pl->omega; pl->RK_vr = pl->vr;
+   }
+   int analytic = planet_motion_analytic();
+   planets_pack(d, pl0);
+   /* after substep 1 */
+   planets_rk_adjust(d, 0.0);
+   movePlanets(d->thePlanets, d->t, dt);
+   planets_clean(d);
+   planets_pack(d, pl1);
+   int rc = dh_set_planets(A->ctx, 0, pl0, d->Npl);
+   if (!rc) rc = dh_set_planets(A->ctx, 1, pl1, d->Npl);
+   if (!rc) rc = dh_timestep(A->ctx, dt);
+   if (rc) {
+      host_fail("dh_timestep: %s", dh_last_error());
+      exit(1);
+   }
+   d->t += dt;
+   if (d->theParList.Timestep != 1 && !analytic) {
+      planets_rk_adjust(d, 0.5);
+      movePlanets(d->thePlanets, d->t, 0.5 * dt);
+      planets_clean(d);
+   }
+   A->host_dirty = 1;
+   d->count_steps += 1;
+}
+
+/* check_dt, domain.c:284-311 (serial; the abort/latest files as in the
+ * reference). */
+void check_dt(struct domain *d, double *dt)
+{
+   int final = 0;
+   if (d->t + *dt > d->t_fin) {
+      *dt = d->t_fin - d->t;
+      final = 1;
+   }
+   FILE *f = fopen("abort", "r");
+   if (f) { final = 1; fclose(f); }
+   if (final) d->final_step = 1;
+}
+
+void disco_free(struct domain *d)
+{
+   struct disco_accel *A = (struct disco_accel *)d->accel;
+   if (A) {
+      if (A->ctx) dh_destroy(A->ctx);
+      free(A->off);
+      free(A);
+      d->accel = NULL;
+   }
+   if (d->theCells) {
+      for (int a = 0; a < d->Nr * d->Nz; ++a) free(d->theCells[a]);
+      free(d->theCells);
+   }
+   free(d->Np);
+   if (d->r_jph) free(d->r_jph - 1);
+   if (d->z_kph) free(d->z_kph - 1);
+   free(d->thePlanets);
+   memset(d, 0, sizeof *d);
+}
+
+/* Writes the device state as a DSTATE01 file (disco_amd/state_io.py). */
+int disco_write_dstate(struct domain *d, const char *path)
+{
+   struct disco_accel *A = (struct disco_accel *)d->accel;
+   long nt = A->ntot;
+   int Nr = d->Nr, Nz = d->Nz;
+   double *buf = (double *)malloc(sizeof(double) * nt * (2 * NUM_Q + 3));
+   double *prim = buf, *cons = buf + nt * NUM_Q, *piph = cons + nt * NUM_Q;
+   double *dphi = piph + nt, *wiph = dphi + nt;
+   if (disco_export_state(d, prim, cons, piph, dphi, wiph)) { free(buf); return 1; }
+   FILE *f = fopen(path, "wb");
+   if (!f) { free(buf); return host_fail("cannot open %s", path); }
+   char magic[8] = {'D', 'S', 'T', 'A', 'T', 'E', '0', '1'};
+   int hdr[16] = {0};
+   hdr[0] = NUM_Q; hdr[1] = 0; hdr[2] = Nr; hdr[3] = Nz;
+   hdr[4] = d->NgRa; hdr[5] = d->NgRb; hdr[6] = d->NgZa; hdr[7] = d->NgZb;
+   hdr[8] = (int)nt; hdr[9] = d->Npl; hdr[10] = d->count_steps; hdr[11] = NUM_C;
+   double dhdr[8] = {d->t, d->phi_max, 0, 0, 0, 0, 0, 0};
+   fwrite(magic, 1, 8, f);
+   fwrite(hdr, sizeof(int), 16, f);
+   fwrite(dhdr, sizeof(double), 8, f);
+   fwrite(d->Np, sizeof(int), Nr * Nz, f);
+   fwrite(d->r_jph - 1, sizeof(double), Nr + 1, f);
+   fwrite(d->z_kph - 1, sizeof(double), Nz + 1, f);
+   fwrite(buf, sizeof(double), nt * (2 * NUM_Q + 3), f);
+   double pl[DH_MAX_PLANETS * DH_PLANET_STRIDE];
+   planets_pack(d, pl);
+   fwrite(pl, sizeof(double), d->Npl * DH_PLANET_STRIDE, f);
+   fclose(f);
+   free(buf);
+   return 0;
+}
+
+/* Heap-allocated, zeroed domain for foreign callers (ctypes). */
+struct domain *disco_new(void) { return (struct domain *)calloc(1, sizeof(struct domain)); }
+
+void disco_delete(struct domain *d)
+{
+   if (!d) return;
+   disco_free(d);
+   free(d);
+}
+
+/* Real (non-ghost) cells, as profiler.c:12-37 counts them. */
+long disco_real_cells(const struct domain *d)
+{
+   long n = 0;
+   for (int k = d->NgZa; k < d->Nz - d->NgZb; ++k)
+      for (int j = d->NgRa; j < d->Nr - d->NgRb; ++j) n += d->Np[j + d->Nr * k];
+   return n;
+}

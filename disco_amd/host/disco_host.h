@@ -1,0 +1,44 @@
+/*
+ * disco_host.h -- host layer of the drop-in (libdisco_host_<config>.so and the
+ * `disco` driver), built per module selection like the reference's Makefile.
+ *
+ * Reference-signature entry points (what a Disco driver links against):
+ *   double getmindt(struct domain *)            misc.c:39-84
+ *   void   timestep(struct domain *, double)    timestep.c:7-45
+ *   void   check_dt(struct domain *, double *)  domain.c:284-311
+ * Set-up (readpar.c:44-188, gridsetup.c:15-144, domain.c:26-242):
+ *   disco_read_par, disco_setup_grid, disco_setup
+ * Device state <-> host mirror: disco_sync_to_host, disco_export_state.
+ */
+#ifndef DISCO_HOST_H
+#define DISCO_HOST_H
+
+#include "paul.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int disco_read_par(struct domain *d, const char *file);
+void disco_setup_grid(struct domain *d);
+int disco_setup(struct domain *d);
+int disco_sync_to_host(struct domain *d);
+int disco_export_state(struct domain *d, double *prim, double *cons, double *piph,
+                       double *dphi, double *wiph);
+long disco_ntot(const struct domain *d);
+void *disco_device_context(struct domain *d);
+void disco_free(struct domain *d);
+const char *disco_last_error(void);
+struct domain *disco_new(void);
+void disco_delete(struct domain *d);
+long disco_real_cells(const struct domain *d);
+int disco_write_dstate(struct domain *d, const char *path);
+
+double getmindt(struct domain *d);
+void timestep(struct domain *d, double dt);
+void check_dt(struct domain *d, double *dt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

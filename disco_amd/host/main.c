@@ -1,0 +1,48 @@
+/*
+ * disco -- driver of one Disco run on the MI355X timestep (main.c:26-97 of the
+ * reference, serial): reads ./in.par, sets the mesh up, then loops
+ *     dt = getmindt(); check_dt(&dt); timestep(dt);
+ * until T_End or an `abort` file, and writes the final state to
+ * output.dstate (DSTATE01 layout, disco_amd/state_io.py).
+ *
+ * Usage: disco [max_steps]
+ */
+#include <sys/time.h>
+
+#include "disco_host.h"
+
+static double wall(void)
+{
+   struct timeval tv;
+   gettimeofday(&tv, NULL);
+   return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+int disco_write_dstate(struct domain *d, const char *path);
+
+int main(int argc, char **argv)
+{
+   long max_steps = argc > 1 ? atol(argv[1]) : -1;
+   struct domain D;
+   memset(&D, 0, sizeof D);
+   if (disco_read_par(&D, "in.par")) return 1;
+   if (disco_setup(&D)) return 1;
+   remove("abort");
+   double t0 = wall();
+   while (!D.final_step && (max_steps < 0 || D.count_steps < max_steps)) {
+      double dt = getmindt(&D);
+      check_dt(&D, &dt);
+      timestep(&D, dt);
+   }
+   double secs = wall() - t0;
+   long ncells = 0;
+   for (int k = D.NgZa; k < D.Nz - D.NgZb; ++k)
+      for (int j = D.NgRa; j < D.Nr - D.NgRb; ++j) ncells += D.Np[j + D.Nr * k];
+   printf("{\"cells\": %ld, \"steps\": %d, \"t\": %.10e, \"seconds\": %.6f, "
+          "\"zone_updates_per_s\": %.6e}\n",
+          ncells, D.count_steps, D.t, secs,
+          secs > 0 ? (double)ncells * D.count_steps / secs : 0.0);
+   if (disco_write_dstate(&D, "output.dstate")) return 1;
+   disco_free(&D);
+   return 0;
+}

@@ -127,6 +127,12 @@ int dh_upload_state(dh_ctx *ctx, const double *prim, const double *cons,
 int dh_download_state(dh_ctx *ctx, double *prim, double *cons, double *piph,
                       double *dphi, double *wiph, double *Phi);
 
+/* Finishes the set-up of uploaded primitives (domain.c:175-241 and
+ * main.c:53-62): cons = prim2cons(prim) and prim = cons2prim(cons) on real
+ * cells, set_wcell on every cell, then the boundaries of every side.
+ * Ghost prims of DH_BC_HELD sides must already hold their boundary values. */
+int dh_init_cells(dh_ctx *ctx);
+
 /* Planets seen by the sources of RK substep `substep` (0 or 1) of the next
  * dh_timestep.  npl <= DH_MAX_PLANETS. */
 int dh_set_planets(dh_ctx *ctx, int substep, const double *planets, int npl);

@@ -1,0 +1,75 @@
+"""Shared helpers of the test suite."""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(HERE, "golden"))
+
+from disco_amd import configs  # noqa: E402
+from disco_amd.state_io import MeshState  # noqa: E402
+import make_golden  # noqa: E402
+
+GOLDEN = os.path.join(HERE, "golden")
+
+
+def _state(z, prefix) -> MeshState:
+    nq, nf, nr, nz = (int(v) for v in z["dims"])
+    return MeshState(nq=nq, nf=nf, nr=nr, nz=nz, ng=tuple(int(v) for v in z["ng"]), t=0.0,
+                     phi_max=float(z["phi_max"][0]), np_=z["np"].astype(np.int32),
+                     r_jph=z["r_jph"], z_kph=z["z_kph"],
+                     prim=z[f"{prefix}_prim"].copy(), cons=z[f"{prefix}_cons"].copy(),
+                     piph=z[f"{prefix}_piph"].copy(), dphi=z[f"{prefix}_dphi"].copy(),
+                     wiph=z[f"{prefix}_wiph"].copy(),
+                     phi_flux=z[f"{prefix}_phi_flux"].copy(), planets=z["planets"].copy())
+
+
+def load_golden(name):
+    """(config, in.par dict, initial state, final state, dt list) of a fixture."""
+    cname, over, steps = make_golden.CASES[name]
+    cfg = configs.get(cname)
+    par = cfg.par_dict(**over)
+    z = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    a, b = _state(z, "init"), _state(z, "final")
+    dts = [float(v) for v in np.atleast_1d(z["dt"])]
+    assert len(dts) == steps
+    return cfg, par, a, b, dts
+
+
+GOLDEN_CASES = list(make_golden.CASES)
+
+
+def rel_l1(x: np.ndarray, y: np.ndarray) -> float:
+    """Relative L1 difference sum|x-y| / sum|y| (0 when both are 0)."""
+    den = np.abs(y).sum()
+    num = np.abs(x - y).sum()
+    return float(num / den) if den > 0 else float(num)
+
+
+# Parity bar of the HIP path against the reference: relative L1 of every
+# primitive (and conserved) field over real cells after the run.
+PARITY_TOL = 1e-10
+
+
+def field_errors(s: MeshState, ref: MeshState):
+    """Relative L1 over real cells of every prim/cons field, and of piph."""
+    m = ref.real_mask()
+    out = {}
+    for q in range(ref.nq):
+        out[f"prim{q}"] = rel_l1(s.prim[q][m], ref.prim[q][m])
+        out[f"cons{q}"] = rel_l1(s.cons[q][m], ref.cons[q][m])
+    out["piph"] = rel_l1(s.piph, ref.piph)
+    return out
+
+
+def compare_states(s: MeshState, ref: MeshState, tol: float = PARITY_TOL):
+    """Asserts every field within tol; returns the worst relative L1."""
+    errs = field_errors(s, ref)
+    worst_key = max(errs, key=errs.get)
+    worst = errs[worst_key]
+    assert worst <= tol, f"relative L1 {worst:.3e} ({worst_key}) > {tol:.1e}: {errs}"
+    return worst
