@@ -126,6 +126,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "dh_sync": ([vp], C.c_int),
         "dh_enable_timers": ([vp, C.c_int], C.c_int),
         "dh_read_timers": ([vp, C.c_int, C.c_void_p, _dp, C.POINTER(C.c_long), _ip], C.c_int),
+        "dh_face_count": ([vp, C.c_int, C.POINTER(C.c_long)], C.c_int),
+        "dh_get_faces": ([vp, C.c_int, _ip, _ip, _ip, _dp, _dp], C.c_int),
         "dh_last_error": ([], C.c_char_p),
     }
     for name, (args, res) in sig.items():
@@ -137,7 +139,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
 
 EXPORTED_SYMBOLS = ["dh_create", "dh_destroy", "dh_sizes", "dh_upload_state",
                     "dh_download_state", "dh_init_cells", "dh_set_planets", "dh_getmindt", "dh_timestep",
-                    "dh_sync", "dh_enable_timers", "dh_read_timers", "dh_last_error"]
+                    "dh_face_count", "dh_get_faces", "dh_sync", "dh_enable_timers",
+                    "dh_read_timers", "dh_last_error"]
 
 _LIB: Optional[C.CDLL] = None
 
@@ -153,8 +156,8 @@ class DiscoHip:
     """One mesh on the GPU, driven through the dh_* C ABI."""
 
     def __init__(self, cfg: "cfgmod.Config", par: Dict[str, object], state: MeshState,
-                 planets: Optional[np.ndarray] = None):
-        self.lib = lib()
+                 planets: Optional[np.ndarray] = None, library: Optional[C.CDLL] = None):
+        self.lib = library if library is not None else lib()
         self.cfg = cfg
         self.grid = GridView(state)
         self.params = make_params(cfg, par)
@@ -191,6 +194,19 @@ class DiscoHip:
     def set_planets(self, substep: int, planets: np.ndarray):
         pl = np.ascontiguousarray(planets, dtype=np.float64).reshape(-1, PLANET_STRIDE)
         self._check(self.lib.dh_set_planets(self.ctx, substep, _ptr(pl), pl.shape[0]))
+
+    def faces(self, dim: int):
+        """(L, R, LRtype, cm[n,3], dA) of the r (1) or z (2) faces of the mesh."""
+        n = C.c_long()
+        self._check(self.lib.dh_face_count(self.ctx, dim, C.byref(n)))
+        L = np.zeros(n.value, np.int32)
+        R = np.zeros(n.value, np.int32)
+        LR = np.zeros(n.value, np.int32)
+        cm = np.zeros((n.value, 3))
+        dA = np.zeros(n.value)
+        ip = lambda a: a.ctypes.data_as(_ip)  # noqa: E731
+        self._check(self.lib.dh_get_faces(self.ctx, dim, ip(L), ip(R), ip(LR), _ptr(cm), _ptr(dA)))
+        return L, R, LR, cm, dA
 
     def init_cells(self):
         self._check(self.lib.dh_init_cells(self.ctx))

@@ -1,0 +1,151 @@
+// dh_ct.h -- constrained transport (CT_MODE 2) on the moving mesh, as
+// owner-computes kernels (device side of the reference bfields.c).
+//
+// State: Phi[5][c] per cell, the magnetic flux through the cell's upper phi
+// face (0), its owned inner/outer r faces (1, 2) and inner/outer z faces
+// (3, 4).  A face between annuli is owned by the cell whose upper phi edge
+// bounds it (faces_alt.c:147-151: LRtype 0 -> L owns it in slot 2 or 4,
+// LRtype 1 -> R owns it in slot 1 or 3); every cell owns exactly one face on
+// each side.  EMFs live on cell edges: E[0..3] on the z-directed edges at
+// (r-, phi+), (r+, phi+), (r-, phi-), (r+, phi-), E[4..7] on the r-directed
+// edges at (z-, phi+), (z+, phi+), (z-, phi-), (z+, phi-) (riemann.c:66-211),
+// and E_phi on the phi-directed edges where four annuli meet.
+//
+// The reference updates these with sequential scatter loops over faces and
+// edges.  Here every output slot is computed by its owner:
+//   k_edges_collect  edge EMFs after the Riemann solves   (riemann.c:66-211)
+//   k_edges_avg      phi-neighbour averaging              (bfields.c:400-441)
+//   k_face_emf       face-interpolated edge EMFs          (bfields.c:443-531)
+//   k_edges_final    owner edges + advective subtraction  (bfields.c:533-690)
+//   k_update_B       Phi += E dl dt of the face edges     (bfields.c:313-382)
+//   k_edge_adjust    E_phi edges, a merge of two face lists instead of the
+//                    reference's 4-way ring walk           (bfields.c:888-1127)
+//   k_flips          face re-association after the mesh moved (bfields.c:693-814)
+//   k_bfaces_c2p     Phi -> cell B, then cons2prim         (bfields.c:88-309)
+#pragma once
+#include "dh_mesh.h"
+
+namespace dh {
+
+// Per-cell face ranges of one direction (rng[d][k*ntot + c]):
+//   k = 0..3: inner pair (cell is R): head lo, head hi, tail lo, tail hi
+//   k = 4..7: outer pair (cell is L): same.  Empty ranges have lo > hi.
+struct CellRanges {
+   int in_h0, in_h1, in_t0, in_t1, out_h0, out_h1, out_t0, out_t1;
+   int base_in, base_out;   // first face of the inner / outer pair
+   bool in, out;            // pair exists
+};
+
+DHD CellRanges load_ranges(const Mesh &m, int dim, const int *rng, const int *fI, int c, int a)
+{
+   size_t nt = (size_t)m.ntot;
+   CellRanges r;
+   r.in_h0 = rng[0 * nt + c]; r.in_h1 = rng[1 * nt + c];
+   r.in_t0 = rng[2 * nt + c]; r.in_t1 = rng[3 * nt + c];
+   r.out_h0 = rng[4 * nt + c]; r.out_h1 = rng[5 * nt + c];
+   r.out_t0 = rng[6 * nt + c]; r.out_t1 = rng[7 * nt + c];
+   int j = a % m.Nr, k = a / m.Nr;
+   if (dim == 1) {
+      r.in = j > 0;
+      r.out = j < m.Nr - 1;
+      r.base_in = r.in ? fI[(j - 1) + (m.Nr - 1) * k] : 0;
+      r.base_out = r.out ? fI[j + (m.Nr - 1) * k] : 0;
+   } else {
+      r.in = k > 0;
+      r.out = k < m.Nz - 1;
+      r.base_in = r.in ? fI[j + m.Nr * (k - 1)] : 0;
+      r.base_out = r.out ? fI[j + m.Nr * k] : 0;
+   }
+   return r;
+}
+
+// The track loops of update_B_fluxes and check_flipped/flip_fluxes
+// (bfields.c:313-382, 693-814) visit track jk-1 at iteration jk, so the last
+// face track of each direction (a ghost corner) is never visited.
+DHD bool track_skipped(int npairs, int JK) { return JK == npairs - 1; }
+
+DHD int pair_index(const Mesh &m, int dim, int j, int k)
+{
+   return dim == 1 ? j + (m.Nr - 1) * k : j + m.Nr * k;
+}
+
+// Slot of the face's owner: the reference's get_phi_pointer (bfields.c:716-740).
+DHD size_t phi_slot(size_t nt, int dim, int LR, int L, int R)
+{
+   if (dim == 1) return LR == 0 ? 2 * nt + L : 1 * nt + R;
+   return LR == 0 ? 4 * nt + L : 3 * nt + R;
+}
+
+// phi_switch (bfields.c:817-829)
+DHD int phi_switch(double dphi, double Pmax, int mode)
+{
+   dphi = wrap_half(dphi, Pmax);
+   if (mode == 1) dphi = -dphi;
+   return dphi > 0. ? 1 : 0;
+}
+
+// get_which4 (bfields.c:831-886)
+DHD int get_which4(double phi, double phiR, double phiU, double phiUR, int *LR_alt,
+                   int *UD_alt, int mode, double Pmax)
+{
+   int LR_D = phi_switch(phi - phiR, Pmax, mode);
+   int LR_U = phi_switch(phiU - phiUR, Pmax, mode);
+   double phi1 = LR_D ? phiR : phi;
+   double phi2 = LR_U ? phiUR : phiU;
+   int UD = phi_switch(phi1 - phi2, Pmax, mode);
+   int w4;
+   if (UD == 0) w4 = LR_D == 0 ? 0 : 1;
+   else w4 = LR_U == 0 ? 2 : 3;
+   if (mode == 0) {
+      *LR_alt = (w4 == 0 || w4 == 1) ? LR_U : LR_D;
+      if (w4 == 0 || w4 == 2) *UD_alt = phi_switch(phiR - phiUR, Pmax, mode);
+      else *UD_alt = phi_switch(phi - phiU, Pmax, mode);
+   }
+   return w4;
+}
+
+// E_phi of the face a cell owns on one side: Eown[s][c], s = 0 r-in, 1 r-out,
+// 2 z-in, 3 z-out (the reference's E_phi[0..3] cell slots).
+struct EdgeCells {
+   int c, cR, cU, cUR;
+};
+
+// One segment of the phi-directed edge line at (r_jph[j], z_kph[k]) where the
+// four cells c (j,k), cR (j+1,k), cU (j,k+1), cUR (j+1,k+1) meet: its EMF and
+// length, exactly as make_edge_adjust forms them (bfields.c:948-1116).
+DHD void edge_segment(const double *piph, const double *dphi, const double *Eown, size_t nt,
+                      const EdgeCells &q, double r, double Pmax, double *E, double *dl)
+{
+   int LR_alt, UD_alt;
+   int w4 = get_which4(piph[q.c], piph[q.cR], piph[q.cU], piph[q.cUR], &LR_alt, &UD_alt, 0, Pmax);
+   int w4b = get_which4(piph[q.c] - dphi[q.c], piph[q.cR] - dphi[q.cR], piph[q.cU] - dphi[q.cU],
+                        piph[q.cUR] - dphi[q.cUR], nullptr, nullptr, 1, Pmax);
+   // Eown slots: 0 r-in (E_phi[0]), 1 r-out (E_phi[1]), 2 z-in (E_phi[2]), 3 z-out (E_phi[3])
+#define EO(s, cell) Eown[(s) * nt + (cell)]
+   double e;
+   if (w4 == 0) {
+      e = .25 * (EO(3, q.c) + EO(1, q.c));
+      e += LR_alt == 0 ? .25 * EO(1, q.cU) : .25 * EO(0, q.cUR);
+      e += UD_alt == 0 ? .25 * EO(3, q.cR) : .25 * EO(2, q.cUR);
+   } else if (w4 == 1) {
+      e = .25 * (EO(3, q.cR) + EO(0, q.cR));
+      e += LR_alt == 0 ? .25 * EO(1, q.cU) : .25 * EO(0, q.cUR);
+      e += UD_alt == 0 ? .25 * EO(3, q.c) : .25 * EO(2, q.cU);
+   } else if (w4 == 2) {
+      e = .25 * (EO(2, q.cU) + EO(1, q.cU));
+      e += LR_alt == 0 ? .25 * EO(1, q.c) : .25 * EO(0, q.cR);
+      e += UD_alt == 0 ? .25 * EO(3, q.cR) : .25 * EO(2, q.cUR);
+   } else {
+      e = .25 * (EO(2, q.cUR) + EO(0, q.cUR));
+      e += LR_alt == 0 ? .25 * EO(1, q.c) : .25 * EO(0, q.cR);
+      e += UD_alt == 0 ? .25 * EO(3, q.c) : .25 * EO(2, q.cU);
+   }
+#undef EO
+   int cs[4] = {q.c, q.cR, q.cU, q.cUR};
+   double xp1 = piph[cs[w4]];
+   double xm1 = piph[cs[w4b]] - dphi[cs[w4b]];
+   *E = e;
+   *dl = .5 * (r + r) * get_dp(xp1, xm1, Pmax);
+}
+
+}  // namespace dh

@@ -15,13 +15,13 @@ enum { RHO = 0, PPP = 1, URR = 2, UPP = 3, UZZ = 4, BRR = 5, BPP = 6, BZZ = 7 };
 enum { DDD = 0, TAU = 1, SRR = 2, LLL = 3, SZZ = 4 };
 enum { RS_HLL = 0, RS_HLLC = 1, RS_HLLD = 2 };
 
-#define DHD __device__ __forceinline__
+#define DHD __host__ __device__ __forceinline__
 
 // get_dp (cylindrical.c:10-15): phi difference wrapped into [0, phi_max].
 DHD double get_dp(double phip, double phim, double Pmax)
 {
    double dp = phip - phim;
-   if (!isfinite(dp)) return dp;  // a non-finite value would never wrap
+   if (!std::isfinite(dp)) return dp;  // a non-finite value would never wrap
    if (fabs(dp) > 64.0 * Pmax) dp = fmod(dp, Pmax);  // bound the loops below
    while (dp < 0.0) dp += Pmax;
    while (dp > Pmax) dp -= Pmax;
@@ -32,7 +32,7 @@ DHD double get_dp(double phip, double phim, double Pmax)
 DHD double get_sdp(double phip, double phim, double Pmax)
 {
    double dp = phip - phim;
-   if (!isfinite(dp)) return dp;
+   if (!std::isfinite(dp)) return dp;
    if (fabs(dp) > 64.0 * Pmax) dp = fmod(dp, Pmax);
    while (dp < -.5 * Pmax) dp += Pmax;
    while (dp > .5 * Pmax) dp -= Pmax;
@@ -43,7 +43,7 @@ DHD double get_sdp(double phip, double phim, double Pmax)
 // [-Pmax/2, Pmax/2] (faces_alt.c:138-139 and friends).
 DHD double wrap_half(double w, double Pmax)
 {
-   if (!isfinite(w)) return w;
+   if (!std::isfinite(w)) return w;
    if (fabs(w) > 64.0 * Pmax) w = fmod(w, Pmax);
    while (w > .5 * Pmax) w -= Pmax;
    while (w < -.5 * Pmax) w += Pmax;

@@ -1,0 +1,1475 @@
+// dh_kernels.h -- the per-thread bodies of every timestep kernel.
+//
+// Each kernel is a functor: its members are the launch arguments and
+// operator()(t) is the work of thread t (a cell, a face, a face track or a
+// boundary line).  The engine launches them through one generic __global__
+// template; the same bodies also build as host code, which the test suite
+// runs on the CPU against the oracle to check indexing and summation order
+// before anything reaches the GPU.
+//
+// Scatter loops of the reference (face fluxes added to both neighbours,
+// slope sums, edge EMFs, magnetic-flux updates) are restated as
+// owner-computes gathers: every output element is written by one thread,
+// which sums its contributions in the reference's order, so a step is
+// deterministic and needs no atomics.
+#pragma once
+#include "dh_ct.h"
+#include "dh_euler.h"
+#include "dh_geom.h"
+#include "dh_mesh.h"
+#include "dh_mhd.h"
+
+namespace dh {
+
+struct PlanetSet {
+   double v[DH_MAX_PLANETS * DH_PLANET_STRIDE];
+   int n;
+};
+
+template <class H>
+struct IsMHD {
+   static constexpr bool value = H::NC == 8;
+};
+
+// One direction's face list: faces of annulus pair P are fI[P] .. fI[P+1]-1.
+struct Faces {
+   int n, npairs;
+   const int *fI;
+   int *L, *R;
+   signed char *LR;
+   double *dA, *phic, *dphi;
+};
+
+// ---------------------------------------------------------------------------
+// Small helpers
+// ---------------------------------------------------------------------------
+
+DHD bool is_real(const Mesh &m, int j, int k)
+{
+   return j >= m.NgRa && j < m.Nr - m.NgRb && k >= m.NgZa && k < m.Nz - m.NgZb;
+}
+
+// Annuli whose phi faces carry fluxes: the real ones, plus one ghost layer
+// when constrained transport needs the edge EMFs there (misc.c:368-399).
+DHD bool phi_in_range(const Mesh &m, int ct, int j, int k)
+{
+   if (!ct) return is_real(m, j, k);
+   int jmin = m.NgRa == 0 ? 0 : m.NgRa - 1;
+   int jmax = m.NgRb == 0 ? m.Nr : m.Nr - m.NgRb + 1;
+   int kmin = m.NgZa == 0 ? 0 : m.NgZa - 1;
+   int kmax = m.NgZb == 0 ? m.Nz : m.Nz - m.NgZb + 1;
+   return j >= jmin && j < jmax && k >= kmin && k < kmax;
+}
+
+// Annulus pairs whose faces carry fluxes (trans_flux, misc.c:431-490).
+DHD bool pair_has_flux(const Mesh &m, int ct, int dim, int pj, int pk)
+{
+   int jmin, jmax, kmin, kmax;
+   if (dim == 1) {
+      jmin = m.NgRa == 0 ? 0 : m.NgRa - 1;
+      jmax = m.NgRb == 0 ? m.Nr - 1 : m.Nr - m.NgRb;
+      kmin = m.NgZa;
+      kmax = m.Nz - m.NgZb;
+      if (ct) {
+         kmin = m.NgZa == 0 ? 0 : m.NgZa - 1;
+         kmax = m.NgZb == 0 ? m.Nz : m.Nz - m.NgZb + 1;
+      }
+   } else {
+      jmin = m.NgRa;
+      jmax = m.Nr - m.NgRb;
+      kmin = m.NgZa == 0 ? 0 : m.NgZa - 1;
+      kmax = m.NgZb == 0 ? m.Nz - 1 : m.Nz - m.NgZb;
+      if (ct) {
+         jmin = m.NgRa == 0 ? 0 : m.NgRa - 1;
+         jmax = m.NgRb == 0 ? m.Nr : m.Nr - m.NgRb + 1;
+      }
+   }
+   return pj >= jmin && pj < jmax && pk >= kmin && pk < kmax;
+}
+
+DHD bool dim_active(const Mesh &m, int dim) { return dim == 1 ? m.Nr > 1 : m.Nz > 1; }
+
+template <int NQ>
+DHD void load_q(const double *soa, size_t nt, long c, double *v)
+{
+#pragma unroll
+   for (int q = 0; q < NQ; ++q) v[q] = soa[q * nt + c];
+}
+
+// Element index of the i-th cell of an annulus in its ring.
+DHD int ring_elem(const Ring &R, int i)
+{
+   int x = i - R.I0;
+   return x < 0 ? x + R.N : x;
+}
+
+// clean_pi (misc.c:8-36) for one cell.
+DHD double clean_phi(double phi, double Pmax)
+{
+   if (!std::isfinite(phi)) return phi;
+   if (fabs(phi) > 64.0 * Pmax) phi = fmod(phi, Pmax);
+   while (phi > Pmax) phi -= Pmax;
+   while (phi < 0.0) phi += Pmax;
+   return phi;
+}
+
+DHD double minmod(double a, double b, double c)
+{
+   double m = a;
+   if (a * b < 0.0) m = 0.0;
+   if (fabs(b) < fabs(m)) m = b;
+   if (b * c < 0.0) m = 0.0;
+   if (fabs(c) < fabs(m)) m = c;
+   return m;
+}
+
+// Start cell of annulus a: the first cell (from i = 0) in the first
+// quadrant that comes after a cell in the fourth (faces_alt.c:59-78).
+DHD int ring_start_of(const Mesh &m, const double *piph, int a)
+{
+   int c0 = m.off[a], np = m.Np[a];
+   double convert = 2. * M_PI / m.Pmax;
+   bool seen4 = false;
+   for (int i = 0; i < np; ++i) {
+      double sn, cs;
+      sincos(piph[c0 + i] * convert, &sn, &cs);
+      if (sn > 0. && cs > 0. && seen4) return i;
+      if (sn < 0. && cs > 0.) seen4 = true;
+   }
+   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// set_wcell (misc.c:101-168)
+// ---------------------------------------------------------------------------
+
+template <int NQ, class H>
+struct KWcell {
+   Mesh m;
+   dh_params P;
+   const double *prim;
+   double *wiph;
+   DHD void operator()(long c) const
+   {
+      int a = m.cjk[c];
+      double w = 0.0;
+      if (P.mesh_motion == 4) {
+         int j = a % m.Nr;
+         w = mesh_om(P, centroid_r(m.r_jph[j], m.r_jph[j - 1]));
+      } else if (P.mesh_motion) {
+         size_t nt = (size_t)m.ntot;
+         int c0 = m.off[a], np = m.Np[a];
+         long cp = (c - c0 + 1 == np) ? c0 : c + 1;
+         double pc[NQ], pp[NQ];
+         load_q<NQ>(prim, nt, c, pc);
+         load_q<NQ>(prim, nt, cp, pp);
+         w = .5 * (H::omega(pc) + H::omega(pp));
+      }
+      wiph[c] = w;
+   }
+};
+
+// Mesh_Motion 3: every face of an annulus moves with the annulus mean.
+struct KWcellAvg {
+   Mesh m;
+   double *wiph;
+   DHD void operator()(long a) const
+   {
+      int c0 = m.off[a], np = m.Np[a];
+      double w = 0.0;
+      for (int i = 0; i < np; ++i) w += wiph[c0 + i];
+      w /= (double)np;
+      for (int i = 0; i < np; ++i) wiph[c0 + i] = w;
+   }
+};
+
+// ---------------------------------------------------------------------------
+// Generalised-minmod slope along phi (plm.c:6-51)
+// ---------------------------------------------------------------------------
+
+template <int NQ>
+struct KPlmPhi {
+   Mesh m;
+   double PLM;
+   const double *prim, *dphi;
+   double *gradp;
+   DHD void operator()(long c) const
+   {
+      size_t nt = (size_t)m.ntot;
+      int a = m.cjk[c];
+      int c0 = m.off[a], np = m.Np[a];
+      int i = (int)(c - c0);
+      long cL = i == 0 ? c0 + np - 1 : c - 1;
+      long cR = i == np - 1 ? c0 : c + 1;
+      double dpL = dphi[cL], dpC = dphi[c], dpR = dphi[cR];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+         double pL = prim[q * nt + cL];
+         double pC = prim[q * nt + c];
+         double pR = prim[q * nt + cR];
+         double sL = pC - pL;
+         sL /= .5 * (dpC + dpL);
+         double sR = pR - pC;
+         sR /= .5 * (dpR + dpC);
+         double sC = pR - pL;
+         sC /= .5 * (dpL + dpR) + dpC;
+         gradp[q * nt + c] = minmod(PLM * sL, sC, PLM * sR);
+      }
+   }
+};
+
+// ---------------------------------------------------------------------------
+// Faces (faces_alt.c:10-34, 135-175)
+// ---------------------------------------------------------------------------
+
+struct FaceGeo {
+   int cL, cR, LR;
+   double phic, dp, dA;
+};
+
+DHD FaceGeo face_geometry(const Mesh &m, int dim, const Pair &pr, const Ring &A,
+                          const Ring &B, int d, const double *piph, const double *dphi)
+{
+   double Pmax = m.Pmax;
+   int x = merge_x(A, B, d, Pmax);
+   FaceGeo g;
+   g.cL = ring_cell(A, x);
+   g.cR = ring_cell(B, d - x);
+   double pL = piph[g.cL], pR = piph[g.cR];
+   double dL = dphi[g.cL], dR = dphi[g.cR];
+   double w = wrap_half(pL - pR, Pmax);
+   g.LR = w > 0. ? 1 : 0;
+   w = wrap_half(pL - dL - pR + dR, Pmax);
+   int LRb = w < 0. ? 1 : 0;
+   double xp1 = g.LR == 0 ? pL : pR;
+   double xm1 = LRb == 0 ? pL - dL : pR - dR;
+   g.dp = get_dp(xp1, xm1, Pmax);
+   g.phic = get_dp(xp1, .5 * g.dp, Pmax);
+   int j = pr.j, k = pr.k;
+   if (dim == 1) {
+      double r = .5 * (m.r_jph[j] + m.r_jph[j]);
+      double dz = m.z_kph[k] - m.z_kph[k - 1];
+      g.dA = r * g.dp * dz;
+   } else {
+      double r = .5 * (m.r_jph[j] + m.r_jph[j - 1]);
+      double dr = m.r_jph[j] - m.r_jph[j - 1];
+      g.dA = r * dr * g.dp;
+   }
+   return g;
+}
+
+// Distances from the two cell centroids to the faces of a pair (faces_alt.c:107-123).
+DHD void pair_dx(const Mesh &m, int dim, int j, int k, double *dxL, double *dxR)
+{
+   if (dim == 1) {
+      double rm = centroid_r(m.r_jph[j], m.r_jph[j - 1]);
+      double rp = centroid_r(m.r_jph[j + 1], m.r_jph[j]);
+      *dxL = m.r_jph[j] - rm;
+      *dxR = rp - m.r_jph[j];
+   } else {
+      double zm = centroid_z(m.z_kph[k], m.z_kph[k - 1]);
+      double zp = centroid_z(m.z_kph[k + 1], m.z_kph[k]);
+      *dxL = m.z_kph[k] - zm;
+      *dxR = zp - m.z_kph[k];
+   }
+}
+
+// Every r (dim 1) or z (dim 2) face and, when slopes is set, its one-sided
+// slope S = (W_R - W_L)/(dx_L + dx_R) of the phi-reconstructed states
+// (plm.c:84-110).
+template <int NQ>
+struct KFaces {
+   Mesh m;
+   int dim;
+   Faces F;
+   const int *I0;
+   const double *piph, *dphi, *prim, *gradp;
+   double *S;
+   int slopes;
+   DHD void operator()(long f) const
+   {
+      size_t nt = (size_t)m.ntot, nfs = (size_t)F.n;
+      int P = pair_of_face(F.fI, F.npairs, (int)f);
+      Pair pr = pair_of(m, dim, P);
+      Ring A = make_ring(piph, m, I0, pr.a);
+      Ring B = make_ring(piph, m, I0, pr.b);
+      FaceGeo g = face_geometry(m, dim, pr, A, B, (int)(f - F.fI[P]), piph, dphi);
+      F.L[f] = g.cL;
+      F.R[f] = g.cR;
+      F.LR[f] = (signed char)g.LR;
+      F.dA[f] = g.dA;
+      F.phic[f] = g.phic;
+      F.dphi[f] = g.dp;
+      if (!slopes) return;
+      double dxL, dxR;
+      pair_dx(m, dim, pr.j, pr.k, &dxL, &dxR);
+      double phiL = piph[g.cL] - .5 * dphi[g.cL];
+      double phiR = piph[g.cR] - .5 * dphi[g.cR];
+      double dpL = get_sdp(g.phic, phiL, m.Pmax);
+      double dpR = get_sdp(phiR, g.phic, m.Pmax);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+         double WL = prim[q * nt + g.cL] + dpL * gradp[q * nt + g.cL];
+         double WR = prim[q * nt + g.cR] - dpR * gradp[q * nt + g.cR];
+         S[q * nfs + f] = (WR - WL) / (dxR + dxL);
+      }
+   }
+};
+
+// Each cell's faces in its inner and outer pair of one direction, as
+// head/tail ranges of the pair's list (merge-path counts, dh_mesh.h).
+struct KCellRanges {
+   Mesh m;
+   int dim;
+   const int *I0;
+   const double *piph;
+   int *rng;
+   DHD void operator()(long c) const
+   {
+      size_t nt = (size_t)m.ntot;
+      int a = m.cjk[c];
+      int i = (int)(c - m.off[a]);
+      int j = a % m.Nr, k = a / m.Nr;
+      Ring self = make_ring(piph, m, I0, a);
+      int xs = ring_elem(self, i);
+      bool in = dim == 1 ? j > 0 : k > 0;
+      bool out = dim == 1 ? j < m.Nr - 1 : k < m.Nz - 1;
+      FaceRange rin = {1, 0, 1, 0}, rout = {1, 0, 1, 0};
+      if (in) {
+         Ring A = make_ring(piph, m, I0, dim == 1 ? a - 1 : a - m.Nr);
+         rin = ranges_as_b(A, self, xs, m.Pmax);
+      }
+      if (out) {
+         Ring B = make_ring(piph, m, I0, dim == 1 ? a + 1 : a + m.Nr);
+         rout = ranges_as_a(self, xs, B, m.Pmax);
+      }
+      rng[0 * nt + c] = rin.h0;
+      rng[1 * nt + c] = rin.h1;
+      rng[2 * nt + c] = rin.t0;
+      rng[3 * nt + c] = rin.t1;
+      rng[4 * nt + c] = rout.h0;
+      rng[5 * nt + c] = rout.h1;
+      rng[6 * nt + c] = rout.t0;
+      rng[7 * nt + c] = rout.t1;
+   }
+};
+
+// ---------------------------------------------------------------------------
+// r/z slope of every cell: area-weighted mean of its face slopes over the
+// analytic face area, limited by every face slope (plm.c:56-195); cells on
+// the r = 0 axis get the polar limiter (plm.c:197-232, cylindrical.c:141-173).
+// ---------------------------------------------------------------------------
+
+template <int NQ>
+struct KGrad {
+   Mesh m;
+   int dim;
+   double PLM;
+   int num_c;
+   Faces F;
+   const int *rng;
+   const double *piph, *dphi, *prim, *S;
+   double *grad;
+   DHD void operator()(long c) const
+   {
+      size_t nt = (size_t)m.ntot, nfs = (size_t)F.n;
+      int a = m.cjk[c];
+      int j = a % m.Nr, k = a / m.Nr;
+      CellRanges cr = load_ranges(m, dim, rng, F.fI, c, a);
+      double acc[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+      auto add_range = [&](int base, int lo, int hi) {
+         for (int f = base + lo; f <= base + hi; ++f) {
+            double dA = F.dA[f];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) acc[q] += S[q * nfs + f] * dA;
+         }
+      };
+      if (cr.in) {
+         add_range(cr.base_in, cr.in_h0, cr.in_h1);
+         add_range(cr.base_in, cr.in_t0, cr.in_t1);
+      }
+      if (cr.out) {
+         add_range(cr.base_out, cr.out_h0, cr.out_h1);
+         add_range(cr.base_out, cr.out_t0, cr.out_t1);
+      }
+      double phip = piph[c];
+      double phim = phip - dphi[c];
+      double dp = get_dp(phip, phim, m.Pmax);
+      double dAp, dAm;
+      if (dim == 1) {
+         double dz = m.z_kph[k] - m.z_kph[k - 1];
+         dAp = .5 * (m.r_jph[j] + m.r_jph[j]) * dp * dz;
+         dAm = .5 * (m.r_jph[j - 1] + m.r_jph[j - 1]) * dp * dz;
+      } else {
+         double r = .5 * (m.r_jph[j] + m.r_jph[j - 1]);
+         double dr = m.r_jph[j] - m.r_jph[j - 1];
+         dAp = r * dr * dp;
+         dAm = dAp;
+      }
+      double dAtot = dAp + dAm;
+      if (!cr.in) dAtot = dAp;
+      if (!cr.out) dAtot = dAm;
+      double g[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) g[q] = acc[q] / dAtot;
+      // Limit against every face slope; the outcome does not depend on order.
+      auto limit_range = [&](int base, int lo, int hi) {
+         for (int f = base + lo; f <= base + hi; ++f) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+               double s = S[q * nfs + f];
+               if (s * g[q] < 0.0) g[q] = 0.0;
+               else if (fabs(PLM * s) < fabs(g[q])) g[q] = PLM * s;
+            }
+         }
+      };
+      if (cr.in) {
+         limit_range(cr.base_in, cr.in_h0, cr.in_h1);
+         limit_range(cr.base_in, cr.in_t0, cr.in_t1);
+      }
+      if (cr.out) {
+         limit_range(cr.base_out, cr.out_h0, cr.out_h1);
+         limit_range(cr.base_out, cr.out_t0, cr.out_t1);
+      }
+      if (dim == 1 && j == 0 && m.NgRa == 0) {
+         double xp0 = m.r_jph[0], xm0 = m.r_jph[-1];
+         if (!(xp0 < 0.0 || fabs(xm0) > 1.0e-10 * xp0)) {
+            double r = centroid_r(xp0, xm0);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+               if (q == URR || (num_c > BZZ && q == BRR)) {
+                  double SL = prim[q * nt + c] / r;
+                  double Sg = g[q];
+                  if (Sg * SL < 0.0) g[q] = 0.0;
+                  else if (fabs(PLM * SL) < fabs(Sg)) g[q] = PLM * SL;
+               } else {
+                  g[q] = 0.0;
+               }
+            }
+         }
+      }
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) grad[q * nt + c] = g[q];
+   }
+};
+
+// ---------------------------------------------------------------------------
+// Riemann fluxes
+// ---------------------------------------------------------------------------
+
+// Viscous flux of an interface (riemann.c:291-309)
+template <int NQ, class H>
+DHD void add_visc(const dh_params &P, const double *pL, const double *pR, const double *gr,
+                  const double *gp, const double *gz, double r, const double *n,
+                  double *Flux)
+{
+   double pa[NQ], vF[NQ];
+#pragma unroll
+   for (int q = 0; q < NQ; ++q) {
+      pa[q] = .5 * (pL[q] + pR[q]);
+      vF[q] = 0.0;
+   }
+   H::visc_flux(P, pa, gr, gp, gz, vF, r, n);
+#pragma unroll
+   for (int q = 0; q < NQ; ++q) Flux[q] += vF[q];
+}
+
+// Flux through the face between cell i and i+1 of every annulus in range
+// (misc.c:368-426, riemann.c:39-112), stored as (F - w U*) dA dt; for MHD
+// also the face's {Ez, Br, Er, Bz} (riemann.c:66-109).
+template <int NQ, class H>
+struct KFluxPhi {
+   Mesh m;
+   dh_params P;
+   double dt;
+   const double *prim, *gradp, *gradr, *gradz, *piph, *dphi, *wiph;
+   double *dFp, *emf_phi;
+   DHD void operator()(long c) const
+   {
+      constexpr bool mhd = IsMHD<H>::value;
+      int a = m.cjk[c];
+      int j = a % m.Nr, k = a / m.Nr;
+      if (!phi_in_range(m, mhd && P.ct, j, k)) return;
+      size_t nt = (size_t)m.ntot;
+      int c0 = m.off[a], np = m.Np[a];
+      long cR = (c - c0 == np - 1) ? c0 : c + 1;
+      double rp = m.r_jph[j], rm = m.r_jph[j - 1];
+      double zp = m.z_kph[k], zm = m.z_kph[k - 1];
+      double r = centroid_r(rp, rm);
+      double dAdt = (rp - rm) * (zp - zm) * dt;
+      double dL = dphi[c], dR = dphi[cR];
+      double pL[NQ], pR[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+         pL[q] = prim[q * nt + c] + .5 * gradp[q * nt + c] * dL;
+         pR[q] = prim[q * nt + cR] - .5 * gradp[q * nt + cR] * dR;
+      }
+      if (mhd) {
+         double Bp = .5 * (pL[BPP] + pR[BPP]);
+         pL[BPP] = Bp;
+         pR[BPP] = Bp;
+      }
+      const double n[3] = {0.0, 1.0, 0.0};
+      double w = r * wiph[c];
+      double Flux[NQ], Ustr[NQ];
+      H::riemann(P, pL, pR, r, n, w, Flux, Ustr);
+      if (P.visc_flag) {
+         double gr[NQ], gp[NQ], gz[NQ];
+#pragma unroll
+         for (int q = 0; q < NQ; ++q) {
+            double g = (prim[q * nt + cR] - prim[q * nt + c]) / (0.5 * (dR + dL));
+            gr[q] = .5 * (gradr[q * nt + c] + gradr[q * nt + cR]);
+            gp[q] = .5 * (g + g);
+            gz[q] = .5 * (gradz[q * nt + c] + gradz[q * nt + cR]);
+         }
+         add_visc<NQ, H>(P, pL, pR, gr, gp, gz, r, n, Flux);
+      }
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) dFp[q * nt + c] = (Flux[q] - w * Ustr[q]) * dAdt;
+      if constexpr (mhd) {
+         double e[4];
+         H::flux_to_E(Flux, Ustr, r, 0, e);
+#pragma unroll
+         for (int s = 0; s < 4; ++s) emf_phi[s * nt + c] = e[s];
+      }
+   }
+};
+
+// Flux through every r (dim 1) or z (dim 2) face of the flux range
+// (riemann.c:114-213), stored as F dA dt (these faces do not move: w = 0);
+// for MHD also the face's {E_rz, B_rz, E_phi}.
+template <int NQ, class H>
+struct KFluxTrans {
+   Mesh m;
+   dh_params P;
+   int dim;
+   double dt;
+   Faces F;
+   const double *prim, *gradp, *gradr, *gradz, *piph, *dphi;
+   double *dF, *emf_face;
+   DHD void operator()(long f) const
+   {
+      constexpr bool mhd = IsMHD<H>::value;
+      int Pi = pair_of_face(F.fI, F.npairs, (int)f);
+      Pair pr = pair_of(m, dim, Pi);
+      if (!pair_has_flux(m, mhd && P.ct, dim, pr.j, pr.k)) return;
+      size_t nt = (size_t)m.ntot, nfs = (size_t)F.n;
+      int cL = F.L[f], cR = F.R[f];
+      double dxL, dxR;
+      pair_dx(m, dim, pr.j, pr.k, &dxL, &dxR);
+      double phi = F.phic[f];
+      double dpL = get_sdp(phi, piph[cL] - .5 * dphi[cL], m.Pmax);
+      double dpR = get_sdp(piph[cR] - .5 * dphi[cR], phi, m.Pmax);
+      const double *grad = dim == 1 ? gradr : gradz;
+      double pL[NQ], pR[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+         pL[q] = prim[q * nt + cL] + grad[q * nt + cL] * dxL + gradp[q * nt + cL] * dpL;
+         pR[q] = prim[q * nt + cR] - grad[q * nt + cR] * dxR - gradp[q * nt + cR] * dpR;
+      }
+      if (mhd) {
+         int bt = dim == 1 ? BRR : BZZ;
+         double Bavg = .5 * (pL[bt] + pR[bt]);
+         pL[bt] = Bavg;
+         pR[bt] = Bavg;
+      }
+      double n[3] = {0.0, 0.0, 0.0};
+      n[dim == 1 ? 0 : 2] = 1.0;
+      int j = pr.j;
+      double r = dim == 1 ? .5 * (m.r_jph[j] + m.r_jph[j]) : centroid_r(m.r_jph[j], m.r_jph[j - 1]);
+      double Flux[NQ], Ustr[NQ];
+      H::riemann(P, pL, pR, r, n, 0.0, Flux, Ustr);
+      if (P.visc_flag) {
+         double gr[NQ], gp[NQ], gz[NQ];
+#pragma unroll
+         for (int q = 0; q < NQ; ++q) {
+            double gpL = gradp[q * nt + cL], gpR = gradp[q * nt + cR];
+            double gd = ((prim[q * nt + cR] - gpR * dpR) - (prim[q * nt + cL] + gpL * dpL)) / (dxL + dxR);
+            gp[q] = .5 * (gpL + gpR);
+            if (dim == 1) {
+               gr[q] = .5 * (gd + gd);
+               gz[q] = .5 * (gradz[q * nt + cL] + gradz[q * nt + cR]);
+            } else {
+               gr[q] = .5 * (gradr[q * nt + cL] + gradr[q * nt + cR]);
+               gz[q] = .5 * (gd + gd);
+            }
+         }
+         add_visc<NQ, H>(P, pL, pR, gr, gp, gz, r, n, Flux);
+      }
+      double dAdt = F.dA[f] * dt;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) dF[q * nfs + f] = (Flux[q] - 0.0 * Ustr[q]) * dAdt;
+      if constexpr (mhd) {
+         double e[4];
+         H::flux_to_E(Flux, Ustr, r, dim, e);
+         emf_face[0 * nfs + f] = e[0];
+         emf_face[1 * nfs + f] = e[1];
+         emf_face[2 * nfs + f] = e[2];
+      }
+   }
+};
+
+// ---------------------------------------------------------------------------
+// Point-mass gravity (planet.c:56-196)
+// ---------------------------------------------------------------------------
+
+DHD double fgrav(double M, double r, double eps, int type)
+{
+   if (type == 0) {
+      double n = 2.0;
+      return M * pow(r, n - 1.) / pow(pow(r, n) + pow(eps, n), 1. + 1. / n);
+   } else if (type == 1) {
+      return M / ((r - 2 * M) * (r - 2 * M));
+   } else if (type == 2) {
+      return M;
+   }
+   return 0.0;
+}
+
+DHD void planet_src(const dh_params &P, const double *pl, const double *prim, double *cons,
+                    double rp, double rm, double phip, double phim, double zp, double zm,
+                    double dVdt, int polar, double Pmax)
+{
+   double rho = prim[RHO], vr = prim[URR], vz = prim[UZZ], omega = prim[UPP];
+   double dphi = get_dp(phip, phim, Pmax);
+   double phi = phim + 0.5 * dphi;
+   double r = centroid_r(rp, rm);
+   double z = centroid_z(zp, zm);
+   double rpl = pl[0], ppl = pl[1];
+   if (P.grav2d == 1) z = 0.0;
+   else if (P.grav2d == 2) { rpl = r; ppl = phi; z = 1.0; }
+   // sin and cos of one angle come from one sincos call, as the reference's
+   // optimised build computes them.
+   double sinp, cosp, sinpl, cospl;
+   sincos(phi, &sinp, &cosp);
+   sincos(ppl, &sinpl, &cospl);
+   double dx = r * cosp - rpl * cospl;
+   double dy = r * sinp - rpl * sinpl;
+   double script_r = sqrt(dx * dx + dy * dy + z * z);
+   double script_r_perp = sqrt(dx * dx + dy * dy);
+   double f1 = -fgrav(pl[2], script_r, pl[5], (int)pl[6]);
+   double cosa = dx / script_r_perp;
+   double sina = dy / script_r_perp;
+   double cosap = cosa * cosp + sina * sinp;
+   double sinap = sina * cosp - cosa * sinp;
+   if (script_r_perp <= 0.0) { cosap = 0.0; sinap = 0.0; }
+   double sint = script_r_perp / script_r;
+   double cost = z / script_r;
+   double F0 = cosap * f1 * sint, F1 = sinap * f1 * sint, F2 = f1 * cost;
+   if (polar) {
+      double adj = sin(0.5 * dphi) / (0.5 * dphi);
+      F0 *= adj;
+      F1 *= adj;
+   }
+   double hr = 1.0, hp = r, hz = 1.0;
+   cons[SRR] += rho * hr * F0 * dVdt;
+   cons[LLL] += rho * hp * F1 * dVdt;
+   cons[SZZ] += rho * hz * F2 * dVdt;
+   cons[TAU] += rho * (hr * F0 * vr + hz * F2 * vz + hp * F1 * omega) * dVdt;
+}
+
+// ---------------------------------------------------------------------------
+// Cell update: RK blend (misc.c:226-244), flux gather in the reference's
+// order (phi, then r, then z faces; each cell's inner pair before its outer
+// pair), sources (misc.c:538-580), mesh motion (misc.c:256-288) and, for
+// hydro, cons2prim (misc.c:290-325).  Ghost cells only move.
+// ---------------------------------------------------------------------------
+
+template <int NQ>
+DHD void gather_range(const double *dF, size_t nfs, int base, int lo, int hi, bool add,
+                      double *cons)
+{
+   for (int f = base + lo; f <= base + hi; ++f) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+         if (add) cons[q] += dF[q * nfs + f];
+         else cons[q] -= dF[q * nfs + f];
+      }
+   }
+}
+
+template <int NQ>
+DHD void gather_pairs(const Mesh &m, int dim, long c, int a, const int *fI, const int *rng,
+                      int nf, const double *dF, double *cons)
+{
+   CellRanges cr = load_ranges(m, dim, rng, fI, c, a);
+   size_t nfs = (size_t)nf;
+   if (cr.in) {
+      gather_range<NQ>(dF, nfs, cr.base_in, cr.in_h0, cr.in_h1, true, cons);
+      gather_range<NQ>(dF, nfs, cr.base_in, cr.in_t0, cr.in_t1, true, cons);
+   }
+   if (cr.out) {
+      gather_range<NQ>(dF, nfs, cr.base_out, cr.out_h0, cr.out_h1, false, cons);
+      gather_range<NQ>(dF, nfs, cr.base_out, cr.out_t0, cr.out_t1, false, cons);
+   }
+}
+
+template <int NQ, class H>
+struct KUpdate {
+   Mesh m;
+   dh_params P;
+   double RK, dt;
+   int first_step;
+   PlanetSet pls;
+   const double *prim_in;
+   double *prim, *cons;
+   const double *rkcons, *gradr, *gradp, *gradz, *piph, *dphi, *wiph;
+   double *piph_out, *dphi_out;
+   const double *dFp, *dFr, *dFz;
+   const int *fIr, *fIz, *rngr, *rngz;
+   int nfr, nfz;
+   DHD void operator()(long c) const
+   {
+      constexpr bool mhd = IsMHD<H>::value;
+      size_t nt = (size_t)m.ntot;
+      int a = m.cjk[c];
+      int c0 = m.off[a], np = m.Np[a];
+      int i = (int)(c - c0);
+      int j = a % m.Nr, k = a / m.Nr;
+      long cm = i == 0 ? c0 + np - 1 : c - 1;
+      double Pmax = m.Pmax;
+
+      // Mesh motion of this cell and of its phi neighbour: move_cells on the
+      // first substep only, clean_pi and calc_dp on every substep.
+      double pp = piph[c], pm = piph[cm];
+      if (first_step) {
+         pp += wiph[c] * dt;
+         pm += wiph[cm] * dt;
+      }
+      pp = clean_phi(pp, Pmax);
+      pm = clean_phi(pm, Pmax);
+      double dp_new = get_dp(pp, pm, Pmax);
+      piph_out[c] = pp;
+      dphi_out[c] = dp_new;
+      if (!is_real(m, j, k)) return;
+
+      double cs[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+         double v = cons[q * nt + c];
+         if (RK != 0.0) v = (1. - RK) * v + RK * rkcons[q * nt + c];
+         cs[q] = v;
+      }
+      // phi faces: cell i is R of face i-1 and L of face i; the reference
+      // reaches face i-1 first except for i = 0, whose R face is the last one.
+      if (i == 0) {
+#pragma unroll
+         for (int q = 0; q < NQ; ++q) cs[q] -= dFp[q * nt + c];
+#pragma unroll
+         for (int q = 0; q < NQ; ++q) cs[q] += dFp[q * nt + cm];
+      } else {
+#pragma unroll
+         for (int q = 0; q < NQ; ++q) cs[q] += dFp[q * nt + cm];
+#pragma unroll
+         for (int q = 0; q < NQ; ++q) cs[q] -= dFp[q * nt + c];
+      }
+      if (m.Nr > 1) gather_pairs<NQ>(m, 1, c, a, fIr, rngr, nfr, dFr, cs);
+      if (m.Nz > 1) gather_pairs<NQ>(m, 2, c, a, fIz, rngz, nfz, dFz, cs);
+
+      // Sources at the cell's position before the move.
+      double rp = m.r_jph[j], rm = m.r_jph[j - 1];
+      double zp = m.z_kph[k], zm = m.z_kph[k - 1];
+      double phip = piph[c];
+      double phim = phip - dphi[c];
+      double dV = cell_dV(rp, rm, phip, phim, zp, zm, Pmax);
+      double dVdt = dV * dt;
+      int polar = mhd ? (m.NgRa == 0) : (P.nobc_rmin == 1);
+      double pr[NQ];
+      load_q<NQ>(prim_in, nt, c, pr);
+      H::source(P, pr, cs, rp, rm, phip, phim, zp, zm, dVdt, polar, Pmax);
+      for (int p = 0; p < pls.n; ++p)
+         planet_src(P, pls.v + p * DH_PLANET_STRIDE, pr, cs, rp, rm, phip, phim, zp, zm, dVdt,
+                    P.nobc_rmin == 1, Pmax);
+      if (P.visc_flag) {
+         double gr[NQ], gp[NQ], gz[NQ];
+         load_q<NQ>(gradr, nt, c, gr);
+         load_q<NQ>(gradp, nt, c, gp);
+         load_q<NQ>(gradz, nt, c, gz);
+         H::visc_source(P, pr, gr, gp, gz, cs, centroid_r(rp, rm), dVdt);
+      }
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) cons[q * nt + c] = cs[q];
+      if constexpr (!mhd) {
+         // cons2prim at the moved position.
+         double r = centroid_r(rp, rm);
+         double dVn = cell_dV(rp, rm, pp, pp - dp_new, zp, zm, Pmax);
+         double out[NQ];
+         H::cons2prim(P, cs, out, r, dVn);
+#pragma unroll
+         for (int q = 0; q < NQ; ++q) prim[q * nt + c] = out[q];
+      }
+   }
+};
+
+// ---------------------------------------------------------------------------
+// Constrained transport (CT_MODE 2), see dh_ct.h for the state layout.
+// Edge slots per cell: E[0..3] z-directed at (r-,phi+), (r+,phi+), (r-,phi-),
+// (r+,phi-); E[4..7] r-directed at (z-,phi+), (z+,phi+), (z-,phi-), (z+,phi-).
+// Ed[s*nt+c] holds E[s], Ed[(8+s)*nt+c] the matching normal field B[s].
+// ---------------------------------------------------------------------------
+
+// Edge EMFs of every cell after the Riemann solves (riemann.c:66-211): the
+// phi faces set E[0..7] (= half their EMF), then every r and z face of the
+// flux range adds half its EMF, weighted by its phi extent over the cell's.
+// Cells outside the phi-flux range keep the values they ended the previous
+// substep with (Ecarry), as the reference's cell arrays do.
+struct KEdgesCollect {
+   Mesh m;
+   int ct;
+   const double *dphi, *emf_phi;
+   Faces Fr, Fz;
+   const int *rngr, *rngz;
+   const double *emf_r, *emf_z, *Ecarry;
+   double *Epre;
+   DHD void operator()(long c) const
+   {
+      size_t nt = (size_t)m.ntot;
+      int a = m.cjk[c];
+      int c0 = m.off[a], np = m.Np[a];
+      long cm = c == c0 ? c0 + np - 1 : c - 1;
+      int j = a % m.Nr, k = a / m.Nr;
+      double E[8], B[8];
+      if (phi_in_range(m, ct, j, k)) {
+         double Ez = emf_phi[0 * nt + c], Br = emf_phi[1 * nt + c];
+         double Er = emf_phi[2 * nt + c], Bz = emf_phi[3 * nt + c];
+         double Ezm = emf_phi[0 * nt + cm], Brm = emf_phi[1 * nt + cm];
+         double Erm = emf_phi[2 * nt + cm], Bzm = emf_phi[3 * nt + cm];
+         E[0] = E[1] = .5 * Ez;
+         E[2] = E[3] = .5 * Ezm;
+         B[0] = B[1] = .5 * Br;
+         B[2] = B[3] = .5 * Brm;
+         E[4] = E[5] = .5 * Er;
+         E[6] = E[7] = .5 * Erm;
+         B[4] = B[5] = .5 * Bz;
+         B[6] = B[7] = .5 * Bzm;
+      } else {
+#pragma unroll
+         for (int s = 0; s < 8; ++s) {
+            E[s] = Ecarry[s * nt + c];
+            B[s] = Ecarry[(8 + s) * nt + c];
+         }
+      }
+      double dpc = dphi[c];
+      for (int dim = 1; dim <= 2; ++dim) {
+         if (!dim_active(m, dim)) continue;
+         const Faces &F = dim == 1 ? Fr : Fz;
+         const double *em = dim == 1 ? emf_r : emf_z;
+         size_t nfs = (size_t)F.n;
+         CellRanges cr = load_ranges(m, dim, dim == 1 ? rngr : rngz, F.fI, c, a);
+         // as R of the inner pair: edges 0,2 (r) or 4,6 (z);
+         // as L of the outer pair: edges 1,3 (r) or 5,7 (z).
+         int i0 = dim == 1 ? 0 : 4, o0 = dim == 1 ? 1 : 5;
+         auto acc = [&](int base, int lo, int hi, int e0) {
+            for (int f = base + lo; f <= base + hi; ++f) {
+               double frac = F.dphi[f] / dpc;
+               double de = .5 * em[0 * nfs + f] * frac;
+               double db = .5 * em[1 * nfs + f] * frac;
+               E[e0] += de;
+               E[e0 + 2] += de;
+               B[e0] += db;
+               B[e0 + 2] += db;
+            }
+         };
+         if (cr.in) {
+            int pj = dim == 1 ? j - 1 : j, pk = dim == 1 ? k : k - 1;
+            if (pair_has_flux(m, ct, dim, pj, pk)) {
+               acc(cr.base_in, cr.in_h0, cr.in_h1, i0);
+               acc(cr.base_in, cr.in_t0, cr.in_t1, i0);
+            }
+         }
+         if (cr.out && pair_has_flux(m, ct, dim, j, k)) {
+            acc(cr.base_out, cr.out_h0, cr.out_h1, o0);
+            acc(cr.base_out, cr.out_t0, cr.out_t1, o0);
+         }
+      }
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+         Epre[s * nt + c] = E[s];
+         Epre[(8 + s) * nt + c] = B[s];
+      }
+   }
+};
+
+// avg_Efields, first part (bfields.c:400-441): each phi-edge takes the mean
+// of the values its two cells hold.
+struct KEdgesAvg {
+   Mesh m;
+   const double *Epre;
+   double *Eavg;
+   DHD void operator()(long c) const
+   {
+      size_t nt = (size_t)m.ntot;
+      int a = m.cjk[c];
+      int c0 = m.off[a], np = m.Np[a];
+      long cp = c - c0 == np - 1 ? c0 : c + 1;
+      long cm = c == c0 ? c0 + np - 1 : c - 1;
+      for (int h = 0; h < 2; ++h) {  // h = 0: E, h = 1: B
+         const double *X = Epre + (size_t)h * 8 * nt;
+         double *Y = Eavg + (size_t)h * 8 * nt;
+         for (int s = 0; s < 2; ++s) {  // s = 0: z-directed edges, 1: r-directed
+            int e0 = 4 * s, e1 = e0 + 1, e2 = e0 + 2, e3 = e0 + 3;
+            Y[e0 * nt + c] = .5 * (X[e0 * nt + c] + X[e2 * nt + cp]);
+            Y[e1 * nt + c] = .5 * (X[e1 * nt + c] + X[e3 * nt + cp]);
+            Y[e2 * nt + c] = .5 * (X[e0 * nt + cm] + X[e2 * nt + c]);
+            Y[e3 * nt + c] = .5 * (X[e1 * nt + cm] + X[e3 * nt + c]);
+         }
+      }
+   }
+};
+
+// avg_Efields, second part (bfields.c:443-531): the edge at a face's upper
+// end is the mean of its owner's value and the other cell's two corner
+// values interpolated in phi.
+struct KFaceEmf {
+   Mesh m;
+   int dim;
+   Faces F;
+   const double *piph, *dphi, *Eavg;
+   double *fE, *fB;
+   DHD void operator()(long f) const
+   {
+      size_t nt = (size_t)m.ntot;
+      const double *E = Eavg, *B = Eavg + 8 * nt;
+      int L = F.L[f], R = F.R[f];
+      int LR = F.LR[f];
+      int c1 = LR == 0 ? L : R, c2 = LR == 0 ? R : L;
+      double dp1 = get_dp(piph[c2], piph[c1], m.Pmax);
+      double dp2 = dphi[c2] - dp1;
+      int eL = dim == 1 ? 1 : 5, eR = dim == 1 ? 0 : 4;
+      if (LR == 0) {
+         double Ea = (dp2 * E[eR * nt + c2] + dp1 * E[(eR + 2) * nt + c2]) / (dp1 + dp2);
+         double Ba = (dp2 * B[eR * nt + c2] + dp1 * B[(eR + 2) * nt + c2]) / (dp1 + dp2);
+         fE[f] = .5 * (E[eL * nt + L] + Ea);
+         fB[f] = .5 * (B[eL * nt + L] + Ba);
+      } else {
+         double Ea = (dp2 * E[eL * nt + c2] + dp1 * E[(eL + 2) * nt + c2]) / (dp1 + dp2);
+         double Ba = (dp2 * B[eL * nt + c2] + dp1 * B[(eL + 2) * nt + c2]) / (dp1 + dp2);
+         fE[f] = .5 * (E[eR * nt + R] + Ea);
+         fB[f] = .5 * (B[eR * nt + R] + Ba);
+      }
+   }
+};
+
+// The four phi+ edges of a cell after avg_Efields, before the advective
+// subtraction: {E0, E1, E4, E5} then {B0, B1, B4, B5}.  An edge bounding
+// the face the cell owns takes that face's value (bfields.c:533-560).
+struct EdgeArgs {
+   Mesh m;
+   Faces Fr, Fz;
+   const int *rngr, *rngz;
+   const double *Eavg, *fEr, *fBr, *fEz, *fBz;
+};
+
+DHD void edges_post_face(const EdgeArgs &g, long c, double *out)
+{
+   const Mesh &m = g.m;
+   size_t nt = (size_t)m.ntot;
+   int a = m.cjk[c];
+   const double *E = g.Eavg, *B = g.Eavg + 8 * nt;
+   out[0] = E[0 * nt + c]; out[1] = E[1 * nt + c]; out[2] = E[4 * nt + c]; out[3] = E[5 * nt + c];
+   out[4] = B[0 * nt + c]; out[5] = B[1 * nt + c]; out[6] = B[4 * nt + c]; out[7] = B[5 * nt + c];
+   if (m.Nr > 1) {
+      CellRanges cr = load_ranges(m, 1, g.rngr, g.Fr.fI, c, a);
+      if (cr.in) { int f = cr.base_in + cr.in_h1; out[0] = g.fEr[f]; out[4] = g.fBr[f]; }
+      if (cr.out) { int f = cr.base_out + cr.out_h1; out[1] = g.fEr[f]; out[5] = g.fBr[f]; }
+   }
+   if (m.Nz > 1) {
+      CellRanges cr = load_ranges(m, 2, g.rngz, g.Fz.fI, c, a);
+      if (cr.in) { int f = cr.base_in + cr.in_h1; out[2] = g.fEz[f]; out[6] = g.fBz[f]; }
+      if (cr.out) { int f = cr.base_out + cr.out_h1; out[3] = g.fEz[f]; out[7] = g.fBz[f]; }
+   }
+}
+
+// Owner edges with the mesh-shear correction E -= h w B
+// (subtract_advective_B_fluxes, bfields.c:646-690): Eo = {E0, E1, E4, E5}.
+// Also E_phi of each owned face, Eown = {r-in, r-out, z-in, z-out}, and the
+// full edge state of cells outside the phi-flux range (Ecarry).
+struct KEdgesFinal {
+   EdgeArgs g;
+   int ct;
+   const double *wiph, *emf_r, *emf_z;
+   double *Eo, *Eown, *Ecarry;
+   DHD void operator()(long c) const
+   {
+      const Mesh &m = g.m;
+      size_t nt = (size_t)m.ntot;
+      int a = m.cjk[c];
+      int j = a % m.Nr, k = a / m.Nr;
+      double v[8];
+      edges_post_face(g, c, v);
+      double own[4] = {0.0, 0.0, 0.0, 0.0};
+      if (m.Nr > 1) {
+         size_t nfs = (size_t)g.Fr.n;
+         CellRanges cr = load_ranges(m, 1, g.rngr, g.Fr.fI, c, a);
+         if (cr.in) own[0] = emf_r[2 * nfs + cr.base_in + cr.in_h1];
+         if (cr.out) own[1] = emf_r[2 * nfs + cr.base_out + cr.out_h1];
+      }
+      if (m.Nz > 1) {
+         size_t nfs = (size_t)g.Fz.n;
+         CellRanges cr = load_ranges(m, 2, g.rngz, g.Fz.fI, c, a);
+         if (cr.in) own[2] = emf_z[2 * nfs + cr.base_in + cr.in_h1];
+         if (cr.out) own[3] = emf_z[2 * nfs + cr.base_out + cr.out_h1];
+      }
+      double xc0 = centroid_r(m.r_jph[j], m.r_jph[j - 1]);
+      double hL = m.r_jph[j - 1], hR = m.r_jph[j], hD = xc0, hU = xc0;
+      double w = wiph[c];
+      double E0 = v[0], E1 = v[1], E4 = v[2], E5 = v[3];
+      E0 -= hL * w * v[4];
+      E1 -= hR * w * v[5];
+      E4 += hD * w * v[6];
+      E5 += hU * w * v[7];
+      Eo[0 * nt + c] = E0;
+      Eo[1 * nt + c] = E1;
+      Eo[2 * nt + c] = E4;
+      Eo[3 * nt + c] = E5;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) Eown[s * nt + c] = own[s];
+      if (!phi_in_range(m, ct, j, k)) {
+         // The phi- corner slots were copied from the lower phi neighbour
+         // before the subtraction (bfields.c:631-644).
+         int c0 = m.off[a], np = m.Np[a];
+         long cm = c == c0 ? c0 + np - 1 : c - 1;
+         double vm[8];
+         edges_post_face(g, cm, vm);
+         double Ec[8] = {E0, E1, vm[0], vm[1], E4, E5, vm[2], vm[3]};
+         double Bc[8] = {v[4], v[5], vm[4], vm[5], v[6], v[7], vm[6], vm[7]};
+#pragma unroll
+         for (int s = 0; s < 8; ++s) {
+            Ecarry[s * nt + c] = Ec[s];
+            Ecarry[(8 + s) * nt + c] = Bc[s];
+         }
+      }
+   }
+};
+
+// The edge EMF of the face a cell owns, by the face's orientation.
+DHD double owner_edge(const double *Eo, size_t nt, int dim, int LR, int L, int R)
+{
+   if (dim == 1) return LR == 0 ? Eo[1 * nt + L] : Eo[0 * nt + R];
+   return LR == 0 ? Eo[3 * nt + L] : Eo[2 * nt + R];
+}
+
+// update_B_fluxes (bfields.c:313-382) per cell: the RK blend of Phi
+// (misc.c:238-240), then E dl dt of the edges that bound the cell's own
+// faces, in the reference's order.
+struct KUpdateB {
+   Mesh m;
+   double RK, dt;
+   Faces Fr, Fz;
+   const int *rngr, *rngz;
+   const double *Eo;
+   double *Phi;
+   const double *rkPhi;
+   DHD void operator()(long c) const
+   {
+      size_t nt = (size_t)m.ntot;
+      int a = m.cjk[c];
+      int j = a % m.Nr, k = a / m.Nr;
+      double ph[5];
+#pragma unroll
+      for (int s = 0; s < 5; ++s) {
+         double v = Phi[s * nt + c];
+         if (RK != 0.0) v = (1. - RK) * v + RK * rkPhi[s * nt + c];
+         ph[s] = v;
+      }
+      for (int dim = 1; dim <= 2; ++dim) {
+         if (!dim_active(m, dim)) continue;
+         const Faces &F = dim == 1 ? Fr : Fz;
+         CellRanges cr = load_ranges(m, dim, dim == 1 ? rngr : rngz, F.fI, c, a);
+         // r faces: Phi[own] -= E dl dt, the next face's Phi += E dl dt,
+         // the owner's phi face +/-; z faces the same with the sign flipped.
+         double sg = dim == 1 ? 1.0 : -1.0;
+         for (int side = 0; side < 2; ++side) {
+            if (side == 0 ? !cr.in : !cr.out) continue;
+            int pj = j, pk = k;
+            if (side == 0) {
+               if (dim == 1) pj = j - 1;
+               else pk = k - 1;
+            }
+            int JK = pair_index(m, dim, pj, pk);
+            if (track_skipped(F.npairs, JK)) continue;
+            int base = side == 0 ? cr.base_in : cr.base_out;
+            int nfp = F.fI[JK + 1] - base;
+            int own = side == 0 ? cr.in_h1 : cr.out_h1;
+            int prev = own == 0 ? nfp - 1 : own - 1;
+            int fo = base + own, fp = base + prev;
+            double dl = dim == 1 ? m.z_kph[pk] - m.z_kph[pk - 1] : m.r_jph[pj] - m.r_jph[pj - 1];
+            double Eown = owner_edge(Eo, nt, dim, F.LR[fo], F.L[fo], F.R[fo]);
+            double Eprev = owner_edge(Eo, nt, dim, F.LR[fp], F.L[fp], F.R[fp]);
+            double x_own = sg * Eown * dl * dt, x_prev = sg * Eprev * dl * dt;
+            int s = dim == 1 ? (side == 0 ? 1 : 2) : (side == 0 ? 3 : 4);
+            if (own == 0) {
+               ph[s] -= x_own;
+               ph[s] += x_prev;
+            } else {
+               ph[s] += x_prev;
+               ph[s] -= x_own;
+            }
+            if (side == 0) ph[0] -= x_own;
+            else ph[0] += x_own;
+         }
+      }
+#pragma unroll
+      for (int s = 0; s < 5; ++s) Phi[s * nt + c] = ph[s];
+   }
+};
+
+// make_edge_adjust (bfields.c:888-1127) per face.  The reference walks the
+// four rings around each phi-directed edge line; each step is the overlap
+// of the current faces on either side of the line, so a face finds its own
+// steps by a merge path between its list and the facing list, and adds
+// +-E dl dt for each, in the reference's line and step order.
+//   r face of pair (j,k): U role on line (j,k-1) [-], D role on line (j,k) [+]
+//   z face of pair (j,k): R role on line (j-1,k) [+], L role on line (j,k) [-]
+struct KEdgeAdjust {
+   Mesh m;
+   double dt;
+   Faces Fr, Fz;
+   const double *piph, *dphi, *Eown;
+   double *Phi;
+   DHD void operator()(long t) const
+   {
+      size_t nt = (size_t)m.ntot;
+      int dim = t < Fr.n ? 1 : 2;
+      const Faces &F = dim == 1 ? Fr : Fz;
+      int f = (int)(dim == 1 ? t : t - Fr.n);
+      int JK = pair_of_face(F.fI, F.npairs, f);
+      Pair pr = pair_of(m, dim, JK);
+      int j = pr.j, k = pr.k;
+      int base = F.fI[JK];
+      int x = f - base;
+      FaceRing self = make_face_ring(piph, F.L, F.R, F.LR, base, F.fI[JK + 1] - base);
+      size_t slot = phi_slot(nt, dim, F.LR[f], F.L[f], F.R[f]);
+      double v = Phi[slot];
+      for (int role = 0; role < 2; ++role) {
+         int oJK, lj;
+         bool self_first;  // self is the first (D or L) list of the merge
+         double sgn;
+         if (dim == 1) {
+            if (role == 0) {
+               if (k == 0) continue;
+               oJK = pair_index(m, 1, j, k - 1); self_first = false; sgn = -1.0;
+            } else {
+               if (k >= m.Nz - 1) continue;
+               oJK = pair_index(m, 1, j, k + 1); self_first = true; sgn = +1.0;
+            }
+            lj = j;
+         } else {
+            if (role == 0) {
+               if (j == 0) continue;
+               oJK = pair_index(m, 2, j - 1, k); self_first = false; sgn = +1.0;
+               lj = j - 1;
+            } else {
+               if (j >= m.Nr - 1) continue;
+               oJK = pair_index(m, 2, j + 1, k); self_first = true; sgn = -1.0;
+               lj = j;
+            }
+         }
+         int obase = F.fI[oJK];
+         FaceRing other = make_face_ring(piph, F.L, F.R, F.LR, obase, F.fI[oJK + 1] - obase);
+         FaceRange rg = self_first ? ranges_as_a(self, x, other, m.Pmax)
+                                   : ranges_as_b(other, self, x, m.Pmax);
+         double rline = m.r_jph[lj];
+         for (int part = 0; part < 2; ++part) {
+            int lo = part == 0 ? rg.h0 : rg.t0, hi = part == 0 ? rg.h1 : rg.t1;
+            for (int e = lo; e <= hi; ++e) {
+               int g;  // the facing face current at step e
+               if (self_first) {
+                  int y = e - merge_x(self, other, e, m.Pmax);
+                  g = obase + (y == other.N ? 0 : y);
+               } else {
+                  int xo = merge_x(other, self, e, m.Pmax);
+                  g = obase + (xo == other.N ? 0 : xo);
+               }
+               int f1 = self_first ? f : g, f2 = self_first ? g : f;
+               EdgeCells q;
+               if (dim == 1) {  // f1 = D face (c, cR), f2 = U face (cU, cUR)
+                  q.c = F.L[f1]; q.cR = F.R[f1]; q.cU = F.L[f2]; q.cUR = F.R[f2];
+               } else {         // f1 = L face (c, cU), f2 = R face (cR, cUR)
+                  q.c = F.L[f1]; q.cU = F.R[f1]; q.cR = F.L[f2]; q.cUR = F.R[f2];
+               }
+               double E, dl;
+               edge_segment(piph, dphi, Eown, nt, q, rline, m.Pmax, &E, &dl);
+               double Edldt = E * dl * dt;
+               if (sgn > 0) v += Edldt;
+               else v -= Edldt;
+            }
+         }
+      }
+      Phi[slot] = v;
+   }
+};
+
+// check_flipped (bfields.c:693-712): after the mesh moved, a face whose
+// upper end now lies before its predecessor's has vanished.  Positions are
+// the moved ones before clean_pi, as the reference has them.
+struct KFlipFlags {
+   Mesh m;
+   int dim;
+   Faces F;
+   const double *piph, *wiph;
+   double dt;
+   signed char *flag;
+   DHD double moved_up(int f) const
+   {
+      int o = F.LR[f] == 0 ? F.L[f] : F.R[f];
+      return piph[o] + wiph[o] * dt;
+   }
+   DHD void operator()(long f) const
+   {
+      int JK = pair_of_face(F.fI, F.npairs, (int)f);
+      int base = F.fI[JK], N = F.fI[JK + 1] - base;
+      int n = (int)f - base;
+      int fm = base + (n == 0 ? N - 1 : n - 1);
+      flag[f] = get_sdp(moved_up((int)f), moved_up(fm), m.Pmax) < 0. ? 1 : 0;
+   }
+};
+
+// flip_fluxes for flagged face n of a track (bfields.c:742-814): its flux
+// joins its predecessor's, the predecessor's is handed on negated, and both
+// faces change owner.
+DHD void flip_one(int dim, size_t nt, Faces F, int base, int N, int n, double *Phi,
+                  double *rkPhi)
+{
+   int f = base + n;
+   int fm = base + (n == 0 ? N - 1 : n - 1);
+   int fp = base + (n == N - 1 ? 0 : n + 1);
+   size_t P = phi_slot(nt, dim, F.LR[f], F.L[f], F.R[f]);
+   size_t Pm = phi_slot(nt, dim, F.LR[fm], F.L[fm], F.R[fm]);
+   size_t Pp = phi_slot(nt, dim, F.LR[fp], F.L[fp], F.R[fp]);
+   double ph = Phi[P], rk = rkPhi[P];
+   Phi[P] = Phi[Pm] + ph;
+   rkPhi[P] = rkPhi[Pm] + rk;
+   Phi[Pm] = -ph;
+   rkPhi[Pm] = -rk;
+   Phi[Pp] = Phi[Pp] + ph;
+   rkPhi[Pp] = rkPhi[Pp] + rk;
+   F.LR[f] = (signed char)!F.LR[f];
+   F.LR[fm] = (signed char)!F.LR[fm];
+}
+
+// One face track, sequentially (the host form of the wavefront kernel).
+struct KFlipsTrack {
+   Mesh m;
+   int dim;
+   Faces F;
+   const signed char *flag;
+   double *Phi, *rkPhi;
+   DHD void operator()(long JK) const
+   {
+      if (track_skipped(F.npairs, (int)JK)) return;
+      int base = F.fI[JK], N = F.fI[JK + 1] - base;
+      for (int n = 0; n < N; ++n)
+         if (flag[base + n]) flip_one(dim, (size_t)m.ntot, F, base, N, n, Phi, rkPhi);
+   }
+};
+
+// B_faces_to_cells (bfields.c:88-309) and, for real cells, cons2prim
+// (misc.c:290-325): cell B from the fluxes through the cell's faces
+// (ownership after the flips) -- the primitive field (type 0) or the
+// conserved one over the moved geometry (type 1).
+template <int NQ, class H>
+struct KBfacesC2p {
+   Mesh m;
+   dh_params P;
+   Faces Fr, Fz;
+   const int *rngr, *rngz;
+   const double *piph_new, *dphi_new, *Phi;
+   double *cons, *prim;
+   int type;    // 0: primitive B (set_B_fields), 1: conserved B
+   int do_c2p;  // cons2prim of real cells afterwards
+   DHD void operator()(long c) const
+   {
+      size_t nt = (size_t)m.ntot;
+      int a = m.cjk[c];
+      int c0 = m.off[a], np = m.Np[a];
+      long cm = c == c0 ? c0 + np - 1 : c - 1;
+      int j = a % m.Nr, k = a / m.Nr;
+      double rp = m.r_jph[j], rm = m.r_jph[j - 1];
+      double zp = m.z_kph[k], zm = m.z_kph[k - 1];
+      double x0 = centroid_r(rp, rm);
+      double dV = cell_dV(rp, rm, piph_new[c], piph_new[cm], zp, zm, m.Pmax);
+      double dA0 = (rp - rm) * (zp - zm);
+      double Br = 0.0, Bz = 0.0;
+      for (int dim = 1; dim <= 2; ++dim) {
+         if (!dim_active(m, dim)) continue;
+         const Faces &F = dim == 1 ? Fr : Fz;
+         CellRanges cr = load_ranges(m, dim, dim == 1 ? rngr : rngz, F.fI, c, a);
+         double sum = 0.0, area = 0.0;
+         auto add = [&](int base, int lo, int hi) {
+            for (int f = base + lo; f <= base + hi; ++f) {
+               area += F.dA[f];
+               sum += Phi[phi_slot(nt, dim, F.LR[f], F.L[f], F.R[f])];
+            }
+         };
+         if (cr.in) {
+            add(cr.base_in, cr.in_h0, cr.in_h1);
+            add(cr.base_in, cr.in_t0, cr.in_t1);
+         }
+         if (cr.out) {
+            add(cr.base_out, cr.out_h0, cr.out_h1);
+            add(cr.base_out, cr.out_t0, cr.out_t1);
+         }
+         if (type == 0) {
+            if (dim == 1) Br = sum / area;
+            else Bz = sum / area;
+         } else if (dim == 1) {
+            Br = sum * (dV * (1.0 / x0) / (area * 1.0));
+         } else {
+            Bz = sum * (dV * 1.0 / (area * 1.0));
+         }
+      }
+      if (type == 0) {
+         prim[BRR * nt + c] = Br;
+         prim[BPP * nt + c] = .5 * (Phi[0 * nt + c] + Phi[0 * nt + cm]) / dA0;
+         prim[BZZ * nt + c] = Bz;
+         return;
+      }
+      double Bp = .5 * (Phi[0 * nt + c] + Phi[0 * nt + cm]) * dV / (dA0 * x0);
+      cons[BRR * nt + c] = Br;
+      cons[BPP * nt + c] = Bp;
+      cons[BZZ * nt + c] = Bz;
+      if (!do_c2p || !is_real(m, j, k)) return;
+      double cs[NQ], out[NQ];
+      load_q<NQ>(cons, nt, c, cs);
+      double dVc = cell_dV(rp, rm, piph_new[c], piph_new[c] - dphi_new[c], zp, zm, m.Pmax);
+      H::cons2prim(P, cs, out, x0, dVc);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) prim[q * nt + c] = out[q];
+   }
+};
+
+// ---------------------------------------------------------------------------
+// Zero-gradient ghost annuli, one line of ghost annuli per thread, in the
+// reference's order (boundary_functions.c:90-159, 269-441).
+//   side 0: r inner, 1: r outer, 2: z bottom, 3: z top.
+// ---------------------------------------------------------------------------
+
+template <int NQ>
+DHD void copy_distant(const Mesh &m, double *prim, const double *piph, const double *dphi,
+                      int dst, int src)
+{
+   size_t nt = (size_t)m.ntot;
+   double pm = m.Pmax;
+   int c0 = m.off[dst], s0 = m.off[src];
+   int Np = m.Np[dst], Np1 = m.Np[src];
+   int i1;
+   double phi0 = piph[c0] - dphi[c0];
+   for (i1 = 0; i1 < Np1; i1++) {
+      double dphip = get_sdp(piph[s0 + i1], phi0, pm);
+      double dphim = dphip - dphi[s0 + i1];
+      if (dphip > 0 && dphim <= 0.0) break;
+   }
+   if (i1 >= Np1) i1 = 0;
+   for (int i = 0; i < Np; ++i) {
+      double phip = piph[c0 + i];
+      double phim = piph[c0 + i] - dphi[c0 + i];
+      double phip1, phim1;
+      double acc[NQ];
+      for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+      int guard = 0;
+      do {
+         phip1 = piph[s0 + i1];
+         phim1 = piph[s0 + i1] - dphi[s0 + i1];
+         if (std::isfinite(phip1) && std::isfinite(phim1) && std::isfinite(phip) && std::isfinite(phim)) {
+            while (phip1 < phim) { phip1 += pm; phim1 += pm; }
+            while (phim1 > phip) { phip1 -= pm; phim1 -= pm; }
+         }
+         double phi1 = phip < phip1 ? phip : phip1;
+         double phi2 = phim > phim1 ? phim : phim1;
+         double d = phi1 - phi2;
+         for (int q = 0; q < NQ; ++q) acc[q] += prim[q * nt + s0 + i1] * d;
+         i1 = i1 < Np1 - 1 ? i1 + 1 : 0;
+      } while (phip1 < phip && ++guard <= Np1);
+      i1 = i1 > 0 ? i1 - 1 : Np1 - 1;
+      for (int q = 0; q < NQ; ++q) prim[q * nt + c0 + i] = acc[q] / dphi[c0 + i];
+   }
+}
+
+template <int NQ>
+struct KBcCopy {
+   Mesh m;
+   int side, diode;
+   double *prim;
+   const double *piph, *dphi;
+   DHD void operator()(long t) const
+   {
+      size_t nt = (size_t)m.ntot;
+      int ng = side == 0 ? m.NgRa : side == 1 ? m.NgRb : side == 2 ? m.NgZa : m.NgZb;
+      int q_d = side < 2 ? URR : UZZ;
+      int line = (int)t;
+      for (int g = 0; g < ng; ++g) {
+         int dst, src;
+         double sgn;  // diode: inflow velocity zeroed
+         if (side == 0) { int j = m.NgRa - 1 - g; dst = j + m.Nr * line; src = dst + 1; sgn = 1.0; }
+         else if (side == 1) { int j = m.Nr - m.NgRb + g; dst = j + m.Nr * line; src = dst - 1; sgn = -1.0; }
+         else if (side == 2) { int k = m.NgZa - 1 - g; dst = line + m.Nr * k; src = dst + m.Nr; sgn = 1.0; }
+         else { int k = m.Nz - m.NgZb + g; dst = line + m.Nr * k; src = dst - m.Nr; sgn = -1.0; }
+         copy_distant<NQ>(m, prim, piph, dphi, dst, src);
+         if (diode)
+            for (int i = 0; i < m.Np[dst]; ++i) {
+               double *v = &prim[q_d * nt + m.off[dst] + i];
+               if (sgn * *v > 0) *v = 0.0;
+            }
+      }
+   }
+};
+
+// ---------------------------------------------------------------------------
+// Initial cons from prim on real cells (domain.c:175-237); hydro also maps
+// back to prim there.
+// ---------------------------------------------------------------------------
+
+template <int NQ, class H>
+struct KInitCells {
+   Mesh m;
+   dh_params P;
+   double *prim, *cons;
+   const double *piph, *dphi;
+   DHD void operator()(long c) const
+   {
+      int a = m.cjk[c];
+      int j = a % m.Nr, k = a / m.Nr;
+      if (!is_real(m, j, k)) return;
+      size_t nt = (size_t)m.ntot;
+      double rp = m.r_jph[j], rm = m.r_jph[j - 1];
+      double zp = m.z_kph[k], zm = m.z_kph[k - 1];
+      double phip = piph[c];
+      double phim = phip - dphi[c];
+      double dV = cell_dV(rp, rm, phip, phim, zp, zm, m.Pmax);
+      double r = centroid_r(rp, rm);
+      double pr[NQ], cs[NQ];
+      load_q<NQ>(prim, nt, c, pr);
+      H::prim2cons(P, pr, cs, r, dV);
+      if constexpr (!IsMHD<H>::value) H::cons2prim(P, cs, pr, r, dV);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+         cons[q * nt + c] = cs[q];
+         prim[q * nt + c] = pr[q];
+      }
+   }
+};
+
+// CFL time of one real cell (misc.c:39-84); HUGE_VAL for ghosts.
+template <int NQ, class H>
+DHD double cell_mindt(const Mesh &m, const dh_params &P, const double *prim, const double *piph,
+                      const double *dphi, const double *wiph, long c)
+{
+   int a = m.cjk[c];
+   int j = a % m.Nr, k = a / m.Nr;
+   if (!is_real(m, j, k)) return HUGE_VAL;
+   size_t nt = (size_t)m.ntot;
+   int c0 = m.off[a], np = m.Np[a];
+   long cm = (c == c0) ? c0 + np - 1 : c - 1;
+   double w = .5 * (wiph[cm] + wiph[c]);
+   double phip = piph[c];
+   double phim = phip - dphi[c];
+   double pr[NQ];
+   load_q<NQ>(prim, nt, c, pr);
+   return H::mindt(P, pr, w, m.r_jph[j], m.r_jph[j - 1], phip, phim, m.z_kph[k], m.z_kph[k - 1],
+                   m.Pmax);
+}
+
+}  // namespace dh

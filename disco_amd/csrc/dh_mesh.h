@@ -53,19 +53,58 @@ DHD int ring_cell(const Ring &R, int x)
    return R.off + i;
 }
 
+// A face list of one annulus pair read as a ring: element x (0 <= x <= N) is
+// face f0 + (x mod N); its key is the face's upper phi edge, the upper edge
+// of whichever of its two cells ends first (faces_alt.c:147-151).
+struct FaceRing {
+   const double *piph;
+   const int *fL, *fR;
+   const signed char *fLR;
+   int f0, N;
+   double p0;
+};
+
+DHD double face_up(const double *piph, const int *fL, const int *fR, const signed char *fLR,
+                   int f)
+{
+   return fLR[f] == 0 ? piph[fL[f]] : piph[fR[f]];
+}
+
+DHD FaceRing make_face_ring(const double *piph, const int *fL, const int *fR,
+                            const signed char *fLR, int f0, int N)
+{
+   FaceRing R;
+   R.piph = piph;
+   R.fL = fL;
+   R.fR = fR;
+   R.fLR = fLR;
+   R.f0 = f0;
+   R.N = N;
+   R.p0 = face_up(piph, fL, fR, fLR, f0);
+   return R;
+}
+
+DHD double ring_key(const Ring &R, int x) { return R.piph[ring_cell(R, x)]; }
+DHD double ring_key(const FaceRing &R, int x)
+{
+   return face_up(R.piph, R.fL, R.fR, R.fLR, R.f0 + (x == R.N ? 0 : x));
+}
+
 // True when A's element x ends (in phi) no later than B's element y, i.e.
 // the reference walk advances A first (faces_alt.c:135-145: LR = 0 when the
-// wrapped difference of upper edges is not positive).  Elements are compared
-// through the reference's wrapped difference whenever they are within half a
-// period of each other in the unwrapped order (always the case for pairs the
-// walk visits); far-apart elements, which the binary searches also probe, are
-// ordered by their unwrapped positions.
-DHD bool a_first(const Ring &A, int x, const Ring &B, int y, double Pmax)
+// wrapped difference of upper edges is not positive; bfields.c:817-866 uses
+// the same rule).  Elements are compared through the reference's wrapped
+// difference whenever they are within half a period of each other in the
+// unwrapped order (always the case for pairs the walk visits); far-apart
+// elements, which the binary searches also probe, are ordered by their
+// unwrapped positions.
+template <class RA, class RB>
+DHD bool a_first(const RA &A, int x, const RB &B, int y, double Pmax)
 {
    if (y == B.N) return x < A.N;
    if (x == A.N) return false;
-   double pa = A.piph[ring_cell(A, x)];
-   double pb = B.piph[ring_cell(B, y)];
+   double pa = ring_key(A, x);
+   double pb = ring_key(B, y);
    double ua = (x > 0 && pa < A.p0) ? pa + Pmax : pa;
    double ub = (y > 0 && pb < B.p0) ? pb + Pmax : pb;
    double D = ua - ub;
@@ -77,7 +116,8 @@ DHD bool a_first(const Ring &A, int x, const Ring &B, int y, double Pmax)
 }
 
 // Number of A elements consumed before step d of the walk (merge path).
-DHD int merge_x(const Ring &A, const Ring &B, int d, double Pmax)
+template <class RA, class RB>
+DHD int merge_x(const RA &A, const RB &B, int d, double Pmax)
 {
    int lo = d - B.N > 0 ? d - B.N : 0;
    int hi = d < A.N ? d : A.N;
@@ -91,7 +131,8 @@ DHD int merge_x(const Ring &A, const Ring &B, int d, double Pmax)
 
 // Step at which A's element x (x < N_A) is consumed: x plus the number of B
 // elements that end before it.
-DHD int a_step(const Ring &A, int x, const Ring &B, double Pmax)
+template <class RA, class RB>
+DHD int a_step(const RA &A, int x, const RB &B, double Pmax)
 {
    int lo = 0, hi = B.N;
    while (lo < hi) {
@@ -103,7 +144,8 @@ DHD int a_step(const Ring &A, int x, const Ring &B, double Pmax)
 }
 
 // Step at which B's element y (y < N_B) is consumed.
-DHD int b_step(const Ring &A, const Ring &B, int y, double Pmax)
+template <class RA, class RB>
+DHD int b_step(const RA &A, const RB &B, int y, double Pmax)
 {
    int lo = 0, hi = A.N;
    while (lo < hi) {
@@ -122,7 +164,8 @@ struct FaceRange {
 };
 
 // Ranges of A's element x (the cell is the L side of these faces).
-DHD FaceRange ranges_as_a(const Ring &A, int x, const Ring &B, double Pmax)
+template <class RA, class RB>
+DHD FaceRange ranges_as_a(const RA &A, int x, const RB &B, double Pmax)
 {
    FaceRange r;
    int nf = A.N + B.N;
@@ -138,7 +181,8 @@ DHD FaceRange ranges_as_a(const Ring &A, int x, const Ring &B, double Pmax)
 }
 
 // Ranges of B's element y (the cell is the R side of these faces).
-DHD FaceRange ranges_as_b(const Ring &A, const Ring &B, int y, double Pmax)
+template <class RA, class RB>
+DHD FaceRange ranges_as_b(const RA &A, const RB &B, int y, double Pmax)
 {
    FaceRange r;
    int nf = A.N + B.N;

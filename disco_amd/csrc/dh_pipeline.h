@@ -1,0 +1,545 @@
+// dh_pipeline.h -- buffers of one mesh and the sequence of kernels that
+// makes a timestep (timestep.c:7-45, onestep.c:84-228).
+//
+// The sequence is written once against an executor X that owns memory and
+// launches: the HIP executor (dh_engine.hip) runs it on the GPU, and the
+// test suite instantiates a host executor to run the same bodies on the CPU
+// against the oracle.  X provides
+//   run(name, kernel, n)          kernel(t) for t in [0, n)
+//   ring_start(m, piph, I0)       start cell of every annulus
+//   flips(m, dim, F, flag, Phi, rkPhi)   hand-overs of flagged faces per track
+//   min_dt<NQ,H>(m, P, prim, piph, dphi, wiph, &dt)
+//   alloc/put/get/copy/zero/sync  memory
+#pragma once
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "dh_kernels.h"
+
+namespace dh {
+
+inline thread_local char g_err[512];
+
+inline int fail(int code, const char *fmt, ...)
+{
+   va_list ap;
+   va_start(ap, fmt);
+   vsnprintf(g_err, sizeof g_err, fmt, ap);
+   va_end(ap);
+   return code;
+}
+
+#define DH_TRY(expr)                                                          \
+   do {                                                                       \
+      int rc_ = (expr);                                                       \
+      if (rc_) return rc_;                                                    \
+   } while (0)
+
+struct Buffers {
+   dh_params P;
+   int nq = 0, nf = 0;
+   bool mhd = false;
+   Mesh m;
+   int *Np = nullptr, *off = nullptr, *cjk = nullptr;
+   double *r = nullptr, *z = nullptr;
+   std::vector<int> h_Np, h_off;
+   Faces F[2];
+   int *rng[2] = {nullptr, nullptr};
+   double *prim = nullptr, *cons = nullptr, *rkcons = nullptr;
+   double *gradp = nullptr, *gradr = nullptr, *gradz = nullptr;
+   double *piph = nullptr, *piph2 = nullptr, *dphi = nullptr, *dphi2 = nullptr;
+   double *wiph = nullptr;
+   double *S = nullptr, *dF[2] = {nullptr, nullptr}, *dFp = nullptr;
+   int *I0 = nullptr;
+   // constrained transport
+   double *Phi = nullptr, *rkPhi = nullptr, *emf_phi = nullptr;
+   double *emf_f[2] = {nullptr, nullptr}, *fE[2] = {nullptr, nullptr}, *fB[2] = {nullptr, nullptr};
+   double *Epre = nullptr, *Eavg = nullptr, *Eo = nullptr, *Eown = nullptr, *Ecarry = nullptr;
+   signed char *flag[2] = {nullptr, nullptr};
+   PlanetSet pl[2];
+};
+
+// Checks a grid and parameter set against what the engine implements.
+inline int validate(const dh_grid *g, const dh_params &P, long *ntot_out)
+{
+   int nq = P.num_c + P.num_n;
+   if (P.hydro == DH_HYDRO_EULER) {
+      if (P.num_c != 5) return fail(DH_EINVAL, "euler needs NUM_C = 5 (got %d)", P.num_c);
+      if (P.ct_mode != 0) return fail(DH_EINVAL, "CT_MODE %d needs the MHD module", P.ct_mode);
+      if (P.riemann_solver != 0 && P.riemann_solver != 1)
+         return fail(DH_EINVAL, "Riemann_Solver %d not supported by euler", P.riemann_solver);
+      if (nq < 5 || nq > 7) return fail(DH_EINVAL, "NUM_C+NUM_N = %d not supported", nq);
+   } else if (P.hydro == DH_HYDRO_MHD) {
+      if (P.num_c != 8) return fail(DH_EINVAL, "mhd needs NUM_C = 8 (got %d)", P.num_c);
+      if (P.ct_mode != 2 || !P.ct)
+         return fail(DH_EINVAL, "mhd is supported with CT_MODE 2 and Constrained_Transport 1");
+      if (P.riemann_solver != 2) return fail(DH_EINVAL, "mhd is supported with HLLD only");
+      if (P.visc_flag) return fail(DH_EINVAL, "mhd has no viscous terms (Use_Viscosity 0)");
+      if (nq < 8 || nq > 9) return fail(DH_EINVAL, "NUM_C+NUM_N = %d not supported", nq);
+      if (g->NgRa == 0 || g->NgRb == 0 || (g->Nz > 1 && (g->NgZa == 0 || g->NgZb == 0)))
+         return fail(DH_EINVAL, "mhd needs ghost annuli on every side");
+   } else {
+      return fail(DH_EINVAL, "hydro module %d not supported", P.hydro);
+   }
+   if (P.rot_frame) return fail(DH_EINVAL, "RotFrame is not supported");
+   if (P.nozzle_type) return fail(DH_EINVAL, "Nozzle_Type is not supported");
+   if (P.r_periodic || P.z_periodic) return fail(DH_EINVAL, "periodic boundaries are not supported");
+   if (g->Nr < 1 || g->Nz < 1) return fail(DH_EINVAL, "empty grid");
+   if (!(g->phi_max > 0.0)) return fail(DH_EINVAL, "phi_max must be positive");
+   int bcs[4] = {P.bc_rmin, P.bc_rmax, P.bc_zmin, P.bc_zmax};
+   int ngs[4] = {g->NgRa, g->NgRb, g->NgZa, g->NgZb};
+   for (int s = 0; s < 4; ++s) {
+      if (bcs[s] < DH_BC_NONE || bcs[s] > DH_BC_DIODE)
+         return fail(DH_EINVAL, "bad boundary kind %d", bcs[s]);
+      if (ngs[s] > 0 && bcs[s] == DH_BC_NONE)
+         return fail(DH_EINVAL, "side %d has ghost annuli but no boundary kind", s);
+   }
+   long ntot = 0;
+   for (int a = 0; a < g->Nr * g->Nz; ++a) {
+      if (g->Np[a] < 1) return fail(DH_EINVAL, "annulus %d has %d cells", a, g->Np[a]);
+      ntot += g->Np[a];
+   }
+   if (ntot >= (1L << 31) / 16) return fail(DH_EINVAL, "mesh too large (%ld cells)", ntot);
+   *ntot_out = ntot;
+   return DH_OK;
+}
+
+template <class X>
+int alloc_n(X &x, void *pp, size_t bytes)
+{
+   return x.alloc((void **)pp, bytes);
+}
+
+// Allocates and fills every buffer of a validated mesh.
+template <class X>
+int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
+{
+   long ntot = 0;
+   DH_TRY(validate(g, P, &ntot));
+   b.P = P;
+   b.nq = P.num_c + P.num_n;
+   b.mhd = P.hydro == DH_HYDRO_MHD;
+   b.nf = b.mhd ? 5 : 0;
+   int nq = b.nq;
+   int nann = g->Nr * g->Nz;
+   b.h_Np.assign(g->Np, g->Np + nann);
+   b.h_off.assign(nann + 1, 0);
+   for (int a = 0; a < nann; ++a) b.h_off[a + 1] = b.h_off[a] + g->Np[a];
+   std::vector<int> cjk(ntot);
+   for (int a = 0; a < nann; ++a)
+      for (int i = 0; i < g->Np[a]; ++i) cjk[b.h_off[a] + i] = a;
+   DH_TRY(alloc_n(x, &b.Np, sizeof(int) * nann));
+   DH_TRY(alloc_n(x, &b.off, sizeof(int) * (nann + 1)));
+   DH_TRY(alloc_n(x, &b.cjk, sizeof(int) * ntot));
+   DH_TRY(alloc_n(x, &b.r, sizeof(double) * (g->Nr + 1)));
+   DH_TRY(alloc_n(x, &b.z, sizeof(double) * (g->Nz + 1)));
+   DH_TRY(x.put(b.Np, g->Np, sizeof(int) * nann));
+   DH_TRY(x.put(b.off, b.h_off.data(), sizeof(int) * (nann + 1)));
+   DH_TRY(x.put(b.cjk, cjk.data(), sizeof(int) * ntot));
+   DH_TRY(x.put(b.r, g->r_jph, sizeof(double) * (g->Nr + 1)));
+   DH_TRY(x.put(b.z, g->z_kph, sizeof(double) * (g->Nz + 1)));
+
+   Mesh &m = b.m;
+   m.Nr = g->Nr;
+   m.Nz = g->Nz;
+   m.NgRa = g->NgRa;
+   m.NgRb = g->NgRb;
+   m.NgZa = g->NgZa;
+   m.NgZb = g->NgZb;
+   m.nann = nann;
+   m.ntot = (int)ntot;
+   m.Np = b.Np;
+   m.off = b.off;
+   m.cjk = b.cjk;
+   m.r_jph = b.r + 1;
+   m.z_kph = b.z + 1;
+   m.Pmax = g->phi_max;
+
+   // Face lists: the pair offsets are fixed because Np never changes
+   // (buildfaces mode 0, faces_alt.c:80-96).
+   for (int dim = 1; dim <= 2; ++dim) {
+      int d = dim - 1;
+      Faces &F = b.F[d];
+      memset(&F, 0, sizeof F);
+      int Nrmax = dim == 2 ? g->Nr : g->Nr - 1;
+      int Nzmax = dim == 1 ? g->Nz : g->Nz - 1;
+      if (Nrmax < 1 || Nzmax < 1) continue;
+      std::vector<int> fI(Nrmax * Nzmax + 1, 0);
+      long n = 0;
+      for (int k = 0; k < Nzmax; ++k)
+         for (int j = 0; j < Nrmax; ++j) {
+            int jk = j + g->Nr * k;
+            int jkp = dim == 2 ? j + g->Nr * (k + 1) : jk + 1;
+            fI[j + Nrmax * k] = (int)n;
+            n += g->Np[jk] + g->Np[jkp];
+         }
+      fI[Nrmax * Nzmax] = (int)n;
+      F.npairs = Nrmax * Nzmax;
+      F.n = (int)n;
+      int *dfI = nullptr;
+      DH_TRY(alloc_n(x, &dfI, sizeof(int) * fI.size()));
+      DH_TRY(x.put(dfI, fI.data(), sizeof(int) * fI.size()));
+      F.fI = dfI;
+      DH_TRY(alloc_n(x, &F.L, sizeof(int) * n));
+      DH_TRY(alloc_n(x, &F.R, sizeof(int) * n));
+      DH_TRY(alloc_n(x, &F.LR, sizeof(signed char) * n));
+      DH_TRY(alloc_n(x, &F.dA, sizeof(double) * n));
+      DH_TRY(alloc_n(x, &F.phic, sizeof(double) * n));
+      DH_TRY(alloc_n(x, &F.dphi, sizeof(double) * n));
+      DH_TRY(alloc_n(x, &b.dF[d], sizeof(double) * n * nq));
+      DH_TRY(alloc_n(x, &b.rng[d], sizeof(int) * 8 * (size_t)ntot));
+      if (b.mhd) {
+         DH_TRY(alloc_n(x, &b.emf_f[d], sizeof(double) * 3 * n));
+         DH_TRY(alloc_n(x, &b.fE[d], sizeof(double) * n));
+         DH_TRY(alloc_n(x, &b.fB[d], sizeof(double) * n));
+         DH_TRY(alloc_n(x, &b.flag[d], sizeof(signed char) * n));
+         DH_TRY(x.zero(b.emf_f[d], sizeof(double) * 3 * n));
+      }
+   }
+   int nfmax = b.F[0].n > b.F[1].n ? b.F[0].n : b.F[1].n;
+   size_t qb = sizeof(double) * (size_t)ntot * nq;
+   size_t cb = sizeof(double) * (size_t)ntot;
+   DH_TRY(alloc_n(x, &b.S, sizeof(double) * (size_t)nfmax * nq));
+   DH_TRY(alloc_n(x, &b.prim, qb));
+   DH_TRY(alloc_n(x, &b.cons, qb));
+   DH_TRY(alloc_n(x, &b.rkcons, qb));
+   DH_TRY(alloc_n(x, &b.gradp, qb));
+   DH_TRY(alloc_n(x, &b.gradr, qb));
+   DH_TRY(alloc_n(x, &b.gradz, qb));
+   DH_TRY(alloc_n(x, &b.dFp, qb));
+   DH_TRY(alloc_n(x, &b.piph, cb));
+   DH_TRY(alloc_n(x, &b.piph2, cb));
+   DH_TRY(alloc_n(x, &b.dphi, cb));
+   DH_TRY(alloc_n(x, &b.dphi2, cb));
+   DH_TRY(alloc_n(x, &b.wiph, cb));
+   DH_TRY(alloc_n(x, &b.I0, sizeof(int) * nann));
+   DH_TRY(x.zero(b.gradr, qb));
+   DH_TRY(x.zero(b.gradz, qb));
+   DH_TRY(x.zero(b.gradp, qb));
+   DH_TRY(x.zero(b.dFp, qb));
+   DH_TRY(x.zero(b.wiph, cb));
+   DH_TRY(x.zero(b.I0, sizeof(int) * nann));
+   if (b.mhd) {
+      DH_TRY(alloc_n(x, &b.Phi, 5 * cb));
+      DH_TRY(alloc_n(x, &b.rkPhi, 5 * cb));
+      DH_TRY(alloc_n(x, &b.emf_phi, 4 * cb));
+      DH_TRY(alloc_n(x, &b.Epre, 16 * cb));
+      DH_TRY(alloc_n(x, &b.Eavg, 16 * cb));
+      DH_TRY(alloc_n(x, &b.Eo, 4 * cb));
+      DH_TRY(alloc_n(x, &b.Eown, 4 * cb));
+      DH_TRY(alloc_n(x, &b.Ecarry, 16 * cb));
+      DH_TRY(x.zero(b.Phi, 5 * cb));
+      DH_TRY(x.zero(b.emf_phi, 4 * cb));
+      DH_TRY(x.zero(b.Ecarry, 16 * cb));
+   }
+   b.pl[0].n = b.pl[1].n = 0;
+   return DH_OK;
+}
+
+template <class X>
+struct Engine {
+   X &x;
+   Buffers &b;
+
+   // Ring starts, faces (+ slopes) and cell ranges of both directions, and
+   // the r/z slopes when grads is set (plm.c:56-232).
+   template <int NQ>
+   int topology(bool grads)
+   {
+      const Mesh &m = b.m;
+      if (!dim_active(m, 1) && !dim_active(m, 2)) return DH_OK;
+      DH_TRY(x.ring_start(m, b.piph, b.I0));
+      for (int dim = 1; dim <= 2; ++dim) {
+         int d = dim - 1;
+         if (!dim_active(m, dim)) continue;
+         double *grad = dim == 1 ? b.gradr : b.gradz;
+         bool slopes = grads && b.P.plm != 0.0;
+         KFaces<NQ> kf{m, dim, b.F[d], b.I0, b.piph, b.dphi, b.prim, b.gradp, b.S, slopes ? 1 : 0};
+         DH_TRY(x.run(dim == 1 ? "faces_r" : "faces_z", kf, b.F[d].n));
+         KCellRanges kr{m, dim, b.I0, b.piph, b.rng[d]};
+         DH_TRY(x.run(dim == 1 ? "ranges_r" : "ranges_z", kr, m.ntot));
+         if (!grads) continue;
+         if (!slopes) {
+            DH_TRY(x.zero(grad, sizeof(double) * NQ * m.ntot));
+            continue;
+         }
+         KGrad<NQ> kg{m, dim, b.P.plm, b.P.num_c, b.F[d], b.rng[d], b.piph, b.dphi, b.prim, b.S, grad};
+         DH_TRY(x.run(dim == 1 ? "grad_r" : "grad_z", kg, m.ntot));
+      }
+      return DH_OK;
+   }
+
+   // Zero-gradient / diode ghost annuli (onestep.c:194-210): r sides, then z.
+   template <int NQ>
+   int boundaries()
+   {
+      const Mesh &m = b.m;
+      const dh_params &P = b.P;
+      int bcs[4] = {P.bc_rmin, P.bc_rmax, P.bc_zmin, P.bc_zmax};
+      for (int side = 0; side < 4; ++side) {
+         if (side >= 2 && m.Nz <= 1) continue;
+         if (bcs[side] != DH_BC_ZEROGRAD && bcs[side] != DH_BC_DIODE) continue;
+         KBcCopy<NQ> k{m, side, bcs[side] == DH_BC_DIODE ? 1 : 0, b.prim, b.piph, b.dphi};
+         DH_TRY(x.run("bc_copy", k, side < 2 ? m.Nz : m.Nr));
+      }
+      return DH_OK;
+   }
+
+   // One substep (onestep.c:84-228).
+   template <int NQ, class H>
+   int substep(double RK, double dt, int first_step, int sub)
+   {
+      constexpr bool mhd = IsMHD<H>::value;
+      const Mesh &m = b.m;
+      const dh_params &P = b.P;
+      if (first_step) {
+         DH_TRY(x.run("wcell", KWcell<NQ, H>{m, P, b.prim, b.wiph}, m.ntot));
+         if (P.mesh_motion == 3) DH_TRY(x.run("wcell", KWcellAvg{m, b.wiph}, m.nann));
+      }
+      DH_TRY(x.run("plm_phi", KPlmPhi<NQ>{m, P.plm, b.prim, b.dphi, b.gradp}, m.ntot));
+      DH_TRY(topology<NQ>(true));
+      KFluxPhi<NQ, H> kp{m, P, dt, b.prim, b.gradp, b.gradr, b.gradz, b.piph, b.dphi, b.wiph,
+                         b.dFp, b.emf_phi};
+      DH_TRY(x.run("flux_phi", kp, m.ntot));
+      for (int dim = 1; dim <= 2; ++dim) {
+         int d = dim - 1;
+         if (!dim_active(m, dim)) continue;
+         KFluxTrans<NQ, H> kt{m, P, dim, dt, b.F[d], b.prim, b.gradp, b.gradr, b.gradz,
+                              b.piph, b.dphi, b.dF[d], b.emf_f[d]};
+         DH_TRY(x.run(dim == 1 ? "flux_r" : "flux_z", kt, b.F[d].n));
+      }
+      if (mhd) {
+         int ct = P.ct;
+         KEdgesCollect kc{m, ct, b.dphi, b.emf_phi, b.F[0], b.F[1], b.rng[0], b.rng[1],
+                          b.emf_f[0], b.emf_f[1], b.Ecarry, b.Epre};
+         DH_TRY(x.run("ct_edges", kc, m.ntot));
+         DH_TRY(x.run("ct_edges", KEdgesAvg{m, b.Epre, b.Eavg}, m.ntot));
+         for (int dim = 1; dim <= 2; ++dim) {
+            int d = dim - 1;
+            if (!dim_active(m, dim)) continue;
+            KFaceEmf kfe{m, dim, b.F[d], b.piph, b.dphi, b.Eavg, b.fE[d], b.fB[d]};
+            DH_TRY(x.run("ct_edges", kfe, b.F[d].n));
+         }
+         EdgeArgs ea{m, b.F[0], b.F[1], b.rng[0], b.rng[1], b.Eavg, b.fE[0], b.fB[0], b.fE[1], b.fB[1]};
+         KEdgesFinal kef{ea, ct, b.wiph, b.emf_f[0], b.emf_f[1], b.Eo, b.Eown, b.Ecarry};
+         DH_TRY(x.run("ct_edges", kef, m.ntot));
+         KUpdateB kb{m, RK, dt, b.F[0], b.F[1], b.rng[0], b.rng[1], b.Eo, b.Phi, b.rkPhi};
+         DH_TRY(x.run("ct_update_B", kb, m.ntot));
+         if (dim_active(m, 2)) {
+            KEdgeAdjust ke{m, dt, b.F[0], b.F[1], b.piph, b.dphi, b.Eown, b.Phi};
+            DH_TRY(x.run("ct_edge_adjust", ke, (long)b.F[0].n + b.F[1].n));
+         }
+      }
+      KUpdate<NQ, H> ku{m, P, RK, dt, first_step, b.pl[sub], b.prim, b.prim, b.cons,
+                        b.rkcons, b.gradr, b.gradp, b.gradz, b.piph, b.dphi, b.wiph,
+                        b.piph2, b.dphi2, b.dFp, b.dF[0], b.dF[1], b.F[0].fI, b.F[1].fI,
+                        b.rng[0], b.rng[1], b.F[0].n, b.F[1].n};
+      DH_TRY(x.run("update", ku, m.ntot));
+      if (mhd) {
+         if (first_step) {
+            for (int dim = 1; dim <= 2; ++dim) {
+               int d = dim - 1;
+               if (!dim_active(m, dim)) continue;
+               KFlipFlags kff{m, dim, b.F[d], b.piph, b.wiph, dt, b.flag[d]};
+               DH_TRY(x.run("ct_flips", kff, b.F[d].n));
+               DH_TRY(x.flips(m, dim, b.F[d], b.flag[d], b.Phi, b.rkPhi));
+            }
+         }
+         KBfacesC2p<NQ, H> kbc{m, P, b.F[0], b.F[1], b.rng[0], b.rng[1], b.piph2, b.dphi2,
+                               b.Phi, b.cons, b.prim, 1, 1};
+         DH_TRY(x.run("ct_bfaces_c2p", kbc, m.ntot));
+      }
+      std::swap(b.piph, b.piph2);
+      std::swap(b.dphi, b.dphi2);
+      return boundaries<NQ>();
+   }
+
+   // timestep (timestep.c:7-45): RK2 unless Timestep = 1 (forward Euler).
+   template <int NQ, class H>
+   int timestep(double dt)
+   {
+      DH_TRY(x.copy(b.rkcons, b.cons, sizeof(double) * NQ * b.m.ntot));
+      if (b.mhd) DH_TRY(x.copy(b.rkPhi, b.Phi, sizeof(double) * 5 * b.m.ntot));
+      DH_TRY((substep<NQ, H>(0.0, dt, 1, 0)));
+      if (b.P.timestep_mode != 1) DH_TRY((substep<NQ, H>(0.5, 0.5 * dt, 0, 1)));
+      return DH_OK;
+   }
+
+   // setupCells after the initial prims are in place (domain.c:139-242):
+   // cons on real cells; for MHD, cell fields from the face fluxes the host
+   // set (set_B_fields' B_faces_to_cells, bfields.c:80-81) and cons2prim;
+   // then face velocities and ghost annuli.
+   template <int NQ, class H>
+   int init()
+   {
+      const Mesh &m = b.m;
+      DH_TRY(x.run("init", KInitCells<NQ, H>{m, b.P, b.prim, b.cons, b.piph, b.dphi}, m.ntot));
+      if constexpr (IsMHD<H>::value) {
+         DH_TRY(topology<NQ>(false));
+         KBfacesC2p<NQ, H> k0{m, b.P, b.F[0], b.F[1], b.rng[0], b.rng[1], b.piph, b.dphi,
+                              b.Phi, b.cons, b.prim, 0, 0};
+         DH_TRY(x.run("init", k0, m.ntot));
+         KBfacesC2p<NQ, H> k1{m, b.P, b.F[0], b.F[1], b.rng[0], b.rng[1], b.piph, b.dphi,
+                              b.Phi, b.cons, b.prim, 1, 1};
+         DH_TRY(x.run("init", k1, m.ntot));
+      }
+      DH_TRY(x.run("init", KWcell<NQ, H>{m, b.P, b.prim, b.wiph}, m.ntot));
+      if (b.P.mesh_motion == 3) DH_TRY(x.run("init", KWcellAvg{m, b.wiph}, m.nann));
+      DH_TRY(boundaries<NQ>());
+      return x.sync();
+   }
+
+   // getmindt (misc.c:39-84)
+   template <int NQ, class H>
+   int mindt(double *dt)
+   {
+      double v = HUGE_VAL;
+      DH_TRY((x.template min_dt<NQ, H>(b.m, b.P, b.prim, b.piph, b.dphi, b.wiph, &v)));
+      double d0 = b.P.max_dt / b.P.cfl;
+      if (d0 <= 0.0) d0 = 1.0e100;
+      if (v < d0) d0 = v;
+      *dt = d0 * b.P.cfl;
+      return DH_OK;
+   }
+
+   // Faces of the current mesh, with no slopes (for a host that sets the
+   // initial face fluxes).
+   template <int NQ, class H>
+   int faces()
+   {
+      DH_TRY(topology<NQ>(false));
+      return x.sync();
+   }
+};
+
+// Dispatch on the hydro module and the number of fields (NUM_Q).
+#define DH_DISPATCH(b, FN, ...)                                               \
+   ([&]() -> int {                                                            \
+      if ((b).mhd) {                                                          \
+         switch ((b).nq) {                                                    \
+         case 8: return FN<8, MHD<8>>(__VA_ARGS__);                           \
+         case 9: return FN<9, MHD<9>>(__VA_ARGS__);                           \
+         default: return fail(DH_EINVAL, "unsupported NQ %d", (b).nq);       \
+         }                                                                    \
+      }                                                                       \
+      switch ((b).nq) {                                                       \
+      case 5: return FN<5, Euler<5>>(__VA_ARGS__);                            \
+      case 6: return FN<6, Euler<6>>(__VA_ARGS__);                            \
+      case 7: return FN<7, Euler<7>>(__VA_ARGS__);                            \
+      default: return fail(DH_EINVAL, "unsupported NQ %d", (b).nq);          \
+      }                                                                       \
+   }())
+
+// Host <-> executor copies of the whole state; any pointer may be null.
+// dphi is recomputed from piph (calc_dp, misc.c:271-288) when it is null.
+template <class X>
+int upload_state(X &x, Buffers &b, const double *prim, const double *cons, const double *piph,
+                 const double *dphi, const double *wiph, const double *Phi)
+{
+   size_t nt = (size_t)b.m.ntot, qb = sizeof(double) * nt * b.nq;
+   if (prim) DH_TRY(x.put(b.prim, prim, qb));
+   if (cons) DH_TRY(x.put(b.cons, cons, qb));
+   if (piph) DH_TRY(x.put(b.piph, piph, sizeof(double) * nt));
+   if (wiph) DH_TRY(x.put(b.wiph, wiph, sizeof(double) * nt));
+   if (Phi && b.nf) DH_TRY(x.put(b.Phi, Phi, sizeof(double) * nt * b.nf));
+   if (dphi) {
+      DH_TRY(x.put(b.dphi, dphi, sizeof(double) * nt));
+   } else if (piph) {
+      std::vector<double> dp(nt);
+      for (int a = 0; a < b.m.nann; ++a) {
+         int c0 = b.h_off[a], np = b.h_Np[a];
+         for (int i = 0; i < np; ++i) {
+            double v = piph[c0 + i] - piph[c0 + (i == 0 ? np - 1 : i - 1)];
+            while (v < 0.0) v += b.m.Pmax;
+            while (v > b.m.Pmax) v -= b.m.Pmax;
+            dp[c0 + i] = v;
+         }
+      }
+      DH_TRY(x.put(b.dphi, dp.data(), sizeof(double) * nt));
+   }
+   return DH_OK;
+}
+
+template <class X>
+int download_state(X &x, Buffers &b, double *prim, double *cons, double *piph, double *dphi,
+                   double *wiph, double *Phi)
+{
+   size_t nt = (size_t)b.m.ntot, qb = sizeof(double) * nt * b.nq;
+   if (prim) DH_TRY(x.get(prim, b.prim, qb));
+   if (cons) DH_TRY(x.get(cons, b.cons, qb));
+   if (piph) DH_TRY(x.get(piph, b.piph, sizeof(double) * nt));
+   if (dphi) DH_TRY(x.get(dphi, b.dphi, sizeof(double) * nt));
+   if (wiph) DH_TRY(x.get(wiph, b.wiph, sizeof(double) * nt));
+   if (Phi && b.nf) DH_TRY(x.get(Phi, b.Phi, sizeof(double) * nt * b.nf));
+   return DH_OK;
+}
+
+// Planets seen by the sources of one RK substep.
+inline int set_planets(Buffers &b, int substep, const double *pl, int npl)
+{
+   if (substep < 0 || substep > 1) return fail(DH_EINVAL, "substep must be 0 or 1");
+   if (npl < 0 || npl > DH_MAX_PLANETS) return fail(DH_EINVAL, "npl %d out of range", npl);
+   if (npl > 0 && !pl) return fail(DH_EINVAL, "null planet list");
+   PlanetSet &s = b.pl[substep];
+   s.n = 0;
+   for (int p = 0; p < npl; ++p) {
+      const double *v = pl + p * DH_PLANET_STRIDE;
+      // A planet of zero mass adds exactly zero force (planet.c:56-76).
+      if (v[2] == 0.0 && ((int)v[6] == 0 || (int)v[6] == 1)) continue;
+      for (int q = 0; q < DH_PLANET_STRIDE; ++q) s.v[s.n * DH_PLANET_STRIDE + q] = v[q];
+      s.n++;
+   }
+   return DH_OK;
+}
+
+// Host-side face description for the host layer (faces_alt.c:10-34 addFace):
+// cells, orientation, centroid {r, phi, z} and area of every face.
+template <class X>
+int export_faces(X &x, Buffers &b, int dim, int *L, int *R, int *LRtype, double *cm, double *dA)
+{
+   int d = dim - 1;
+   const Faces &F = b.F[d];
+   size_t n = (size_t)F.n;
+   if (n == 0) return DH_OK;
+   std::vector<int> hL(n), hR(n), fI(F.npairs + 1);
+   std::vector<signed char> hLR(n);
+   std::vector<double> hphic(n), hdA(n), rj(b.m.Nr + 1), zk(b.m.Nz + 1);
+   DH_TRY(x.get(hL.data(), F.L, sizeof(int) * n));
+   DH_TRY(x.get(hR.data(), F.R, sizeof(int) * n));
+   DH_TRY(x.get(hLR.data(), F.LR, n));
+   DH_TRY(x.get(hphic.data(), F.phic, sizeof(double) * n));
+   DH_TRY(x.get(hdA.data(), F.dA, sizeof(double) * n));
+   DH_TRY(x.get(fI.data(), F.fI, sizeof(int) * fI.size()));
+   DH_TRY(x.get(rj.data(), b.r, sizeof(double) * rj.size()));
+   DH_TRY(x.get(zk.data(), b.z, sizeof(double) * zk.size()));
+   const double *r_jph = rj.data() + 1, *z_kph = zk.data() + 1;
+   int Nrmax = dim == 2 ? b.m.Nr : b.m.Nr - 1;
+   for (int P = 0; P < F.npairs; ++P) {
+      int j = P % Nrmax, k = P / Nrmax;
+      double r, z;
+      if (dim == 1) {
+         r = .5 * (r_jph[j] + r_jph[j]);
+         z = .5 * (z_kph[k] + z_kph[k - 1]);
+      } else {
+         double xp = r_jph[j], xm = r_jph[j - 1];
+         r = 2.0 * (xp * xp + xp * xm + xm * xm) / (3.0 * (xp + xm));
+         z = .5 * (z_kph[k] + z_kph[k]);
+      }
+      for (int f = fI[P]; f < fI[P + 1]; ++f) {
+         if (L) L[f] = hL[f];
+         if (R) R[f] = hR[f];
+         if (LRtype) LRtype[f] = hLR[f];
+         if (cm) {
+            cm[3 * (size_t)f] = r;
+            cm[3 * (size_t)f + 1] = hphic[f];
+            cm[3 * (size_t)f + 2] = z;
+         }
+         if (dA) dA[f] = hdA[f];
+      }
+   }
+   return DH_OK;
+}
+
+}  // namespace dh

@@ -388,7 +388,14 @@ int disco_setup(struct domain *d)
    initializePlanets(d->thePlanets);
    if (d->Npl > DH_MAX_PLANETS) return host_fail("at most %d planets", DH_MAX_PLANETS);
    double Pmax = d->phi_max;
-   for (int k = 0; k < Nz; ++k)
+   /* One rand() per annulus of the global mesh, ghost layers included, in
+    * (k, j) order; annuli this mesh does not hold still consume theirs
+    * (domain.c:52-88): with Num_Z = 1 the z ghost layers exist globally
+    * unless NoBC_Zmin/NoBC_Zmax say otherwise. */
+   for (int k = d->N0z_glob; k < d->N0z; ++k)
+      for (int j = 0; j < d->Nr_glob; ++j) rand();
+   for (int k = 0; k < Nz; ++k) {
+      for (int j = d->N0r_glob; j < d->N0r; ++j) rand();
       for (int j = 0; j < Nr; ++j) {
          int jk = k * Nr + j;
          double p0 = Pmax * (double)rand() / (double)RAND_MAX;
@@ -400,6 +407,8 @@ int disco_setup(struct domain *d)
             d->theCells[jk][i].dphi = dp;
          }
       }
+      for (int j = d->N0r + Nr; j < d->N0r_glob + d->Nr_glob; ++j) rand();
+   }
    d->t = L->t_min;
    d->t_init = L->t_min;
    d->t_fin = L->t_max;

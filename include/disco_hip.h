@@ -133,6 +133,19 @@ int dh_download_state(dh_ctx *ctx, double *prim, double *cons, double *piph,
  * Ghost prims of DH_BC_HELD sides must already hold their boundary values. */
 int dh_init_cells(dh_ctx *ctx);
 
+/* Number of r (dim 1) or z (dim 2) faces of the mesh: the sum over
+ * neighbouring annulus pairs of Np[a] + Np[b] (faces_alt.c:80-96). */
+int dh_face_count(dh_ctx *ctx, int dim, long *nfaces);
+
+/* Builds the r (dim 1) or z (dim 2) faces of the current mesh and copies
+ * them out: cell indices L and R, LRtype (0: the face's upper phi end is
+ * L's), the face centroid cm[3*f..3*f+2] = {r, phi, z} and area dA.  Any
+ * pointer may be NULL.  Replaces the face walk of setup_faces
+ * (faces_alt.c:36-179) for a host that needs the faces, as set_B_fields
+ * does to set the initial magnetic fluxes (bfields.c:45-78). */
+int dh_get_faces(dh_ctx *ctx, int dim, int *L, int *R, int *LRtype, double *cm,
+                 double *dA);
+
 /* Planets seen by the sources of RK substep `substep` (0 or 1) of the next
  * dh_timestep.  npl <= DH_MAX_PLANETS. */
 int dh_set_planets(dh_ctx *ctx, int substep, const double *planets, int npl);

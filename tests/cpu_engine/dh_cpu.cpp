@@ -1,0 +1,292 @@
+// dh_cpu.cpp -- the timestep kernels of disco_amd/csrc built as host code,
+// for the test suite only.
+//
+// Every kernel body (dh_kernels.h) and the launch sequence (dh_pipeline.h)
+// are the ones the GPU library runs; this host executor runs each "launch"
+// as a loop over its threads.  Built with -ffp-contract=off and the host
+// libm, the results are expected to match the oracle bit for bit, which
+// checks the owner-computes restatement (merge paths, gather order, edge
+// walks) without a GPU.  The library exports the same dh_* entry points as
+// libdisco_hip.so.  Nothing in the product loads it.
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "disco_hip.h"
+#include "dh_pipeline.h"
+
+using namespace dh;
+
+struct HostExec {
+   std::vector<void *> allocs;
+   long flipped = 0;  // faces handed over so far (a check that flips ran)
+
+   template <class K>
+   int run(const char *, const K &k, long n)
+   {
+      for (long t = 0; t < n; ++t) k(t);
+      return DH_OK;
+   }
+   int ring_start(const Mesh &m, const double *piph, int *I0)
+   {
+      for (int a = 0; a < m.nann; ++a) I0[a] = ring_start_of(m, piph, a);
+      return DH_OK;
+   }
+   int flips(const Mesh &m, int dim, const Faces &F, const signed char *flag, double *Phi,
+             double *rkPhi)
+   {
+      KFlipsTrack k{m, dim, F, flag, Phi, rkPhi};
+      for (long JK = 0; JK < F.npairs; ++JK) {
+         if (track_skipped(F.npairs, (int)JK)) continue;
+         for (int f = F.fI[JK]; f < F.fI[JK + 1]; ++f) flipped += flag[f] != 0;
+         k(JK);
+      }
+      return DH_OK;
+   }
+   template <int NQ, class H>
+   int min_dt(const Mesh &m, const dh_params &P, const double *prim, const double *piph,
+              const double *dphi, const double *wiph, double *out)
+   {
+      double v = HUGE_VAL;
+      for (long c = 0; c < m.ntot; ++c) {
+         double d = cell_mindt<NQ, H>(m, P, prim, piph, dphi, wiph, c);
+         if (d < HUGE_VAL && d > 0.0 && d < v) v = d;
+      }
+      *out = v;
+      return DH_OK;
+   }
+   int alloc(void **p, size_t bytes)
+   {
+      *p = calloc(bytes ? bytes : 8, 1);
+      if (!*p) return fail(DH_ENOMEM, "out of host memory (%zu bytes)", bytes);
+      allocs.push_back(*p);
+      return DH_OK;
+   }
+   int put(void *dst, const void *src, size_t bytes)
+   {
+      memcpy(dst, src, bytes);
+      return DH_OK;
+   }
+   int get(void *dst, const void *src, size_t bytes)
+   {
+      memcpy(dst, src, bytes);
+      return DH_OK;
+   }
+   int copy(void *dst, const void *src, size_t bytes)
+   {
+      memcpy(dst, src, bytes);
+      return DH_OK;
+   }
+   int zero(void *p, size_t bytes)
+   {
+      memset(p, 0, bytes);
+      return DH_OK;
+   }
+   int sync() { return DH_OK; }
+};
+
+struct dh_ctx {
+   HostExec x;
+   Buffers b;
+};
+
+static void release(dh_ctx *c)
+{
+   if (!c) return;
+   for (void *p : c->x.allocs) free(p);
+   delete c;
+}
+
+template <int NQ, class H>
+static int do_init(dh_ctx *c)
+{
+   Engine<HostExec> e{c->x, c->b};
+   return e.template init<NQ, H>();
+}
+template <int NQ, class H>
+static int do_timestep(dh_ctx *c, double dt)
+{
+   Engine<HostExec> e{c->x, c->b};
+   return e.template timestep<NQ, H>(dt);
+}
+template <int NQ, class H>
+static int do_mindt(dh_ctx *c, double *dt)
+{
+   Engine<HostExec> e{c->x, c->b};
+   return e.template mindt<NQ, H>(dt);
+}
+template <int NQ, class H>
+static int do_faces(dh_ctx *c)
+{
+   Engine<HostExec> e{c->x, c->b};
+   return e.template faces<NQ, H>();
+}
+
+template <int NQ, class H>
+static int do_grads(dh_ctx *c, int dim, double *out)
+{
+   Engine<HostExec> e{c->x, c->b};
+   Buffers &b = c->b;
+   KPlmPhi<NQ> kp{b.m, b.P.plm, b.prim, b.dphi, b.gradp};
+   DH_TRY(c->x.run("plm_phi", kp, b.m.ntot));
+   DH_TRY(e.template topology<NQ>(true));
+   const double *g = dim == 0 ? b.gradp : dim == 1 ? b.gradr : b.gradz;
+   memcpy(out, g, sizeof(double) * NQ * b.m.ntot);
+   return DH_OK;
+}
+
+extern "C" {
+// Stage check for the tests: phi (dim 0), r (1) or z (2) slopes of the
+// current state.
+int dhx_slopes(dh_ctx *c, int dim, double *out)
+{
+   if (!c || !out || dim < 0 || dim > 2) return fail(DH_EINVAL, "bad argument");
+   return DH_DISPATCH(c->b, do_grads, c, dim, out);
+}
+
+// Stage check for the tests: one Riemann solve of the module the context
+// was created for; F - w U* into Fw (riemann.c:216-318).
+int dhx_riemann(dh_ctx *c, const double *pL, const double *pR, double r, const double *n,
+                double w, double *Fw)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   double F[9], U[9];
+   if (c->b.mhd && c->b.nq == 8) MHD<8>::riemann(c->b.P, pL, pR, r, n, w, F, U);
+   else if (!c->b.mhd && c->b.nq == 5) Euler<5>::riemann(c->b.P, pL, pR, r, n, w, F, U);
+   else return fail(DH_EINVAL, "dhx_riemann: NQ %d not wired", c->b.nq);
+   for (int q = 0; q < c->b.nq; ++q) Fw[q] = F[q] - w * U[q];
+   return DH_OK;
+}
+}
+
+template <int NQ, class H>
+static int do_fluxes(dh_ctx *c, double dt, double *dFp, double *dFr, double *dFz)
+{
+   Engine<HostExec> e{c->x, c->b};
+   Buffers &b = c->b;
+   DH_TRY(c->x.run("plm_phi", KPlmPhi<NQ>{b.m, b.P.plm, b.prim, b.dphi, b.gradp}, b.m.ntot));
+   DH_TRY(e.template topology<NQ>(true));
+   KFluxPhi<NQ, H> kp{b.m, b.P, dt, b.prim, b.gradp, b.gradr, b.gradz, b.piph, b.dphi, b.wiph,
+                      b.dFp, b.emf_phi};
+   DH_TRY(c->x.run("flux_phi", kp, b.m.ntot));
+   for (int dim = 1; dim <= 2; ++dim) {
+      int d = dim - 1;
+      if (!dim_active(b.m, dim)) continue;
+      KFluxTrans<NQ, H> kt{b.m, b.P, dim, dt, b.F[d], b.prim, b.gradp, b.gradr, b.gradz,
+                           b.piph, b.dphi, b.dF[d], b.emf_f[d]};
+      DH_TRY(c->x.run("flux", kt, b.F[d].n));
+   }
+   memcpy(dFp, b.dFp, sizeof(double) * NQ * b.m.ntot);
+   if (dFr) memcpy(dFr, b.dF[0], sizeof(double) * NQ * b.F[0].n);
+   if (dFz) memcpy(dFz, b.dF[1], sizeof(double) * NQ * b.F[1].n);
+   return DH_OK;
+}
+
+extern "C" long dhx_flip_count(dh_ctx *c) { return c ? c->x.flipped : -1; }
+
+extern "C" int dhx_fluxes(dh_ctx *c, double dt, double *dFp, double *dFr, double *dFz)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return DH_DISPATCH(c->b, do_fluxes, c, dt, dFp, dFr, dFz);
+}
+
+extern "C" {
+
+const char *dh_last_error(void) { return g_err; }
+
+int dh_destroy(dh_ctx *c)
+{
+   release(c);
+   return DH_OK;
+}
+
+int dh_create(const dh_grid *g, const dh_params *par, dh_ctx **out)
+{
+   if (!g || !par || !out) return fail(DH_EINVAL, "null argument");
+   *out = nullptr;
+   dh_ctx *c = new dh_ctx();
+   int rc = create_buffers(c->x, c->b, g, *par);
+   if (rc) {
+      release(c);
+      return rc;
+   }
+   *out = c;
+   return DH_OK;
+}
+
+int dh_sizes(const dh_ctx *c, long *ntot, int *nq, int *nf)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   if (ntot) *ntot = c->b.m.ntot;
+   if (nq) *nq = c->b.nq;
+   if (nf) *nf = c->b.nf;
+   return DH_OK;
+}
+
+int dh_upload_state(dh_ctx *c, const double *prim, const double *cons, const double *piph,
+                    const double *dphi, const double *wiph, const double *Phi)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return upload_state(c->x, c->b, prim, cons, piph, dphi, wiph, Phi);
+}
+
+int dh_download_state(dh_ctx *c, double *prim, double *cons, double *piph, double *dphi,
+                      double *wiph, double *Phi)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return download_state(c->x, c->b, prim, cons, piph, dphi, wiph, Phi);
+}
+
+int dh_init_cells(dh_ctx *c)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return DH_DISPATCH(c->b, do_init, c);
+}
+
+int dh_face_count(dh_ctx *c, int dim, long *nfaces)
+{
+   if (!c || !nfaces || (dim != 1 && dim != 2)) return fail(DH_EINVAL, "bad argument");
+   *nfaces = c->b.F[dim - 1].n;
+   return DH_OK;
+}
+
+int dh_get_faces(dh_ctx *c, int dim, int *L, int *R, int *LRtype, double *cm, double *dA)
+{
+   if (!c || (dim != 1 && dim != 2)) return fail(DH_EINVAL, "bad argument");
+   if (c->b.F[dim - 1].n == 0) return DH_OK;
+   DH_TRY(DH_DISPATCH(c->b, do_faces, c));
+   return export_faces(c->x, c->b, dim, L, R, LRtype, cm, dA);
+}
+
+int dh_set_planets(dh_ctx *c, int substep, const double *pl, int npl)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return set_planets(c->b, substep, pl, npl);
+}
+
+int dh_getmindt(dh_ctx *c, double *dt)
+{
+   if (!c || !dt) return fail(DH_EINVAL, "null argument");
+   return DH_DISPATCH(c->b, do_mindt, c, dt);
+}
+
+int dh_timestep(dh_ctx *c, double dt)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   if (!(dt > 0.0)) return fail(DH_EINVAL, "dt must be positive");
+   return DH_DISPATCH(c->b, do_timestep, c, dt);
+}
+
+int dh_sync(dh_ctx *c) { return c ? DH_OK : fail(DH_EINVAL, "null context"); }
+
+int dh_enable_timers(dh_ctx *c, int) { return c ? DH_OK : fail(DH_EINVAL, "null context"); }
+
+int dh_read_timers(dh_ctx *c, int, char (*)[48], double *, long *, int *nrows)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   if (nrows) *nrows = 0;
+   return DH_OK;
+}
+
+}  // extern "C"

@@ -32,6 +32,8 @@ CASES = {
     "euler_yee_hll_nr16": ("euler_yee", {"Num_R": 16, "Riemann_Solver": 0}, 6),
     "euler_yee_n1_nr16": ("euler_yee_n1", {"Num_R": 16}, 6),
     "euler_vortex_nr16": ("euler_vortex", {"Num_R": 16}, 10),
+    # 3-D MHD Keplerian disk with constrained transport, 8 x 2 real annuli.
+    "mhd_kep_r8z2": ("mhd_kep", {"Num_R": 8, "Num_Z": 2}, 12),
 }
 
 

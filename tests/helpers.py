@@ -73,3 +73,36 @@ def compare_states(s: MeshState, ref: MeshState, tol: float = PARITY_TOL):
     worst = errs[worst_key]
     assert worst <= tol, f"relative L1 {worst:.3e} ({worst_key}) > {tol:.1e}: {errs}"
     return worst
+
+
+def face_flux(phi_flux: np.ndarray, dim: int, L: np.ndarray, R: np.ndarray,
+              LR: np.ndarray) -> np.ndarray:
+    """Magnetic flux through every r (dim 1) or z (dim 2) face: the Phi slot
+    of the face's owner (LRtype 0: L's outer slot, 1: R's inner slot)."""
+    if dim == 1:
+        return np.where(LR == 0, phi_flux[2][L], phi_flux[1][R])
+    return np.where(LR == 0, phi_flux[4][L], phi_flux[3][R])
+
+
+def div_b(s: MeshState, faces) -> np.ndarray:
+    """Relative divergence of B per cell: the net outward flux through the
+    cell's faces over the sum of their absolute fluxes.  faces = {dim: (L, R,
+    LRtype)} of the state's current mesh."""
+    nt = s.ntot
+    net = np.zeros(nt)
+    mag = np.zeros(nt)
+    off = np.concatenate([[0], np.cumsum(s.np_)])
+    cm = np.empty(nt, dtype=np.int64)
+    for a in range(len(s.np_)):
+        i = np.arange(s.np_[a])
+        cm[off[a] + i] = off[a] + (i - 1) % s.np_[a]
+    p0 = s.phi_flux[0]
+    net += p0 - p0[cm]
+    mag += np.abs(p0) + np.abs(p0[cm])
+    for dim, (L, R, LR) in faces.items():
+        f = face_flux(s.phi_flux, dim, L, R, LR)
+        np.add.at(net, L, f)
+        np.add.at(net, R, -f)
+        np.add.at(mag, L, np.abs(f))
+        np.add.at(mag, R, np.abs(f))
+    return np.abs(net) / np.where(mag > 0, mag, 1.0)

@@ -4,7 +4,9 @@ reference fixtures.
 Tolerances: the GPU contracts multiply-adds into FMAs and sums each cell's
 faces in the reference order, so it differs from the reference only by
 rounding.  Every primitive and conserved field must agree to relative L1
-1e-10 over real cells (PARITY_TOL), and the CFL dt to 1e-13.  Dominant
+1e-10 over real cells (PARITY_TOL), the magnetic fluxes Phi likewise, and the
+CFL dt to 1e-13; div B (net outward flux over the sum of face fluxes) stays
+below 1e-12 where the shear per step is below one cell.  Dominant
 fields (density, pressure) agree far tighter; fields that vanish in the
 exact solution (v_r of the steady vortex) carry relative round-off of a few
 1e-12.
@@ -17,7 +19,8 @@ import numpy as np
 import pytest
 
 from disco_amd import capi, configs, state_io
-from helpers import GOLDEN_CASES, PARITY_TOL, compare_states, field_errors, load_golden
+from helpers import (GOLDEN_CASES, PARITY_TOL, compare_states, div_b, field_errors, load_golden,
+                     rel_l1)
 from oracle.bind import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -47,6 +50,11 @@ def test_gpu_matches_reference_fixture(name):
     e = field_errors(out, b)
     print(f"  worst rel L1 {worst:.2e}; rho {e['prim0']:.2e}, P {e['prim1']:.2e}", flush=True)
     assert e["prim0"] <= 1e-13 and e["prim1"] <= 1e-13
+    if a.nf:
+        m = b.real_mask()
+        ephi = rel_l1(out.phi_flux[:, m], b.phi_flux[:, m])
+        print(f"  Phi rel L1 {ephi:.2e}", flush=True)
+        assert ephi <= TOL
 
 
 def _ref_initial(cfg, par):
@@ -59,8 +67,13 @@ def _ref_initial(cfg, par):
     return state_io.read_state(os.path.join(d, "state_0000.bin"))
 
 
-@pytest.mark.parametrize("cname,over,steps", [("euler_yee", {}, 10)])
-def test_gpu_matches_oracle_full_size(cname, over, steps):
+@pytest.mark.parametrize("cname,over,steps", [
+    ("euler_yee", {}, 10),                                     # the 2-D workload at full size
+    ("euler_yee", {"Num_R": 32, "Num_Z": 8}, 6),               # z faces
+    ("mhd_kep", {"Num_R": 48, "Num_Z": 12}, 10),               # CT, CFL steps
+    ("mhd_kep", {"Num_R": 32, "Num_Z": 8, "Max_DT": 0.01}, 10),  # CT, div B kept
+])
+def test_gpu_matches_oracle(cname, over, steps):
     cfg = configs.get(cname)
     par = cfg.par_dict(**over)
     a = _ref_initial(cfg, par)
@@ -75,9 +88,19 @@ def test_gpu_matches_oracle_full_size(cname, over, steps):
         g.timestep(dto)
     ref = o.get_state()
     out = g.download(a)
-    g.close()
     worst = compare_states(out, ref, TOL)
     print(f"  worst rel L1 {worst:.2e}", flush=True)
+    if a.nf:
+        m = ref.real_mask()
+        ephi = rel_l1(out.phi_flux[:, m], ref.phi_flux[:, m])
+        print(f"  Phi rel L1 {ephi:.2e}", flush=True)
+        assert ephi <= TOL
+        if par.get("Max_DT"):
+            faces = {d: g.faces(d)[:3] for d in (1, 2)}
+            div = div_b(out, faces)[m].max()
+            print(f"  div B (relative, max over real cells) {div:.2e}", flush=True)
+            assert div <= 1e-12
+    g.close()
 
 
 def test_gpu_mass_conservation_polar():
