@@ -5,18 +5,21 @@
 
 A step is one full RK2 timestep (getmindt + timestep, both through the
 reference-signature host entry points of libdisco_host_<config>.so) over the
-whole mesh, with the state resident in HBM.  The timed region is exactly K
-steps between device synchronisations.  Prints one JSON line:
+whole mesh, with the state resident in HBM.  The default workload is the 3-D
+MHD Keplerian disk (Nr=256, Nz=64, constrained transport).  The timed region
+is exactly K steps between device synchronisations.  Prints one JSON line:
 
-  value        zone-updates/s = real cells x K / seconds
-  roofline     the dominant kernel (most device time per step): algorithmic
-               bytes per launch / its mean launch time (HIP events on the
-               library's stream), against the 8 TB/s HBM3E peak
+  value          zone-updates/s = real cells x K / seconds
+  roofline       the dominant kernel (most device time per step): algorithmic
+                 bytes per launch / its mean launch time (HIP events on the
+                 library's stream), against the 8 TB/s HBM3E peak; traffic is
+                 the measured HBM bytes per launch of that kernel from the
+                 rocprofv3 counter summary under profiles/, when there is one
   step_roofline  B_alg x zone-updates/s against 8 TB/s, with
-               B_alg = 40 (NQ + NF + 1) bytes per zone-update (the state
-               {cons, Phi, piph} read and written over the RK2 step)
-  cpu_baseline the reference build (oracle/_ref) on this host, one core, on a
-               bounded number of steps of the same mesh
+                 B_alg = 40 (NQ + NF + 1) bytes per zone-update (the state
+                 {cons, Phi, piph} read and written over the RK2 step)
+  cpu_baseline   the reference build (oracle/_ref) on this host, one core, on a
+                 bounded sample of the same workload
 """
 from __future__ import annotations
 
@@ -35,119 +38,175 @@ from disco_amd import configs, state_io  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
 
-# name -> (config, in.par overrides, description, cpu-baseline steps)
 WORKLOADS = {
-    "euler_yee": ("euler_yee", {},
-                  "2-D Euler isentropic vortex, Nr=128 polar mesh (51,809 cells), "
-                  "HLLC, PLM 1.5, RK2, CFL 0.5, rigid mesh rotation", 150),
+    "mhd_kep": dict(
+        config="mhd_kep", over={}, steps=20, warmup=2,
+        desc="3-D Newtonian MHD Keplerian disk (MRI seed): r in [0.5,2], z in [-0.25,0.25], "
+             "Nr=256, Nz=64, Np=int(2 pi r/dr) per annulus, rho=1, cs=v_K/20, vertical field "
+             "at beta=1000, velocity noise 5e-5 v_phi; HLLD, PLM 1.5, constrained transport, "
+             "Keplerian mesh motion, RK2, CFL 0.5",
+        cpu_over={"Num_R": 64, "Num_Z": 16}, cpu_steps=4),
+    "euler_yee": dict(
+        config="euler_yee", over={}, steps=100, warmup=5,
+        desc="2-D Euler isentropic vortex, Nr=128 polar mesh (51,809 cells), HLLC, PLM 1.5, "
+             "RK2, CFL 0.5, rigid mesh rotation",
+        cpu_over={}, cpu_steps=150),
 }
-DEFAULT_WORKLOAD = "euler_yee"
+DEFAULT_WORKLOAD = "mhd_kep"
 
 
 def b_alg(nq: int, nf: int) -> int:
     return 40 * (nq + nf + 1)
 
 
-def kernel_bytes(name: str, nq: int, ntot: int, nreal: int, nfr: int, nfz: int) -> float:
-    """Algorithmic bytes of one launch of a kernel family: every input array
-    element the kernel needs read once and every output written once."""
-    d = 8
-    if name == "update":
-        # cons, rkcons (second substep: half the launches), prim in/out,
-        # phi/r/z face deltas, geometry in/out
-        return d * (nq * nreal * (1 + 0.5 + 1 + 1 + 1) + nq * (nreal + nfr + nfz) + 5 * ntot)
-    if name in ("flux_r", "flux_z"):
-        nf = nfr if name == "flux_r" else nfz
-        return 4 * 2 * nf + d * 2 * nf + d * (3 * nq * ntot + 2 * ntot) + d * nq * nf
-    if name in ("face_slopes_r", "face_slopes_z"):
-        nf = nfr if name == "face_slopes_r" else nfz
-        return d * (2 * nq * ntot + 2 * ntot) + 4 * 2 * nf + d * 2 * nf + d * nq * nf
-    if name in ("grad_r", "grad_z"):
-        nf = nfr if name == "grad_r" else nfz
-        return d * (nq * nf + nf + 2 * ntot + ntot) + d * nq * ntot
+def kernel_bytes(name: str, nq: int, mhd: bool, nt: int, nr: int, nfr: int, nfz: int) -> float:
+    """Algorithmic bytes of one launch of a kernel: every array element the
+    kernel needs read once and every output element written once (8-byte
+    doubles, 4-byte indices, 1-byte face orientations)."""
+    d, i = 8, 4
+    nf = {"r": nfr, "z": nfz}.get(name[-1:], 0)
+    rng = 2 * 8 * i * nt  # head/tail face ranges of both directions
     if name == "plm_phi":
-        return d * (nq * ntot + ntot + nq * ntot)
+        return d * (2 * nq * nt + nt) + i * nt
+    if name in ("faces_r", "faces_z"):
+        # topology out (L, R, LR, dA, phic, dphi), cell geometry and the
+        # phi-reconstructed states in, one slope per field out
+        return nf * (2 * i + 1 + 3 * d) + d * 2 * nt + d * 2 * nq * nt + d * nq * nf
+    if name in ("ranges_r", "ranges_z"):
+        return d * nt + i * nt + 8 * i * nt
+    if name in ("grad_r", "grad_z"):
+        return 8 * i * nt + d * (nq * nf + nf) + d * 2 * nt + d * nq * nt
     if name == "flux_phi":
-        return d * (2 * nq * ntot + 3 * ntot + nq * nreal)
-    if name == "wcell":
-        return d * 2 * ntot
-    if name == "ring_start":
-        return d * ntot
+        return d * (2 * nq * nt + 2 * nt + nq * nt) + (d * 4 * nt if mhd else 0)
+    if name in ("flux_r", "flux_z"):
+        return (2 * i + 2 * d) * nf + d * (3 * nq * nt + 2 * nt) + d * nq * nf + (3 * d * nf if mhd else 0)
+    if name == "edges_collect":
+        return d * (4 * nt + nt) + rng + d * 3 * (nfr + nfz) + d * 16 * nt
+    if name == "edges_avg":
+        return d * 32 * nt
+    if name in ("face_emf_r", "face_emf_z"):
+        return nf * (2 * i + 1) + d * 2 * nt + d * 16 * nt + d * 2 * nf
+    if name == "edges_final":
+        return d * 16 * nt + rng + d * 4 * (nt) + d * 2 * nt * 2 + d * nt + d * 8 * nt
+    if name == "update_B":
+        return d * 15 * nt + rng + d * 4 * nt + (2 * i + 1) * 2 * nt
+    if name == "edge_adjust":
+        return (nfr + nfz) * (2 * i + 1 + 2 * d) + d * 2 * nt + d * 4 * nt
+    if name == "update":
+        # cons and rkcons in, cons out (+ prim out for hydro), face deltas,
+        # ranges, geometry in and out
+        return (d * nq * nr * (3 + (0 if mhd else 1) + 1) + d * nq * (nt + nfr + nfz) + rng
+                + d * 5 * nt)
+    if name == "bfaces_c2p":
+        return rng + (nfr + nfz) * (2 * i + 1 + d) + d * 5 * nt + d * 2 * nt + d * nq * nr * 2
+    if name in ("flip_flags_r", "flip_flags_z"):
+        return nf * (2 * i + 1 + 1) + d * 2 * nt
     if name == "mindt":
-        return d * (nq * nreal + 3 * ntot)
+        return d * (nq * nr + 3 * nt)
+    if name == "wcell":
+        return d * nt + i * nt
+    if name == "ring_start":
+        return d * nt
     return 0.0
 
 
-def cpu_baseline(cfg, par, steps):
+def cpu_baseline(cfg, over, steps):
     exe = configs.ref_binary(cfg)
     if not os.path.exists(exe):
         return None
+    par = cfg.par_dict(**over)
     with tempfile.TemporaryDirectory(prefix="bench_ref_") as d:
         state_io.write_par(d, par)
         r = subprocess.run([exe, d, str(steps)], cwd=d, capture_output=True, text=True,
-                           timeout=600)
+                           timeout=900)
         if r.returncode != 0:
             return None
-        line = [l for l in r.stdout.splitlines() if l.startswith("{")][-1]
+        line = [ln for ln in r.stdout.splitlines() if ln.startswith("{")][-1]
         res = json.loads(line)
+    size = ", ".join(f"{k}={v}" for k, v in over.items()) or "the full mesh"
     return {"value": res["zone_updates_per_s"], "unit": "zone-updates/s", "cores": 1,
             "kind": "reference",
-            "sample": f"{steps} RK2 steps of the full mesh ({res['cells']} cells) with the "
-                      f"reference timestep built from its sources (gcc -O3, serial), "
-                      f"{res['seconds']:.1f} s"}
+            "sample": f"{steps} RK2 steps of the same workload at {size} ({res['cells']} cells) "
+                      f"with the reference timestep built from its sources (gcc -O3, serial, "
+                      f"USE_MPI=0), {res['seconds']:.1f} s of stepping"}
+
+
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
+    summary (profiles/traffic_<workload>.json), or None."""
+    path = os.path.join(REPO, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    row = table.get("kernels", {}).get(kernel)
+    return row.get("hbm_bytes_per_launch") if row else None
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timers", action="store_true")
     args = ap.parse_args(argv)
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the "
-                          "annulus mesh is described in DESIGN.md but not built",
-                          "n_gpus": args.gpus}))
+        print(json.dumps({"error": "disco_amd runs one mesh on one GPU; sharding the annulus "
+                          "mesh over several GPUs is not implemented", "n_gpus": args.gpus}))
         return 2
 
     import ctypes as C
     from disco_amd import capi
     from disco_amd.host_lib import HostRun
 
-    cname, over, desc, cpu_steps = WORKLOADS[args.workload]
-    cfg = configs.get(cname)
-    par = cfg.par_dict(**over)
+    w = WORKLOADS[args.workload]
+    steps = args.steps if args.steps is not None else w["steps"]
+    warmup = args.warmup if args.warmup is not None else w["warmup"]
+    cfg = configs.get(w["config"])
+    par = cfg.par_dict(**w["over"])
     tmp = tempfile.mkdtemp(prefix="bench_")
     par_path = state_io.write_par(tmp, par)
-    run = HostRun(cname, par_path)
+    t_setup = time.perf_counter()
+    run = HostRun(w["config"], par_path)
+    t_setup = time.perf_counter() - t_setup
     lib = capi.lib()
     ctx = C.c_void_p(run.ctx)
 
     def sync():
-        rc = lib.dh_sync(ctx)
-        if rc:
+        if lib.dh_sync(ctx):
             raise RuntimeError(lib.dh_last_error().decode())
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         run.timestep(run.getmindt())
     sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         run.timestep(run.getmindt())
     sync()
-    t1 = time.perf_counter()
-    secs = t1 - t0
+    secs = time.perf_counter() - t0
     ncell = run.real_cells
-    value = ncell * args.steps / secs
+    value = ncell * steps / secs
     nq, nf = cfg.nq, cfg.nf
+    mhd = cfg.hydro == "mhd"
+
+    nt = C.c_long()
+    lib.dh_sizes(ctx, C.byref(nt), None, None)
+    ntot = nt.value
+    nfaces = []
+    for dim in (1, 2):
+        n = C.c_long()
+        lib.dh_face_count(ctx, dim, C.byref(n))
+        nfaces.append(n.value)
+    nfr, nfz = nfaces
 
     roofline = None
     kernels = None
     if not args.no_kernel_timers:
+        tsteps = max(2, min(steps, 5))
         lib.dh_enable_timers(ctx, 1)
-        for _ in range(args.steps):
+        for _ in range(tsteps):
             run.timestep(run.getmindt())
         sync()
         cap = 64
@@ -157,25 +216,24 @@ def main(argv=None):
         n = C.c_int()
         lib.dh_read_timers(ctx, cap, names, ms, calls, C.byref(n))
         lib.dh_enable_timers(ctx, 0)
-        nt = C.c_long()
-        lib.dh_sizes(ctx, C.byref(nt), None, None)
-        ntot = nt.value
-        # face counts: every pair of neighbouring annuli has Np[a] + Np[b] faces
-        g = state_io  # noqa
-        nfr, nfz = _face_counts(par)
         kernels = {}
-        for i in range(n.value):
-            nm = bytes(names[i]).split(b"\0")[0].decode()
-            kernels[nm] = {"ms_per_step": ms[i] / args.steps, "launches": calls[i],
-                           "mean_ms": ms[i] / max(calls[i], 1)}
+        for k in range(n.value):
+            nm = bytes(names[k]).split(b"\0")[0].decode()
+            alg = kernel_bytes(nm, nq, mhd, ntot, ncell, nfr, nfz)
+            mean_s = ms[k] / max(calls[k], 1) * 1e-3
+            kernels[nm] = {"ms_per_step": ms[k] / tsteps, "launches_per_step": calls[k] / tsteps,
+                           "mean_launch_ms": mean_s * 1e3,
+                           "alg_bytes_per_launch": alg,
+                           "alg_GBps": alg / mean_s / 1e9 if mean_s > 0 and alg else None}
         dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
-        alg = kernel_bytes(dom, nq, ntot, ncell, nfr, nfz)
-        mean_s = kernels[dom]["mean_ms"] * 1e-3
-        ach = alg / mean_s / 1e9 if mean_s > 0 else 0.0
+        row = kernels[dom]
+        ach = row["alg_GBps"] or 0.0
+        traffic = measured_traffic(dom, args.workload)
         roofline = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS,
-                    "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": None,
-                    "algorithmic_bytes_per_launch": alg,
-                    "mean_launch_ms": kernels[dom]["mean_ms"]}
+                    "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": traffic,
+                    "algorithmic_bytes_per_launch": row["alg_bytes_per_launch"],
+                    "mean_launch_ms": row["mean_launch_ms"],
+                    "share_of_step": row["ms_per_step"] / sum(v["ms_per_step"] for v in kernels.values())}
 
     ba = b_alg(nq, nf)
     step_ach = ba * value / 1e9
@@ -184,15 +242,17 @@ def main(argv=None):
         "value": value,
         "unit": "zone-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": secs / args.steps * 1e3,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": secs / steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (initial conditions from the workload's Initial plugin)",
-        "config": {"workload": desc, "name": args.workload, "real_cells": ncell},
+        "data": "synthetic (initial conditions from the workload's Initial plugin, seeded noise)",
+        "config": {"workload": w["desc"], "name": args.workload, "real_cells": ncell,
+                   "cells_with_ghosts": ntot, "r_faces": nfr, "z_faces": nfz,
+                   "setup_s": t_setup},
         "roofline": roofline,
         "step_roofline": {"b_alg_bytes_per_zone_update": ba, "achieved": step_ach,
                           "peak": PEAK_HBM_GBS, "unit": "GB/s",
@@ -202,36 +262,9 @@ def main(argv=None):
     }
     run.close()
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg, par, cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(cfg, w["cpu_over"], w["cpu_steps"])
     print(json.dumps(out), flush=True)
     return 0
-
-
-def _face_counts(par):
-    """r- and z-face counts of the mesh (sum over neighbouring annulus pairs)."""
-    import types
-    d = types.SimpleNamespace()
-    nr = int(par["Num_R"])
-    nz = int(par.get("Num_Z", 1))
-    ng = 2
-    ngra = 0 if par.get("NoBC_Rmin", 0) else ng
-    ngrb = 0 if par.get("NoBC_Rmax", 0) else ng
-    ngza = 0 if (nz == 1 or par.get("NoBC_Zmin", 0)) else ng
-    ngzb = 0 if (nz == 1 or par.get("NoBC_Zmax", 0)) else ng
-    NR, NZ = nr + ngra + ngrb, nz + ngza + ngzb
-    rmin, rmax = float(par["R_Min"]), float(par["R_Max"])
-    aspect = float(par.get("aspect", 1.0))
-    npr = []
-    for j in range(NR):
-        x1 = (j - ngra + 1) / nr
-        x0 = (j - ngra) / nr
-        rp = rmin + x1 * (rmax - rmin)
-        rm = rmin + x0 * (rmax - rmin)
-        npr.append(max(4, int(2 * 3.141592653589793 / (aspect * (rp - rm) / rp))))
-    nfr = sum(npr[j] + npr[j + 1] for j in range(NR - 1)) * NZ
-    nfz = sum(2 * npr[j] for j in range(NR)) * (NZ - 1)
-    del d
-    return nfr, nfz
 
 
 if __name__ == "__main__":

@@ -52,6 +52,8 @@ HOST_CONFIGS = {
                       num_c=5, num_n=0, ct_mode=0),
     "euler_yee_n1": dict(initial="yee_vortex", planets="none", boundary="polar",
                          num_c=5, num_n=1, ct_mode=0),
+    "mhd_kep": dict(initial="kep_mhd_disk", planets="central", boundary="fixed_r_outflow_z",
+                    num_c=8, num_n=0, ct_mode=2),
 }
 
 

@@ -197,7 +197,7 @@ struct HipExec {
              double *rkPhi)
    {
       {
-         Bracket b(*this, "ct_flips");
+         Bracket b(*this, "flips");
          k_flips<<<(unsigned)((F.npairs + 3) / 4), TPB, 0, stream>>>(m, dim, F, flag, Phi, rkPhi);
       }
       LAUNCH_CHECK();

@@ -355,6 +355,58 @@ static void planets_pack(const struct domain *d, double *out)
    }
 }
 
+/* set_B_fields (bfields.c:13-86): the initial magnetic fluxes, from
+ * initial() at the upper phi face of every cell (Phi[0]) and at the centroid
+ * of every r and z face (the owner's Phi[1..4]); the device then forms the
+ * cell fields from them (B_faces_to_cells, bfields.c:80-81). */
+static int set_B_fields(struct domain *d)
+{
+   struct disco_accel *A = (struct disco_accel *)d->accel;
+   long nt = A->ntot;
+   int Nr = d->Nr, Nz = d->Nz;
+   double *Phi = (double *)calloc((size_t)nt * 5, sizeof(double));
+   if (!Phi) return host_fail("out of memory for the face fluxes");
+   double prim[NUM_Q];
+   for (int k = 0; k < Nz; ++k) {
+      double z = .5 * (d->z_kph[k] + d->z_kph[k - 1]);
+      for (int j = 0; j < Nr; ++j) {
+         double r = centroid_r(d->r_jph[j], d->r_jph[j - 1]);
+         double dA = (d->r_jph[j] - d->r_jph[j - 1]) * (d->z_kph[k] - d->z_kph[k - 1]);
+         int jk = j + Nr * k;
+         for (int i = 0; i < d->Np[jk]; ++i) {
+            double x[3] = {r, d->theCells[jk][i].piph, z};
+            initial(prim, x);
+            Phi[A->off[jk] + i] = NUM_Q > BPP ? prim[BPP] * dA : 0.0;
+         }
+      }
+   }
+   for (int dim = 1; dim <= 2; ++dim) {
+      if ((dim == 1 && Nr < 2) || (dim == 2 && Nz < 2)) continue;
+      long n = 0;
+      if (dh_face_count(A->ctx, dim, &n)) { free(Phi); return host_fail("%s", dh_last_error()); }
+      int *fL = (int *)malloc(sizeof(int) * n), *fR = (int *)malloc(sizeof(int) * n);
+      int *fLR = (int *)malloc(sizeof(int) * n);
+      double *cm = (double *)malloc(sizeof(double) * 3 * n), *dA = (double *)malloc(sizeof(double) * n);
+      int rc = (!fL || !fR || !fLR || !cm || !dA) ? DH_ENOMEM
+                                                   : dh_get_faces(A->ctx, dim, fL, fR, fLR, cm, dA);
+      if (!rc) {
+         int q = dim == 1 ? BRR : BZZ;
+         int sL = dim == 1 ? 2 : 4, sR = dim == 1 ? 1 : 3;
+         for (long f = 0; f < n; ++f) {
+            initial(prim, cm + 3 * f);
+            double v = NUM_Q > q ? prim[q] * dA[f] : 0.0;
+            if (fLR[f] == 0) Phi[sL * nt + fL[f]] = v;
+            else Phi[sR * nt + fR[f]] = v;
+         }
+      }
+      free(fL); free(fR); free(fLR); free(cm); free(dA);
+      if (rc) { free(Phi); return host_fail("dh_get_faces: %s", rc == DH_ENOMEM ? "out of memory" : dh_last_error()); }
+   }
+   int rc = dh_upload_state(A->ctx, NULL, NULL, NULL, NULL, NULL, Phi);
+   free(Phi);
+   return rc ? host_fail("dh_upload_state: %s", dh_last_error()) : 0;
+}
+
 int disco_setup(struct domain *d)
 {
    struct param_list *L = &d->theParList;
@@ -487,6 +539,7 @@ int disco_setup(struct domain *d)
    int rc = dh_upload_state(A->ctx, prim, NULL, piph, dphi, NULL, NULL);
    free(prim); free(piph); free(dphi);
    if (rc) return host_fail("dh_upload_state: %s", dh_last_error());
+   if (NUM_FACES == 5 && L->CT && set_B_fields(d)) return 1;
    if (dh_init_cells(A->ctx)) return host_fail("dh_init_cells: %s", dh_last_error());
    A->host_dirty = 1;
    return disco_sync_to_host(d);
@@ -653,15 +706,21 @@ int disco_write_dstate(struct domain *d, const char *path)
    struct disco_accel *A = (struct disco_accel *)d->accel;
    long nt = A->ntot;
    int Nr = d->Nr, Nz = d->Nz;
-   double *buf = (double *)malloc(sizeof(double) * nt * (2 * NUM_Q + 3));
+   int nf = NUM_FACES == 5 ? 5 : 0;
+   size_t nbuf = (size_t)nt * (2 * NUM_Q + 3 + nf);
+   double *buf = (double *)malloc(sizeof(double) * nbuf);
+   if (!buf) return host_fail("out of memory for the state dump");
    double *prim = buf, *cons = buf + nt * NUM_Q, *piph = cons + nt * NUM_Q;
-   double *dphi = piph + nt, *wiph = dphi + nt;
-   if (disco_export_state(d, prim, cons, piph, dphi, wiph)) { free(buf); return 1; }
+   double *dphi = piph + nt, *wiph = dphi + nt, *Phi = wiph + nt;
+   if (dh_download_state(A->ctx, prim, cons, piph, dphi, wiph, nf ? Phi : NULL)) {
+      free(buf);
+      return host_fail("dh_download_state: %s", dh_last_error());
+   }
    FILE *f = fopen(path, "wb");
    if (!f) { free(buf); return host_fail("cannot open %s", path); }
    char magic[8] = {'D', 'S', 'T', 'A', 'T', 'E', '0', '1'};
    int hdr[16] = {0};
-   hdr[0] = NUM_Q; hdr[1] = 0; hdr[2] = Nr; hdr[3] = Nz;
+   hdr[0] = NUM_Q; hdr[1] = nf; hdr[2] = Nr; hdr[3] = Nz;
    hdr[4] = d->NgRa; hdr[5] = d->NgRb; hdr[6] = d->NgZa; hdr[7] = d->NgZb;
    hdr[8] = (int)nt; hdr[9] = d->Npl; hdr[10] = d->count_steps; hdr[11] = NUM_C;
    double dhdr[8] = {d->t, d->phi_max, 0, 0, 0, 0, 0, 0};
@@ -671,7 +730,7 @@ int disco_write_dstate(struct domain *d, const char *path)
    fwrite(d->Np, sizeof(int), Nr * Nz, f);
    fwrite(d->r_jph - 1, sizeof(double), Nr + 1, f);
    fwrite(d->z_kph - 1, sizeof(double), Nz + 1, f);
-   fwrite(buf, sizeof(double), nt * (2 * NUM_Q + 3), f);
+   fwrite(buf, sizeof(double), nbuf, f);
    double pl[DH_MAX_PLANETS * DH_PLANET_STRIDE];
    planets_pack(d, pl);
    fwrite(pl, sizeof(double), d->Npl * DH_PLANET_STRIDE, f);

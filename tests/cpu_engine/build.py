@@ -32,3 +32,30 @@ def build(force: bool = False) -> str:
 
 if __name__ == "__main__":
     print(build(force="--force" in sys.argv))
+
+
+def build_driver(config: str) -> str:
+    """The `disco` driver of one module selection (disco_amd/host + plugins)
+    linked to the host build of the engine instead of libdisco_hip.so."""
+    sys.path.insert(0, REPO)
+    from disco_amd.build import HOST_CONFIGS
+    lib = build()
+    c = HOST_CONFIGS[config]
+    out = os.path.join(HERE, "_build", f"disco_{config}")
+    host = os.path.join(REPO, "disco_amd", "host")
+    plug = os.path.join(REPO, "disco_amd", "plugins")
+    srcs = [os.path.join(host, "main.c"), os.path.join(host, "disco_host.c"),
+            os.path.join(plug, "Initial", c["initial"] + ".c"),
+            os.path.join(plug, "Planets", c["planets"] + ".c")]
+    if os.path.exists(out) and all(os.path.getmtime(out) >= os.path.getmtime(s)
+                                   for s in srcs + [lib]):
+        return out
+    cmd = ["gcc", "-O2", "-Wall", "-Wno-unused-parameter", f"-DNUM_C={c['num_c']}",
+           f"-DNUM_N={c['num_n']}", f"-DCT_MODE={c['ct_mode']}", f'-DBOUNDARY="{c["boundary"]}"',
+           f'-DINITIAL="{c["initial"]}"', f"-I{os.path.join(REPO, 'include', 'disco')}",
+           f"-I{os.path.join(REPO, 'include')}", f"-I{host}", *srcs, "-o", out + ".tmp",
+           f"-L{os.path.dirname(lib)}", "-ldh_cpu", "-lm",
+           f"-Wl,-rpath,{os.path.dirname(lib)}"]
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
+    return out

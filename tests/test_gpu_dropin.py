@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from disco_amd import configs, state_io
-from helpers import compare_states
+from helpers import PARITY_TOL, compare_states, rel_l1
 
 pytestmark = pytest.mark.gpu
 HOST = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
@@ -22,6 +22,7 @@ HOST = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
 @pytest.mark.parametrize("cname,over,steps", [
     ("euler_yee", {}, 20),
     ("euler_yee_n1", {"Num_R": 64, "Riemann_Solver": 0}, 10),
+    ("mhd_kep", {"Num_R": 32, "Num_Z": 8}, 6),
 ])
 def test_disco_driver_matches_reference(cname, over, steps):
     cfg = configs.get(cname)
@@ -45,3 +46,8 @@ def test_disco_driver_matches_reference(cname, over, steps):
     worst = compare_states(got, want)
     assert abs(got.t - want.t) <= 1e-12 * want.t
     print(f"  worst rel L1 vs reference {worst:.2e}", flush=True)
+    if want.nf:
+        m = want.real_mask()
+        ephi = rel_l1(got.phi_flux[:, m], want.phi_flux[:, m])
+        print(f"  magnetic fluxes rel L1 {ephi:.2e}", flush=True)
+        assert ephi <= PARITY_TOL
