@@ -27,34 +27,69 @@
 
 namespace dh {
 
-// Per-cell face ranges of one direction (rng[d][k*ntot + c]):
-//   k = 0..3: inner pair (cell is R): head lo, head hi, tail lo, tail hi
-//   k = 4..7: outer pair (cell is L): same.  Empty ranges have lo > hi.
+// The faces of a cell in its inner and outer pair of one direction, as
+// inclusive head/tail ranges of the pair's face list (see ranges_as_a and
+// ranges_as_b in dh_mesh.h).  Empty ranges have lo > hi.
 struct CellRanges {
    int in_h0, in_h1, in_t0, in_t1, out_h0, out_h1, out_t0, out_t1;
    int base_in, base_out;   // first face of the inner / outer pair
    bool in, out;            // pair exists
 };
 
-DHD CellRanges load_ranges(const Mesh &m, int dim, const int *rng, const int *fI, int c, int a)
+// rng[0*nt + c] is the merge step at which cell c is consumed in its inner
+// pair (where its ring is the second), rng[1*nt + c] the same in its outer
+// pair (its ring first).  A cell's faces run from the step after its phi
+// predecessor's to its own; the ring's start cell also owns the faces after
+// the last cell's step (the walk ends on it again).
+DHD void range_from_steps(int h1, int hp, bool start, int nf, int *h0, int *t0, int *t1)
+{
+   if (start) {
+      *h0 = 0;
+      *t0 = hp + 1;
+      *t1 = nf - 1;
+   } else {
+      *h0 = hp + 1;
+      *t0 = 1;
+      *t1 = 0;
+   }
+   (void)h1;
+}
+
+DHD CellRanges load_ranges(const Mesh &m, int dim, const int *rng, const int *fI, long c, int a)
 {
    size_t nt = (size_t)m.ntot;
-   CellRanges r;
-   r.in_h0 = rng[0 * nt + c]; r.in_h1 = rng[1 * nt + c];
-   r.in_t0 = rng[2 * nt + c]; r.in_t1 = rng[3 * nt + c];
-   r.out_h0 = rng[4 * nt + c]; r.out_h1 = rng[5 * nt + c];
-   r.out_t0 = rng[6 * nt + c]; r.out_t1 = rng[7 * nt + c];
+   int c0 = m.off[a], np = m.Np[a];
+   int i = (int)(c - c0);
+   long cp = i == 0 ? c0 + np - 1 : c - 1;
+   bool start = i == m.I0[a];
    int j = a % m.Nr, k = a / m.Nr;
+   CellRanges r;
+   int Jin, Jout;
    if (dim == 1) {
       r.in = j > 0;
       r.out = j < m.Nr - 1;
-      r.base_in = r.in ? fI[(j - 1) + (m.Nr - 1) * k] : 0;
-      r.base_out = r.out ? fI[j + (m.Nr - 1) * k] : 0;
+      Jin = (j - 1) + (m.Nr - 1) * k;
+      Jout = j + (m.Nr - 1) * k;
    } else {
       r.in = k > 0;
       r.out = k < m.Nz - 1;
-      r.base_in = r.in ? fI[j + m.Nr * (k - 1)] : 0;
-      r.base_out = r.out ? fI[j + m.Nr * k] : 0;
+      Jin = j + m.Nr * (k - 1);
+      Jout = j + m.Nr * k;
+   }
+   r.base_in = r.base_out = 0;
+   r.in_h0 = r.in_t0 = r.out_h0 = r.out_t0 = 1;
+   r.in_h1 = r.in_t1 = r.out_h1 = r.out_t1 = 0;
+   if (r.in) {
+      r.base_in = fI[Jin];
+      r.in_h1 = rng[c];
+      range_from_steps(r.in_h1, rng[cp], start, fI[Jin + 1] - r.base_in, &r.in_h0, &r.in_t0,
+                       &r.in_t1);
+   }
+   if (r.out) {
+      r.base_out = fI[Jout];
+      r.out_h1 = rng[nt + c];
+      range_from_steps(r.out_h1, rng[nt + cp], start, fI[Jout + 1] - r.base_out, &r.out_h0,
+                       &r.out_t0, &r.out_t1);
    }
    return r;
 }

@@ -227,16 +227,17 @@ struct FaceGeo {
    double phic, dp, dA;
 };
 
-DHD FaceGeo face_geometry(const Mesh &m, int dim, const Pair &pr, const Ring &A,
-                          const Ring &B, int d, const double *piph, const double *dphi)
+// addFace (faces_alt.c:10-34, 135-175) for the face between cells cL (inner
+// or lower annulus) and cR of pair (j, k).
+DHD FaceGeo face_geometry(const Mesh &m, int dim, int j, int k, int cL, int cR,
+                          const double *piph, const double *dphi)
 {
    double Pmax = m.Pmax;
-   int x = merge_x(A, B, d, Pmax);
    FaceGeo g;
-   g.cL = ring_cell(A, x);
-   g.cR = ring_cell(B, d - x);
-   double pL = piph[g.cL], pR = piph[g.cR];
-   double dL = dphi[g.cL], dR = dphi[g.cR];
+   g.cL = cL;
+   g.cR = cR;
+   double pL = piph[cL], pR = piph[cR];
+   double dL = dphi[cL], dR = dphi[cR];
    double w = wrap_half(pL - pR, Pmax);
    g.LR = w > 0. ? 1 : 0;
    w = wrap_half(pL - dL - pR + dR, Pmax);
@@ -245,7 +246,6 @@ DHD FaceGeo face_geometry(const Mesh &m, int dim, const Pair &pr, const Ring &A,
    double xm1 = LRb == 0 ? pL - dL : pR - dR;
    g.dp = get_dp(xp1, xm1, Pmax);
    g.phic = get_dp(xp1, .5 * g.dp, Pmax);
-   int j = pr.j, k = pr.k;
    if (dim == 1) {
       double r = .5 * (m.r_jph[j] + m.r_jph[j]);
       double dz = m.z_kph[k] - m.z_kph[k - 1];
@@ -276,24 +276,23 @@ DHD void pair_dx(const Mesh &m, int dim, int j, int k, double *dxL, double *dxR)
 
 // Every r (dim 1) or z (dim 2) face and, when slopes is set, its one-sided
 // slope S = (W_R - W_L)/(dx_L + dx_R) of the phi-reconstructed states
-// (plm.c:84-110).
+// (plm.c:84-110).  One thread per cell of the inner (lower) annulus of a
+// pair writes the faces the cell bounds there: face h of that range lies
+// between the cell and element h - x of the outer ring (x = the cell's
+// element, or the ring length for the start cell's tail).
 template <int NQ>
 struct KFaces {
    Mesh m;
    int dim;
    Faces F;
-   const int *I0;
+   const int *rng;
    const double *piph, *dphi, *prim, *gradp;
    double *S;
    int slopes;
-   DHD void operator()(long f) const
+   DHD void face(int f, int j, int k, int cL, int cR, double dxL, double dxR) const
    {
       size_t nt = (size_t)m.ntot, nfs = (size_t)F.n;
-      int P = pair_of_face(F.fI, F.npairs, (int)f);
-      Pair pr = pair_of(m, dim, P);
-      Ring A = make_ring(piph, m, I0, pr.a);
-      Ring B = make_ring(piph, m, I0, pr.b);
-      FaceGeo g = face_geometry(m, dim, pr, A, B, (int)(f - F.fI[P]), piph, dphi);
+      FaceGeo g = face_geometry(m, dim, j, k, cL, cR, piph, dphi);
       F.L[f] = g.cL;
       F.R[f] = g.cR;
       F.LR[f] = (signed char)g.LR;
@@ -301,8 +300,6 @@ struct KFaces {
       F.phic[f] = g.phic;
       F.dphi[f] = g.dp;
       if (!slopes) return;
-      double dxL, dxR;
-      pair_dx(m, dim, pr.j, pr.k, &dxL, &dxR);
       double phiL = piph[g.cL] - .5 * dphi[g.cL];
       double phiR = piph[g.cR] - .5 * dphi[g.cR];
       double dpL = get_sdp(g.phic, phiL, m.Pmax);
@@ -314,14 +311,39 @@ struct KFaces {
          S[q * nfs + f] = (WR - WL) / (dxR + dxL);
       }
    }
+   DHD void operator()(long c) const
+   {
+      int a = m.cjk[c];
+      int j = a % m.Nr, k = a / m.Nr;
+      if (dim == 1 ? j >= m.Nr - 1 : k >= m.Nz - 1) return;
+      CellRanges cr = load_ranges(m, dim, rng, F.fI, c, a);
+      int b = dim == 1 ? a + 1 : a + m.Nr;
+      int bN = m.Np[b], b0 = m.off[b], bI0 = m.I0[b];
+      int aN = m.Np[a];
+      int x = (int)(c - m.off[a]) - m.I0[a];
+      if (x < 0) x += aN;
+      double dxL, dxR;
+      pair_dx(m, dim, j, k, &dxL, &dxR);
+      for (int part = 0; part < 2; ++part) {
+         int lo = part == 0 ? cr.out_h0 : cr.out_t0, hi = part == 0 ? cr.out_h1 : cr.out_t1;
+         int consumed = part == 0 ? x : aN;
+         for (int h = lo; h <= hi; ++h) {
+            int y = h - consumed;
+            if (y == bN) y = 0;
+            int ib = bI0 + y;
+            if (ib >= bN) ib -= bN;
+            face(cr.base_out + h, j, k, (int)c, b0 + ib, dxL, dxR);
+         }
+      }
+   }
 };
 
-// Each cell's faces in its inner and outer pair of one direction, as
-// head/tail ranges of the pair's list (merge-path counts, dh_mesh.h).
+// The merge steps at which each cell is consumed in its inner and outer pair
+// of one direction (one merge-path search each, dh_mesh.h); every face range
+// of the cell follows from them (load_ranges, dh_ct.h).
 struct KCellRanges {
    Mesh m;
    int dim;
-   const int *I0;
    const double *piph;
    int *rng;
    DHD void operator()(long c) const
@@ -330,27 +352,21 @@ struct KCellRanges {
       int a = m.cjk[c];
       int i = (int)(c - m.off[a]);
       int j = a % m.Nr, k = a / m.Nr;
-      Ring self = make_ring(piph, m, I0, a);
+      Ring self = make_ring(piph, m, m.I0, a);
       int xs = ring_elem(self, i);
       bool in = dim == 1 ? j > 0 : k > 0;
       bool out = dim == 1 ? j < m.Nr - 1 : k < m.Nz - 1;
-      FaceRange rin = {1, 0, 1, 0}, rout = {1, 0, 1, 0};
+      int s_in = 0, s_out = 0;
       if (in) {
-         Ring A = make_ring(piph, m, I0, dim == 1 ? a - 1 : a - m.Nr);
-         rin = ranges_as_b(A, self, xs, m.Pmax);
+         Ring A = make_ring(piph, m, m.I0, dim == 1 ? a - 1 : a - m.Nr);
+         s_in = b_step(A, self, xs, m.Pmax);
       }
       if (out) {
-         Ring B = make_ring(piph, m, I0, dim == 1 ? a + 1 : a + m.Nr);
-         rout = ranges_as_a(self, xs, B, m.Pmax);
+         Ring B = make_ring(piph, m, m.I0, dim == 1 ? a + 1 : a + m.Nr);
+         s_out = a_step(self, xs, B, m.Pmax);
       }
-      rng[0 * nt + c] = rin.h0;
-      rng[1 * nt + c] = rin.h1;
-      rng[2 * nt + c] = rin.t0;
-      rng[3 * nt + c] = rin.t1;
-      rng[4 * nt + c] = rout.h0;
-      rng[5 * nt + c] = rout.h1;
-      rng[6 * nt + c] = rout.t0;
-      rng[7 * nt + c] = rout.t1;
+      rng[c] = s_in;
+      rng[nt + c] = s_out;
    }
 };
 
@@ -1174,15 +1190,13 @@ struct KEdgeAdjust {
          double rline = m.r_jph[lj];
          for (int part = 0; part < 2; ++part) {
             int lo = part == 0 ? rg.h0 : rg.t0, hi = part == 0 ? rg.h1 : rg.t1;
+            // While this face is current, the merge has consumed x of its
+            // list (all N in the tail), so the facing face at step e is
+            // element e - x (e - N) of the other list.
+            int consumed = part == 0 ? x : self.N;
             for (int e = lo; e <= hi; ++e) {
-               int g;  // the facing face current at step e
-               if (self_first) {
-                  int y = e - merge_x(self, other, e, m.Pmax);
-                  g = obase + (y == other.N ? 0 : y);
-               } else {
-                  int xo = merge_x(other, self, e, m.Pmax);
-                  g = obase + (xo == other.N ? 0 : xo);
-               }
+               int y = e - consumed;
+               int g = obase + (y == other.N ? 0 : y);
                int f1 = self_first ? f : g, f2 = self_first ? g : f;
                EdgeCells q;
                if (dim == 1) {  // f1 = D face (c, cR), f2 = U face (cU, cUR)

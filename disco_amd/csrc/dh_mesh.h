@@ -25,6 +25,7 @@ struct Mesh {
    const double *r_jph;  // index -1 valid
    const double *z_kph;  // index -1 valid
    double Pmax;
+   const int *I0;        // [nann] ring start cells of the current substep
 };
 
 // One ring of cells read from its start cell I0: element x (0 <= x <= N) is

@@ -189,7 +189,7 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
       DH_TRY(alloc_n(x, &F.phic, sizeof(double) * n));
       DH_TRY(alloc_n(x, &F.dphi, sizeof(double) * n));
       DH_TRY(alloc_n(x, &b.dF[d], sizeof(double) * n * nq));
-      DH_TRY(alloc_n(x, &b.rng[d], sizeof(int) * 8 * (size_t)ntot));
+      DH_TRY(alloc_n(x, &b.rng[d], sizeof(int) * 2 * (size_t)ntot));
       if (b.mhd) {
          DH_TRY(alloc_n(x, &b.emf_f[d], sizeof(double) * 3 * n));
          DH_TRY(alloc_n(x, &b.fE[d], sizeof(double) * n));
@@ -215,6 +215,7 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
    DH_TRY(alloc_n(x, &b.dphi2, cb));
    DH_TRY(alloc_n(x, &b.wiph, cb));
    DH_TRY(alloc_n(x, &b.I0, sizeof(int) * nann));
+   m.I0 = b.I0;
    DH_TRY(x.zero(b.gradr, qb));
    DH_TRY(x.zero(b.gradz, qb));
    DH_TRY(x.zero(b.gradp, qb));
@@ -256,10 +257,10 @@ struct Engine {
          if (!dim_active(m, dim)) continue;
          double *grad = dim == 1 ? b.gradr : b.gradz;
          bool slopes = grads && b.P.plm != 0.0;
-         KFaces<NQ> kf{m, dim, b.F[d], b.I0, b.piph, b.dphi, b.prim, b.gradp, b.S, slopes ? 1 : 0};
-         DH_TRY(x.run(dim == 1 ? "faces_r" : "faces_z", kf, b.F[d].n));
-         KCellRanges kr{m, dim, b.I0, b.piph, b.rng[d]};
+         KCellRanges kr{m, dim, b.piph, b.rng[d]};
          DH_TRY(x.run(dim == 1 ? "ranges_r" : "ranges_z", kr, m.ntot));
+         KFaces<NQ> kf{m, dim, b.F[d], b.rng[d], b.piph, b.dphi, b.prim, b.gradp, b.S, slopes ? 1 : 0};
+         DH_TRY(x.run(dim == 1 ? "faces_r" : "faces_z", kf, m.ntot));
          if (!grads) continue;
          if (!slopes) {
             DH_TRY(x.zero(grad, sizeof(double) * NQ * m.ntot));
