@@ -38,6 +38,7 @@ struct Faces {
    int *L, *R;
    signed char *LR;
    double *dA, *phic, *dphi;
+   double *up;  // upper phi edge of the face (its owner's piph)
 };
 
 // ---------------------------------------------------------------------------
@@ -296,6 +297,7 @@ struct KFaces {
       F.L[f] = g.cL;
       F.R[f] = g.cR;
       F.LR[f] = (signed char)g.LR;
+      F.up[f] = piph[g.LR == 0 ? g.cL : g.cR];
       F.dA[f] = g.dA;
       F.phic[f] = g.phic;
       F.dphi[f] = g.dp;
@@ -1156,7 +1158,7 @@ struct KEdgeAdjust {
       int j = pr.j, k = pr.k;
       int base = F.fI[JK];
       int x = f - base;
-      FaceRing self = make_face_ring(piph, F.L, F.R, F.LR, base, F.fI[JK + 1] - base);
+      FaceRing self = make_face_ring(F.up, base, F.fI[JK + 1] - base);
       size_t slot = phi_slot(nt, dim, F.LR[f], F.L[f], F.R[f]);
       double v = Phi[slot];
       for (int role = 0; role < 2; ++role) {
@@ -1184,7 +1186,7 @@ struct KEdgeAdjust {
             }
          }
          int obase = F.fI[oJK];
-         FaceRing other = make_face_ring(piph, F.L, F.R, F.LR, obase, F.fI[oJK + 1] - obase);
+         FaceRing other = make_face_ring(F.up, obase, F.fI[oJK + 1] - obase);
          FaceRange rg = self_first ? ranges_as_a(self, x, other, m.Pmax)
                                    : ranges_as_b(other, self, x, m.Pmax);
          double rline = m.r_jph[lj];

@@ -56,40 +56,26 @@ DHD int ring_cell(const Ring &R, int x)
 
 // A face list of one annulus pair read as a ring: element x (0 <= x <= N) is
 // face f0 + (x mod N); its key is the face's upper phi edge, the upper edge
-// of whichever of its two cells ends first (faces_alt.c:147-151).
+// of whichever of its two cells ends first (faces_alt.c:147-151), stored per
+// face when the faces are built.
 struct FaceRing {
-   const double *piph;
-   const int *fL, *fR;
-   const signed char *fLR;
+   const double *up;
    int f0, N;
    double p0;
 };
 
-DHD double face_up(const double *piph, const int *fL, const int *fR, const signed char *fLR,
-                   int f)
-{
-   return fLR[f] == 0 ? piph[fL[f]] : piph[fR[f]];
-}
-
-DHD FaceRing make_face_ring(const double *piph, const int *fL, const int *fR,
-                            const signed char *fLR, int f0, int N)
+DHD FaceRing make_face_ring(const double *up, int f0, int N)
 {
    FaceRing R;
-   R.piph = piph;
-   R.fL = fL;
-   R.fR = fR;
-   R.fLR = fLR;
+   R.up = up;
    R.f0 = f0;
    R.N = N;
-   R.p0 = face_up(piph, fL, fR, fLR, f0);
+   R.p0 = up[f0];
    return R;
 }
 
 DHD double ring_key(const Ring &R, int x) { return R.piph[ring_cell(R, x)]; }
-DHD double ring_key(const FaceRing &R, int x)
-{
-   return face_up(R.piph, R.fL, R.fR, R.fLR, R.f0 + (x == R.N ? 0 : x));
-}
+DHD double ring_key(const FaceRing &R, int x) { return R.up[R.f0 + (x == R.N ? 0 : x)]; }
 
 // True when A's element x ends (in phi) no later than B's element y, i.e.
 // the reference walk advances A first (faces_alt.c:135-145: LR = 0 when the

@@ -188,6 +188,7 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
       DH_TRY(alloc_n(x, &F.dA, sizeof(double) * n));
       DH_TRY(alloc_n(x, &F.phic, sizeof(double) * n));
       DH_TRY(alloc_n(x, &F.dphi, sizeof(double) * n));
+      DH_TRY(alloc_n(x, &F.up, sizeof(double) * n));
       DH_TRY(alloc_n(x, &b.dF[d], sizeof(double) * n * nq));
       DH_TRY(alloc_n(x, &b.rng[d], sizeof(int) * 2 * (size_t)ntot));
       if (b.mhd) {
