@@ -275,24 +275,19 @@ DHD void pair_dx(const Mesh &m, int dim, int j, int k, double *dxL, double *dxR)
    }
 }
 
-// Every r (dim 1) or z (dim 2) face and, when slopes is set, its one-sided
-// slope S = (W_R - W_L)/(dx_L + dx_R) of the phi-reconstructed states
-// (plm.c:84-110).  One thread per cell of the inner (lower) annulus of a
-// pair writes the faces the cell bounds there: face h of that range lies
-// between the cell and element h - x of the outer ring (x = the cell's
-// element, or the ring length for the start cell's tail).
-template <int NQ>
+// Every r (dim 1) or z (dim 2) face (faces_alt.c:36-179).  One thread per
+// cell of the inner (lower) annulus of a pair writes the faces the cell
+// bounds there: face h of that range lies between the cell and element
+// h - x of the outer ring (x = the cell's element, or the ring length for
+// the start cell's tail).
 struct KFaces {
    Mesh m;
    int dim;
    Faces F;
    const int *rng;
-   const double *piph, *dphi, *prim, *gradp;
-   double *S;
-   int slopes;
-   DHD void face(int f, int j, int k, int cL, int cR, double dxL, double dxR) const
+   const double *piph, *dphi;
+   DHD void face(int f, int j, int k, int cL, int cR) const
    {
-      size_t nt = (size_t)m.ntot, nfs = (size_t)F.n;
       FaceGeo g = face_geometry(m, dim, j, k, cL, cR, piph, dphi);
       F.L[f] = g.cL;
       F.R[f] = g.cR;
@@ -301,17 +296,6 @@ struct KFaces {
       F.dA[f] = g.dA;
       F.phic[f] = g.phic;
       F.dphi[f] = g.dp;
-      if (!slopes) return;
-      double phiL = piph[g.cL] - .5 * dphi[g.cL];
-      double phiR = piph[g.cR] - .5 * dphi[g.cR];
-      double dpL = get_sdp(g.phic, phiL, m.Pmax);
-      double dpR = get_sdp(phiR, g.phic, m.Pmax);
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-         double WL = prim[q * nt + g.cL] + dpL * gradp[q * nt + g.cL];
-         double WR = prim[q * nt + g.cR] - dpR * gradp[q * nt + g.cR];
-         S[q * nfs + f] = (WR - WL) / (dxR + dxL);
-      }
    }
    DHD void operator()(long c) const
    {
@@ -324,8 +308,6 @@ struct KFaces {
       int aN = m.Np[a];
       int x = (int)(c - m.off[a]) - m.I0[a];
       if (x < 0) x += aN;
-      double dxL, dxR;
-      pair_dx(m, dim, j, k, &dxL, &dxR);
       for (int part = 0; part < 2; ++part) {
          int lo = part == 0 ? cr.out_h0 : cr.out_t0, hi = part == 0 ? cr.out_h1 : cr.out_t1;
          int consumed = part == 0 ? x : aN;
@@ -334,7 +316,7 @@ struct KFaces {
             if (y == bN) y = 0;
             int ib = bI0 + y;
             if (ib >= bN) ib -= bN;
-            face(cr.base_out + h, j, k, (int)c, b0 + ib, dxL, dxR);
+            face(cr.base_out + h, j, k, (int)c, b0 + ib);
          }
       }
    }
@@ -386,31 +368,51 @@ struct KGrad {
    int num_c;
    Faces F;
    const int *rng;
-   const double *piph, *dphi, *prim, *S;
+   const double *piph, *dphi, *prim, *gradp;
    double *grad;
+   // One-sided slope of face f (plm.c:84-110): S = (W_R - W_L)/(dx_L + dx_R)
+   // of the phi-reconstructed states at the face's phi centre.
+   DHD void face_slope(int f, double dxL, double dxR, double *S) const
+   {
+      size_t nt = (size_t)m.ntot;
+      int cL = F.L[f], cR = F.R[f];
+      double phic = F.phic[f];
+      double dpL = get_sdp(phic, piph[cL] - .5 * dphi[cL], m.Pmax);
+      double dpR = get_sdp(piph[cR] - .5 * dphi[cR], phic, m.Pmax);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+         double WL = prim[q * nt + cL] + dpL * gradp[q * nt + cL];
+         double WR = prim[q * nt + cR] - dpR * gradp[q * nt + cR];
+         S[q] = (WR - WL) / (dxR + dxL);
+      }
+   }
    DHD void operator()(long c) const
    {
-      size_t nt = (size_t)m.ntot, nfs = (size_t)F.n;
+      size_t nt = (size_t)m.ntot;
       int a = m.cjk[c];
       int j = a % m.Nr, k = a / m.Nr;
       CellRanges cr = load_ranges(m, dim, rng, F.fI, c, a);
-      double acc[NQ];
+      double dxLi = 0, dxRi = 0, dxLo = 0, dxRo = 0;
+      if (cr.in) pair_dx(m, dim, dim == 1 ? j - 1 : j, dim == 1 ? k : k - 1, &dxLi, &dxRi);
+      if (cr.out) pair_dx(m, dim, j, k, &dxLo, &dxRo);
+      double acc[NQ], Sf[NQ];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-      auto add_range = [&](int base, int lo, int hi) {
+      auto add_range = [&](int base, int lo, int hi, double dxL, double dxR) {
          for (int f = base + lo; f <= base + hi; ++f) {
             double dA = F.dA[f];
+            face_slope(f, dxL, dxR, Sf);
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) acc[q] += S[q * nfs + f] * dA;
+            for (int q = 0; q < NQ; ++q) acc[q] += Sf[q] * dA;
          }
       };
       if (cr.in) {
-         add_range(cr.base_in, cr.in_h0, cr.in_h1);
-         add_range(cr.base_in, cr.in_t0, cr.in_t1);
+         add_range(cr.base_in, cr.in_h0, cr.in_h1, dxLi, dxRi);
+         add_range(cr.base_in, cr.in_t0, cr.in_t1, dxLi, dxRi);
       }
       if (cr.out) {
-         add_range(cr.base_out, cr.out_h0, cr.out_h1);
-         add_range(cr.base_out, cr.out_t0, cr.out_t1);
+         add_range(cr.base_out, cr.out_h0, cr.out_h1, dxLo, dxRo);
+         add_range(cr.base_out, cr.out_t0, cr.out_t1, dxLo, dxRo);
       }
       double phip = piph[c];
       double phim = phip - dphi[c];
@@ -433,23 +435,24 @@ struct KGrad {
 #pragma unroll
       for (int q = 0; q < NQ; ++q) g[q] = acc[q] / dAtot;
       // Limit against every face slope; the outcome does not depend on order.
-      auto limit_range = [&](int base, int lo, int hi) {
+      auto limit_range = [&](int base, int lo, int hi, double dxL, double dxR) {
          for (int f = base + lo; f <= base + hi; ++f) {
+            face_slope(f, dxL, dxR, Sf);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
-               double s = S[q * nfs + f];
+               double s = Sf[q];
                if (s * g[q] < 0.0) g[q] = 0.0;
                else if (fabs(PLM * s) < fabs(g[q])) g[q] = PLM * s;
             }
          }
       };
       if (cr.in) {
-         limit_range(cr.base_in, cr.in_h0, cr.in_h1);
-         limit_range(cr.base_in, cr.in_t0, cr.in_t1);
+         limit_range(cr.base_in, cr.in_h0, cr.in_h1, dxLi, dxRi);
+         limit_range(cr.base_in, cr.in_t0, cr.in_t1, dxLi, dxRi);
       }
       if (cr.out) {
-         limit_range(cr.base_out, cr.out_h0, cr.out_h1);
-         limit_range(cr.base_out, cr.out_t0, cr.out_t1);
+         limit_range(cr.base_out, cr.out_h0, cr.out_h1, dxLo, dxRo);
+         limit_range(cr.base_out, cr.out_t0, cr.out_t1, dxLo, dxRo);
       }
       if (dim == 1 && j == 0 && m.NgRa == 0) {
          double xp0 = m.r_jph[0], xm0 = m.r_jph[-1];
@@ -910,32 +913,23 @@ struct KEdgesCollect {
    }
 };
 
-// avg_Efields, first part (bfields.c:400-441): each phi-edge takes the mean
-// of the values its two cells hold.
-struct KEdgesAvg {
-   Mesh m;
-   const double *Epre;
-   double *Eavg;
-   DHD void operator()(long c) const
-   {
-      size_t nt = (size_t)m.ntot;
-      int a = m.cjk[c];
-      int c0 = m.off[a], np = m.Np[a];
+// avg_Efields, first part (bfields.c:400-441), evaluated where it is read:
+// each phi-edge takes the mean of the values its two cells hold.  Slot s of
+// E (h = 0) or B (h = 1) of cell c.
+DHD double edge_avg(const Mesh &m, const double *Epre, int h, int s, long c)
+{
+   size_t nt = (size_t)m.ntot;
+   const double *X = Epre + (size_t)h * 8 * nt;
+   int a = m.cjk[c];
+   int c0 = m.off[a], np = m.Np[a];
+   int e = s & 3, base = s & 4;
+   if (e < 2) {
       long cp = c - c0 == np - 1 ? c0 : c + 1;
-      long cm = c == c0 ? c0 + np - 1 : c - 1;
-      for (int h = 0; h < 2; ++h) {  // h = 0: E, h = 1: B
-         const double *X = Epre + (size_t)h * 8 * nt;
-         double *Y = Eavg + (size_t)h * 8 * nt;
-         for (int s = 0; s < 2; ++s) {  // s = 0: z-directed edges, 1: r-directed
-            int e0 = 4 * s, e1 = e0 + 1, e2 = e0 + 2, e3 = e0 + 3;
-            Y[e0 * nt + c] = .5 * (X[e0 * nt + c] + X[e2 * nt + cp]);
-            Y[e1 * nt + c] = .5 * (X[e1 * nt + c] + X[e3 * nt + cp]);
-            Y[e2 * nt + c] = .5 * (X[e0 * nt + cm] + X[e2 * nt + c]);
-            Y[e3 * nt + c] = .5 * (X[e1 * nt + cm] + X[e3 * nt + c]);
-         }
-      }
+      return .5 * (X[(base + e) * nt + c] + X[(base + e + 2) * nt + cp]);
    }
-};
+   long cm = c == c0 ? c0 + np - 1 : c - 1;
+   return .5 * (X[(base + e - 2) * nt + cm] + X[(base + e) * nt + c]);
+}
 
 // avg_Efields, second part (bfields.c:443-531): the edge at a face's upper
 // end is the mean of its owner's value and the other cell's two corner
@@ -944,29 +938,23 @@ struct KFaceEmf {
    Mesh m;
    int dim;
    Faces F;
-   const double *piph, *dphi, *Eavg;
+   const double *piph, *dphi, *Epre;
    double *fE, *fB;
    DHD void operator()(long f) const
    {
-      size_t nt = (size_t)m.ntot;
-      const double *E = Eavg, *B = Eavg + 8 * nt;
       int L = F.L[f], R = F.R[f];
       int LR = F.LR[f];
       int c1 = LR == 0 ? L : R, c2 = LR == 0 ? R : L;
       double dp1 = get_dp(piph[c2], piph[c1], m.Pmax);
       double dp2 = dphi[c2] - dp1;
       int eL = dim == 1 ? 1 : 5, eR = dim == 1 ? 0 : 4;
-      if (LR == 0) {
-         double Ea = (dp2 * E[eR * nt + c2] + dp1 * E[(eR + 2) * nt + c2]) / (dp1 + dp2);
-         double Ba = (dp2 * B[eR * nt + c2] + dp1 * B[(eR + 2) * nt + c2]) / (dp1 + dp2);
-         fE[f] = .5 * (E[eL * nt + L] + Ea);
-         fB[f] = .5 * (B[eL * nt + L] + Ba);
-      } else {
-         double Ea = (dp2 * E[eL * nt + c2] + dp1 * E[(eL + 2) * nt + c2]) / (dp1 + dp2);
-         double Ba = (dp2 * B[eL * nt + c2] + dp1 * B[(eL + 2) * nt + c2]) / (dp1 + dp2);
-         fE[f] = .5 * (E[eR * nt + R] + Ea);
-         fB[f] = .5 * (B[eR * nt + R] + Ba);
-      }
+      // LRtype 0: the face ends at L's upper edge; R's corners are interpolated
+      int eo = LR == 0 ? eL : eR, ec = LR == 0 ? eR : eL;
+      int co = LR == 0 ? L : R;
+      double Ea = (dp2 * edge_avg(m, Epre, 0, ec, c2) + dp1 * edge_avg(m, Epre, 0, ec + 2, c2)) / (dp1 + dp2);
+      double Ba = (dp2 * edge_avg(m, Epre, 1, ec, c2) + dp1 * edge_avg(m, Epre, 1, ec + 2, c2)) / (dp1 + dp2);
+      fE[f] = .5 * (edge_avg(m, Epre, 0, eo, co) + Ea);
+      fB[f] = .5 * (edge_avg(m, Epre, 1, eo, co) + Ba);
    }
 };
 
@@ -977,7 +965,7 @@ struct EdgeArgs {
    Mesh m;
    Faces Fr, Fz;
    const int *rngr, *rngz;
-   const double *Eavg, *fEr, *fBr, *fEz, *fBz;
+   const double *Epre, *fEr, *fBr, *fEz, *fBz;
 };
 
 DHD void edges_post_face(const EdgeArgs &g, long c, double *out)
@@ -985,9 +973,12 @@ DHD void edges_post_face(const EdgeArgs &g, long c, double *out)
    const Mesh &m = g.m;
    size_t nt = (size_t)m.ntot;
    int a = m.cjk[c];
-   const double *E = g.Eavg, *B = g.Eavg + 8 * nt;
-   out[0] = E[0 * nt + c]; out[1] = E[1 * nt + c]; out[2] = E[4 * nt + c]; out[3] = E[5 * nt + c];
-   out[4] = B[0 * nt + c]; out[5] = B[1 * nt + c]; out[6] = B[4 * nt + c]; out[7] = B[5 * nt + c];
+   const int slots[4] = {0, 1, 4, 5};
+#pragma unroll
+   for (int s = 0; s < 4; ++s) {
+      out[s] = edge_avg(m, g.Epre, 0, slots[s], c);
+      out[4 + s] = edge_avg(m, g.Epre, 1, slots[s], c);
+   }
    if (m.Nr > 1) {
       CellRanges cr = load_ranges(m, 1, g.rngr, g.Fr.fI, c, a);
       if (cr.in) { int f = cr.base_in + cr.in_h1; out[0] = g.fEr[f]; out[4] = g.fBr[f]; }

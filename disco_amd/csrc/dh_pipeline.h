@@ -51,12 +51,12 @@ struct Buffers {
    double *gradp = nullptr, *gradr = nullptr, *gradz = nullptr;
    double *piph = nullptr, *piph2 = nullptr, *dphi = nullptr, *dphi2 = nullptr;
    double *wiph = nullptr;
-   double *S = nullptr, *dF[2] = {nullptr, nullptr}, *dFp = nullptr;
+   double *dF[2] = {nullptr, nullptr}, *dFp = nullptr;
    int *I0 = nullptr;
    // constrained transport
    double *Phi = nullptr, *rkPhi = nullptr, *emf_phi = nullptr;
    double *emf_f[2] = {nullptr, nullptr}, *fE[2] = {nullptr, nullptr}, *fB[2] = {nullptr, nullptr};
-   double *Epre = nullptr, *Eavg = nullptr, *Eo = nullptr, *Eown = nullptr, *Ecarry = nullptr;
+   double *Epre = nullptr, *Eo = nullptr, *Eown = nullptr, *Ecarry = nullptr;
    signed char *flag[2] = {nullptr, nullptr};
    PlanetSet pl[2];
 };
@@ -199,10 +199,8 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
          DH_TRY(x.zero(b.emf_f[d], sizeof(double) * 3 * n));
       }
    }
-   int nfmax = b.F[0].n > b.F[1].n ? b.F[0].n : b.F[1].n;
    size_t qb = sizeof(double) * (size_t)ntot * nq;
    size_t cb = sizeof(double) * (size_t)ntot;
-   DH_TRY(alloc_n(x, &b.S, sizeof(double) * (size_t)nfmax * nq));
    DH_TRY(alloc_n(x, &b.prim, qb));
    DH_TRY(alloc_n(x, &b.cons, qb));
    DH_TRY(alloc_n(x, &b.rkcons, qb));
@@ -228,7 +226,6 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
       DH_TRY(alloc_n(x, &b.rkPhi, 5 * cb));
       DH_TRY(alloc_n(x, &b.emf_phi, 4 * cb));
       DH_TRY(alloc_n(x, &b.Epre, 16 * cb));
-      DH_TRY(alloc_n(x, &b.Eavg, 16 * cb));
       DH_TRY(alloc_n(x, &b.Eo, 4 * cb));
       DH_TRY(alloc_n(x, &b.Eown, 4 * cb));
       DH_TRY(alloc_n(x, &b.Ecarry, 16 * cb));
@@ -245,8 +242,8 @@ struct Engine {
    X &x;
    Buffers &b;
 
-   // Ring starts, faces (+ slopes) and cell ranges of both directions, and
-   // the r/z slopes when grads is set (plm.c:56-232).
+   // Ring starts, cell ranges and faces of both directions, and the r/z
+   // slopes when grads is set (plm.c:56-232).
    template <int NQ>
    int topology(bool grads)
    {
@@ -260,14 +257,15 @@ struct Engine {
          bool slopes = grads && b.P.plm != 0.0;
          KCellRanges kr{m, dim, b.piph, b.rng[d]};
          DH_TRY(x.run(dim == 1 ? "ranges_r" : "ranges_z", kr, m.ntot));
-         KFaces<NQ> kf{m, dim, b.F[d], b.rng[d], b.piph, b.dphi, b.prim, b.gradp, b.S, slopes ? 1 : 0};
+         KFaces kf{m, dim, b.F[d], b.rng[d], b.piph, b.dphi};
          DH_TRY(x.run(dim == 1 ? "faces_r" : "faces_z", kf, m.ntot));
          if (!grads) continue;
          if (!slopes) {
             DH_TRY(x.zero(grad, sizeof(double) * NQ * m.ntot));
             continue;
          }
-         KGrad<NQ> kg{m, dim, b.P.plm, b.P.num_c, b.F[d], b.rng[d], b.piph, b.dphi, b.prim, b.S, grad};
+         KGrad<NQ> kg{m, dim, b.P.plm, b.P.num_c, b.F[d], b.rng[d], b.piph, b.dphi, b.prim,
+                      b.gradp, grad};
          DH_TRY(x.run(dim == 1 ? "grad_r" : "grad_z", kg, m.ntot));
       }
       return DH_OK;
@@ -317,14 +315,13 @@ struct Engine {
          KEdgesCollect kc{m, ct, b.dphi, b.emf_phi, b.F[0], b.F[1], b.rng[0], b.rng[1],
                           b.emf_f[0], b.emf_f[1], b.Ecarry, b.Epre};
          DH_TRY(x.run("edges_collect", kc, m.ntot));
-         DH_TRY(x.run("edges_avg", KEdgesAvg{m, b.Epre, b.Eavg}, m.ntot));
          for (int dim = 1; dim <= 2; ++dim) {
             int d = dim - 1;
             if (!dim_active(m, dim)) continue;
-            KFaceEmf kfe{m, dim, b.F[d], b.piph, b.dphi, b.Eavg, b.fE[d], b.fB[d]};
+            KFaceEmf kfe{m, dim, b.F[d], b.piph, b.dphi, b.Epre, b.fE[d], b.fB[d]};
             DH_TRY(x.run(dim == 1 ? "face_emf_r" : "face_emf_z", kfe, b.F[d].n));
          }
-         EdgeArgs ea{m, b.F[0], b.F[1], b.rng[0], b.rng[1], b.Eavg, b.fE[0], b.fB[0], b.fE[1], b.fB[1]};
+         EdgeArgs ea{m, b.F[0], b.F[1], b.rng[0], b.rng[1], b.Epre, b.fE[0], b.fB[0], b.fE[1], b.fB[1]};
          KEdgesFinal kef{ea, ct, b.wiph, b.emf_f[0], b.emf_f[1], b.Eo, b.Eown, b.Ecarry};
          DH_TRY(x.run("edges_final", kef, m.ntot));
          KUpdateB kb{m, RK, dt, b.F[0], b.F[1], b.rng[0], b.rng[1], b.Eo, b.Phi, b.rkPhi};
