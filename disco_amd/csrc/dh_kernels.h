@@ -280,9 +280,10 @@ DHD void pair_dx(const Mesh &m, int dim, int j, int k, double *dxL, double *dxR)
 // bounds there: face h of that range lies between the cell and element
 // h - x of the outer ring (x = the cell's element, or the ring length for
 // the start cell's tail).
+template <int DIM>
 struct KFaces {
    Mesh m;
-   int dim;
+   static constexpr int dim = DIM;
    Faces F;
    const int *rng;
    const double *piph, *dphi;
@@ -325,9 +326,10 @@ struct KFaces {
 // The merge steps at which each cell is consumed in its inner and outer pair
 // of one direction (one merge-path search each, dh_mesh.h); every face range
 // of the cell follows from them (load_ranges, dh_ct.h).
+template <int DIM>
 struct KCellRanges {
    Mesh m;
-   int dim;
+   static constexpr int dim = DIM;
    const double *piph;
    int *rng;
    DHD void operator()(long c) const
@@ -360,10 +362,10 @@ struct KCellRanges {
 // the r = 0 axis get the polar limiter (plm.c:197-232, cylindrical.c:141-173).
 // ---------------------------------------------------------------------------
 
-template <int NQ>
+template <int NQ, int DIM>
 struct KGrad {
    Mesh m;
-   int dim;
+   static constexpr int dim = DIM;
    double PLM;
    int num_c;
    Faces F;
@@ -561,11 +563,11 @@ struct KFluxPhi {
 // Flux through every r (dim 1) or z (dim 2) face of the flux range
 // (riemann.c:114-213), stored as F dA dt (these faces do not move: w = 0);
 // for MHD also the face's {E_rz, B_rz, E_phi}.
-template <int NQ, class H>
+template <int NQ, class H, int DIM>
 struct KFluxTrans {
    Mesh m;
    dh_params P;
-   int dim;
+   static constexpr int dim = DIM;
    double dt;
    Faces F;
    const double *prim, *gradp, *gradr, *gradz, *piph, *dphi;
@@ -934,9 +936,10 @@ DHD double edge_avg(const Mesh &m, const double *Epre, int h, int s, long c)
 // avg_Efields, second part (bfields.c:443-531): the edge at a face's upper
 // end is the mean of its owner's value and the other cell's two corner
 // values interpolated in phi.
+template <int DIM>
 struct KFaceEmf {
    Mesh m;
-   int dim;
+   static constexpr int dim = DIM;
    Faces F;
    const double *piph, *dphi, *Epre;
    double *fE, *fB;
@@ -1132,10 +1135,73 @@ struct KUpdateB {
 // +-E dl dt for each, in the reference's line and step order.
 //   r face of pair (j,k): U role on line (j,k-1) [-], D role on line (j,k) [+]
 //   z face of pair (j,k): R role on line (j-1,k) [+], L role on line (j,k) [-]
+// The facing list of a role, and whether this face's list is the first of
+// the line's merge.
+struct EdgeRole {
+   bool ok, self_first;
+   int oJK, lj;
+   double sgn;
+};
+
+DHD EdgeRole edge_role(const Mesh &m, int dim, int j, int k, int role)
+{
+   EdgeRole r;
+   r.ok = true;
+   if (dim == 1) {
+      r.lj = j;
+      if (role == 0) {
+         r.ok = k > 0;
+         r.oJK = pair_index(m, 1, j, k - 1); r.self_first = false; r.sgn = -1.0;
+      } else {
+         r.ok = k < m.Nz - 1;
+         r.oJK = pair_index(m, 1, j, k + 1); r.self_first = true; r.sgn = +1.0;
+      }
+   } else {
+      if (role == 0) {
+         r.ok = j > 0;
+         r.oJK = pair_index(m, 2, j - 1, k); r.self_first = false; r.sgn = +1.0; r.lj = j - 1;
+      } else {
+         r.ok = j < m.Nr - 1;
+         r.oJK = pair_index(m, 2, j + 1, k); r.self_first = true; r.sgn = -1.0; r.lj = j;
+      }
+   }
+   return r;
+}
+
+// The merge step at which each face is consumed on each of its two edge
+// lines (one merge-path search per role); a face's steps on a line run from
+// the step after its predecessor's to its own (cf. load_ranges).
+template <int DIM>
+struct KLineSteps {
+   Mesh m;
+   Faces F;
+   int *estep;  // [2][F.n]
+   DHD void operator()(long t) const
+   {
+      int f = (int)t;
+      int JK = pair_of_face(F.fI, F.npairs, f);
+      Pair pr = pair_of(m, DIM, JK);
+      int base = F.fI[JK];
+      FaceRing self = make_face_ring(F.up, base, F.fI[JK + 1] - base);
+      int x = f - base;
+      for (int role = 0; role < 2; ++role) {
+         EdgeRole er = edge_role(m, DIM, pr.j, pr.k, role);
+         int step = 0;
+         if (er.ok) {
+            int obase = F.fI[er.oJK];
+            FaceRing other = make_face_ring(F.up, obase, F.fI[er.oJK + 1] - obase);
+            step = er.self_first ? a_step(self, x, other, m.Pmax) : b_step(other, self, x, m.Pmax);
+         }
+         estep[(size_t)role * F.n + f] = step;
+      }
+   }
+};
+
 struct KEdgeAdjust {
    Mesh m;
    double dt;
    Faces Fr, Fz;
+   const int *estep_r, *estep_z;
    const double *piph, *dphi, *Eown;
    double *Phi;
    DHD void operator()(long t) const
@@ -1143,54 +1209,34 @@ struct KEdgeAdjust {
       size_t nt = (size_t)m.ntot;
       int dim = t < Fr.n ? 1 : 2;
       const Faces &F = dim == 1 ? Fr : Fz;
+      const int *estep = dim == 1 ? estep_r : estep_z;
       int f = (int)(dim == 1 ? t : t - Fr.n);
       int JK = pair_of_face(F.fI, F.npairs, f);
       Pair pr = pair_of(m, dim, JK);
-      int j = pr.j, k = pr.k;
       int base = F.fI[JK];
+      int N = F.fI[JK + 1] - base;
       int x = f - base;
-      FaceRing self = make_face_ring(F.up, base, F.fI[JK + 1] - base);
+      int fp = base + (x == 0 ? N - 1 : x - 1);
       size_t slot = phi_slot(nt, dim, F.LR[f], F.L[f], F.R[f]);
       double v = Phi[slot];
       for (int role = 0; role < 2; ++role) {
-         int oJK, lj;
-         bool self_first;  // self is the first (D or L) list of the merge
-         double sgn;
-         if (dim == 1) {
-            if (role == 0) {
-               if (k == 0) continue;
-               oJK = pair_index(m, 1, j, k - 1); self_first = false; sgn = -1.0;
-            } else {
-               if (k >= m.Nz - 1) continue;
-               oJK = pair_index(m, 1, j, k + 1); self_first = true; sgn = +1.0;
-            }
-            lj = j;
-         } else {
-            if (role == 0) {
-               if (j == 0) continue;
-               oJK = pair_index(m, 2, j - 1, k); self_first = false; sgn = +1.0;
-               lj = j - 1;
-            } else {
-               if (j >= m.Nr - 1) continue;
-               oJK = pair_index(m, 2, j + 1, k); self_first = true; sgn = -1.0;
-               lj = j;
-            }
-         }
-         int obase = F.fI[oJK];
-         FaceRing other = make_face_ring(F.up, obase, F.fI[oJK + 1] - obase);
-         FaceRange rg = self_first ? ranges_as_a(self, x, other, m.Pmax)
-                                   : ranges_as_b(other, self, x, m.Pmax);
-         double rline = m.r_jph[lj];
+         EdgeRole er = edge_role(m, dim, pr.j, pr.k, role);
+         if (!er.ok) continue;
+         int obase = F.fI[er.oJK];
+         int oN = F.fI[er.oJK + 1] - obase;
+         int h0, h1 = estep[(size_t)role * F.n + f], t0, t1;
+         range_from_steps(h1, estep[(size_t)role * F.n + fp], x == 0, N + oN, &h0, &t0, &t1);
+         double rline = m.r_jph[er.lj];
          for (int part = 0; part < 2; ++part) {
-            int lo = part == 0 ? rg.h0 : rg.t0, hi = part == 0 ? rg.h1 : rg.t1;
+            int lo = part == 0 ? h0 : t0, hi = part == 0 ? h1 : t1;
             // While this face is current, the merge has consumed x of its
             // list (all N in the tail), so the facing face at step e is
             // element e - x (e - N) of the other list.
-            int consumed = part == 0 ? x : self.N;
+            int consumed = part == 0 ? x : N;
             for (int e = lo; e <= hi; ++e) {
                int y = e - consumed;
-               int g = obase + (y == other.N ? 0 : y);
-               int f1 = self_first ? f : g, f2 = self_first ? g : f;
+               int g = obase + (y == oN ? 0 : y);
+               int f1 = er.self_first ? f : g, f2 = er.self_first ? g : f;
                EdgeCells q;
                if (dim == 1) {  // f1 = D face (c, cR), f2 = U face (cU, cUR)
                   q.c = F.L[f1]; q.cR = F.R[f1]; q.cU = F.L[f2]; q.cUR = F.R[f2];
@@ -1200,7 +1246,7 @@ struct KEdgeAdjust {
                double E, dl;
                edge_segment(piph, dphi, Eown, nt, q, rline, m.Pmax, &E, &dl);
                double Edldt = E * dl * dt;
-               if (sgn > 0) v += Edldt;
+               if (er.sgn > 0) v += Edldt;
                else v -= Edldt;
             }
          }
@@ -1212,9 +1258,10 @@ struct KEdgeAdjust {
 // check_flipped (bfields.c:693-712): after the mesh moved, a face whose
 // upper end now lies before its predecessor's has vanished.  Positions are
 // the moved ones before clean_pi, as the reference has them.
+template <int DIM>
 struct KFlipFlags {
    Mesh m;
-   int dim;
+   static constexpr int dim = DIM;
    Faces F;
    const double *piph, *wiph;
    double dt;
@@ -1349,77 +1396,132 @@ struct KBfacesC2p {
 };
 
 // ---------------------------------------------------------------------------
-// Zero-gradient ghost annuli, one line of ghost annuli per thread, in the
-// reference's order (boundary_functions.c:90-159, 269-441).
+// Zero-gradient ghost annuli (set_cells_copy_distant, boundary_functions.c:
+// 90-159, and the diode of outflow3d, boundary_functions.c:269-441): each
+// ghost cell is the phi-overlap-weighted mean of the cells of the next
+// annulus inward.  The reference walks both rings once per ghost annulus;
+// here one thread per ghost annulus runs that walk over positions only and
+// records where each ghost cell's overlap sum starts, then one thread per
+// ghost cell forms the sum, layer by layer (a layer copies from the one
+// inside it).
 //   side 0: r inner, 1: r outer, 2: z bottom, 3: z top.
 // ---------------------------------------------------------------------------
 
-template <int NQ>
-DHD void copy_distant(const Mesh &m, double *prim, const double *piph, const double *dphi,
-                      int dst, int src)
+// Destination and source annulus of ghost layer g of one line of a side.
+DHD void bc_pair(const Mesh &m, int side, int line, int g, int *dst, int *src)
 {
-   size_t nt = (size_t)m.ntot;
-   double pm = m.Pmax;
-   int c0 = m.off[dst], s0 = m.off[src];
-   int Np = m.Np[dst], Np1 = m.Np[src];
-   int i1;
-   double phi0 = piph[c0] - dphi[c0];
-   for (i1 = 0; i1 < Np1; i1++) {
-      double dphip = get_sdp(piph[s0 + i1], phi0, pm);
-      double dphim = dphip - dphi[s0 + i1];
-      if (dphip > 0 && dphim <= 0.0) break;
-   }
-   if (i1 >= Np1) i1 = 0;
-   for (int i = 0; i < Np; ++i) {
-      double phip = piph[c0 + i];
-      double phim = piph[c0 + i] - dphi[c0 + i];
-      double phip1, phim1;
-      double acc[NQ];
-      for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-      int guard = 0;
-      do {
-         phip1 = piph[s0 + i1];
-         phim1 = piph[s0 + i1] - dphi[s0 + i1];
-         if (std::isfinite(phip1) && std::isfinite(phim1) && std::isfinite(phip) && std::isfinite(phim)) {
-            while (phip1 < phim) { phip1 += pm; phim1 += pm; }
-            while (phim1 > phip) { phip1 -= pm; phim1 -= pm; }
-         }
-         double phi1 = phip < phip1 ? phip : phip1;
-         double phi2 = phim > phim1 ? phim : phim1;
-         double d = phi1 - phi2;
-         for (int q = 0; q < NQ; ++q) acc[q] += prim[q * nt + s0 + i1] * d;
-         i1 = i1 < Np1 - 1 ? i1 + 1 : 0;
-      } while (phip1 < phip && ++guard <= Np1);
-      i1 = i1 > 0 ? i1 - 1 : Np1 - 1;
-      for (int q = 0; q < NQ; ++q) prim[q * nt + c0 + i] = acc[q] / dphi[c0 + i];
-   }
+   if (side == 0) { int j = m.NgRa - 1 - g; *dst = j + m.Nr * line; *src = *dst + 1; }
+   else if (side == 1) { int j = m.Nr - m.NgRb + g; *dst = j + m.Nr * line; *src = *dst - 1; }
+   else if (side == 2) { int k = m.NgZa - 1 - g; *dst = line + m.Nr * k; *src = *dst + m.Nr; }
+   else { int k = m.Nz - m.NgZb + g; *dst = line + m.Nr * k; *src = *dst - m.Nr; }
 }
 
-template <int NQ>
-struct KBcCopy {
+DHD int bc_lines(const Mesh &m, int side) { return side < 2 ? m.Nz : m.Nr; }
+DHD int bc_layers(const Mesh &m, int side)
+{
+   return side == 0 ? m.NgRa : side == 1 ? m.NgRb : side == 2 ? m.NgZa : m.NgZb;
+}
+
+// Overlap in phi of destination cell (phim, phip] with source cell i1,
+// shifted by whole periods next to it as the reference does; returns the
+// shifted upper edge of the source cell.
+DHD double bc_overlap(double phim, double phip, const double *piph, const double *dphi, int s,
+                      double pm, double *d)
+{
+   double phip1 = piph[s];
+   double phim1 = piph[s] - dphi[s];
+   if (std::isfinite(phip1) && std::isfinite(phim1) && std::isfinite(phip) && std::isfinite(phim)) {
+      if (phip1 < phim - 64.0 * pm || phim1 > phip + 64.0 * pm) { *d = 0.0; return phip; }
+      while (phip1 < phim) { phip1 += pm; phim1 += pm; }
+      while (phim1 > phip) { phip1 -= pm; phim1 -= pm; }
+   }
+   double phi1 = phip < phip1 ? phip : phip1;
+   double phi2 = phim > phim1 ? phim : phim1;
+   *d = phi1 - phi2;
+   return phip1;
+}
+
+// The ring walk over positions: start[c] = the source element (index within
+// its annulus) where ghost cell c's overlap sum begins.
+struct KBcScan {
    Mesh m;
-   int side, diode;
+   int side;
+   const double *piph, *dphi;
+   int *start;
+   DHD void operator()(long t) const
+   {
+      int lines = bc_lines(m, side);
+      int line = (int)(t % lines), g = (int)(t / lines);
+      int dst, src;
+      bc_pair(m, side, line, g, &dst, &src);
+      double pm = m.Pmax;
+      int c0 = m.off[dst], s0 = m.off[src];
+      int Np = m.Np[dst], Np1 = m.Np[src];
+      int i1;
+      double phi0 = piph[c0] - dphi[c0];
+      for (i1 = 0; i1 < Np1; i1++) {
+         double dphip = get_sdp(piph[s0 + i1], phi0, pm);
+         double dphim = dphip - dphi[s0 + i1];
+         if (dphip > 0 && dphim <= 0.0) break;
+      }
+      if (i1 >= Np1) i1 = 0;
+      for (int i = 0; i < Np; ++i) {
+         start[c0 + i] = i1;
+         double phip = piph[c0 + i];
+         double phim = piph[c0 + i] - dphi[c0 + i];
+         double phip1, d;
+         int guard = 0;
+         do {
+            phip1 = bc_overlap(phim, phip, piph, dphi, s0 + i1, pm, &d);
+            i1 = i1 < Np1 - 1 ? i1 + 1 : 0;
+         } while (phip1 < phip && ++guard <= Np1);
+         i1 = i1 > 0 ? i1 - 1 : Np1 - 1;
+      }
+   }
+};
+
+// One ghost cell of layer g: the overlap-weighted sum over the source cells
+// from its start, in the walk's order, then the diode.
+template <int NQ>
+struct KBcFill {
+   Mesh m;
+   int side, g, diode;
+   const int *start;
    double *prim;
    const double *piph, *dphi;
+   int maxnp;  // cells per line slot: the largest Np of the layer's annuli
    DHD void operator()(long t) const
    {
       size_t nt = (size_t)m.ntot;
-      int ng = side == 0 ? m.NgRa : side == 1 ? m.NgRb : side == 2 ? m.NgZa : m.NgZb;
-      int q_d = side < 2 ? URR : UZZ;
-      int line = (int)t;
-      for (int g = 0; g < ng; ++g) {
-         int dst, src;
-         double sgn;  // diode: inflow velocity zeroed
-         if (side == 0) { int j = m.NgRa - 1 - g; dst = j + m.Nr * line; src = dst + 1; sgn = 1.0; }
-         else if (side == 1) { int j = m.Nr - m.NgRb + g; dst = j + m.Nr * line; src = dst - 1; sgn = -1.0; }
-         else if (side == 2) { int k = m.NgZa - 1 - g; dst = line + m.Nr * k; src = dst + m.Nr; sgn = 1.0; }
-         else { int k = m.Nz - m.NgZb + g; dst = line + m.Nr * k; src = dst - m.Nr; sgn = -1.0; }
-         copy_distant<NQ>(m, prim, piph, dphi, dst, src);
-         if (diode)
-            for (int i = 0; i < m.Np[dst]; ++i) {
-               double *v = &prim[q_d * nt + m.off[dst] + i];
-               if (sgn * *v > 0) *v = 0.0;
-            }
+      int line = (int)(t / maxnp), i = (int)(t % maxnp);
+      int dst, src;
+      bc_pair(m, side, line, g, &dst, &src);
+      int Np = m.Np[dst];
+      if (i >= Np) return;
+      int c = m.off[dst] + i;
+      int s0 = m.off[src], Np1 = m.Np[src];
+      double pm = m.Pmax;
+      double phip = piph[c];
+      double phim = piph[c] - dphi[c];
+      double acc[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+      int i1 = start[c];
+      double phip1, d;
+      int guard = 0;
+      do {
+         phip1 = bc_overlap(phim, phip, piph, dphi, s0 + i1, pm, &d);
+#pragma unroll
+         for (int q = 0; q < NQ; ++q) acc[q] += prim[q * nt + s0 + i1] * d;
+         i1 = i1 < Np1 - 1 ? i1 + 1 : 0;
+      } while (phip1 < phip && ++guard <= Np1);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) prim[q * nt + c] = acc[q] / dphi[c];
+      if (diode) {
+         int qd = side < 2 ? URR : UZZ;
+         double sgn = (side == 0 || side == 2) ? 1.0 : -1.0;
+         double *v = &prim[qd * nt + c];
+         if (sgn * *v > 0) *v = 0.0;
       }
    }
 };

@@ -58,6 +58,9 @@ struct Buffers {
    double *emf_f[2] = {nullptr, nullptr}, *fE[2] = {nullptr, nullptr}, *fB[2] = {nullptr, nullptr};
    double *Epre = nullptr, *Eo = nullptr, *Eown = nullptr, *Ecarry = nullptr;
    signed char *flag[2] = {nullptr, nullptr};
+   int *estep[2] = {nullptr, nullptr};  // merge steps of each face on its two edge lines
+   int *bcstart = nullptr;          // where each ghost cell's overlap sum starts
+   int bc_maxnp[4][8] = {{0}};      // largest Np of each side's ghost layers
    PlanetSet pl[2];
 };
 
@@ -196,6 +199,7 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
          DH_TRY(alloc_n(x, &b.fE[d], sizeof(double) * n));
          DH_TRY(alloc_n(x, &b.fB[d], sizeof(double) * n));
          DH_TRY(alloc_n(x, &b.flag[d], sizeof(signed char) * n));
+         DH_TRY(alloc_n(x, &b.estep[d], sizeof(int) * 2 * n));
          DH_TRY(x.zero(b.emf_f[d], sizeof(double) * 3 * n));
       }
    }
@@ -215,6 +219,24 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
    DH_TRY(alloc_n(x, &b.wiph, cb));
    DH_TRY(alloc_n(x, &b.I0, sizeof(int) * nann));
    m.I0 = b.I0;
+   DH_TRY(alloc_n(x, &b.bcstart, sizeof(int) * (size_t)ntot));
+   DH_TRY(x.zero(b.bcstart, sizeof(int) * (size_t)ntot));
+   {
+      int ngs[4] = {g->NgRa, g->NgRb, g->NgZa, g->NgZb};
+      for (int side = 0; side < 4; ++side) {
+         if (ngs[side] > 8) return fail(DH_EINVAL, "more than 8 ghost annuli on side %d", side);
+         int lines = side < 2 ? g->Nz : g->Nr;
+         for (int gl = 0; gl < ngs[side]; ++gl) {
+            int mx = 0;
+            for (int line = 0; line < lines; ++line) {
+               int dst, src;
+               bc_pair(m, side, line, gl, &dst, &src);
+               if (g->Np[dst] > mx) mx = g->Np[dst];
+            }
+            b.bc_maxnp[side][gl] = mx;
+         }
+      }
+   }
    DH_TRY(x.zero(b.gradr, qb));
    DH_TRY(x.zero(b.gradz, qb));
    DH_TRY(x.zero(b.gradp, qb));
@@ -255,18 +277,27 @@ struct Engine {
          if (!dim_active(m, dim)) continue;
          double *grad = dim == 1 ? b.gradr : b.gradz;
          bool slopes = grads && b.P.plm != 0.0;
-         KCellRanges kr{m, dim, b.piph, b.rng[d]};
-         DH_TRY(x.run(dim == 1 ? "ranges_r" : "ranges_z", kr, m.ntot));
-         KFaces kf{m, dim, b.F[d], b.rng[d], b.piph, b.dphi};
-         DH_TRY(x.run(dim == 1 ? "faces_r" : "faces_z", kf, m.ntot));
+         if (dim == 1) {
+            DH_TRY(x.run("ranges_r", KCellRanges<1>{m, b.piph, b.rng[d]}, m.ntot));
+            DH_TRY(x.run("faces_r", KFaces<1>{m, b.F[d], b.rng[d], b.piph, b.dphi}, m.ntot));
+         } else {
+            DH_TRY(x.run("ranges_z", KCellRanges<2>{m, b.piph, b.rng[d]}, m.ntot));
+            DH_TRY(x.run("faces_z", KFaces<2>{m, b.F[d], b.rng[d], b.piph, b.dphi}, m.ntot));
+         }
          if (!grads) continue;
          if (!slopes) {
             DH_TRY(x.zero(grad, sizeof(double) * NQ * m.ntot));
             continue;
          }
-         KGrad<NQ> kg{m, dim, b.P.plm, b.P.num_c, b.F[d], b.rng[d], b.piph, b.dphi, b.prim,
-                      b.gradp, grad};
-         DH_TRY(x.run(dim == 1 ? "grad_r" : "grad_z", kg, m.ntot));
+         if (dim == 1) {
+            KGrad<NQ, 1> kg{m, b.P.plm, b.P.num_c, b.F[d], b.rng[d], b.piph, b.dphi, b.prim,
+                            b.gradp, grad};
+            DH_TRY(x.run("grad_r", kg, m.ntot));
+         } else {
+            KGrad<NQ, 2> kg{m, b.P.plm, b.P.num_c, b.F[d], b.rng[d], b.piph, b.dphi, b.prim,
+                            b.gradp, grad};
+            DH_TRY(x.run("grad_z", kg, m.ntot));
+         }
       }
       return DH_OK;
    }
@@ -281,8 +312,15 @@ struct Engine {
       for (int side = 0; side < 4; ++side) {
          if (side >= 2 && m.Nz <= 1) continue;
          if (bcs[side] != DH_BC_ZEROGRAD && bcs[side] != DH_BC_DIODE) continue;
-         KBcCopy<NQ> k{m, side, bcs[side] == DH_BC_DIODE ? 1 : 0, b.prim, b.piph, b.dphi};
-         DH_TRY(x.run("bc_copy", k, side < 2 ? m.Nz : m.Nr));
+         int lines = bc_lines(m, side), layers = bc_layers(m, side);
+         if (layers == 0) continue;
+         DH_TRY(x.run("bc_scan", KBcScan{m, side, b.piph, b.dphi, b.bcstart}, (long)lines * layers));
+         for (int g = 0; g < layers; ++g) {
+            int maxnp = b.bc_maxnp[side][g];
+            KBcFill<NQ> k{m, side, g, bcs[side] == DH_BC_DIODE ? 1 : 0, b.bcstart, b.prim,
+                          b.piph, b.dphi, maxnp};
+            DH_TRY(x.run("bc_fill", k, (long)lines * maxnp));
+         }
       }
       return DH_OK;
    }
@@ -306,9 +344,15 @@ struct Engine {
       for (int dim = 1; dim <= 2; ++dim) {
          int d = dim - 1;
          if (!dim_active(m, dim)) continue;
-         KFluxTrans<NQ, H> kt{m, P, dim, dt, b.F[d], b.prim, b.gradp, b.gradr, b.gradz,
-                              b.piph, b.dphi, b.dF[d], b.emf_f[d]};
-         DH_TRY(x.run(dim == 1 ? "flux_r" : "flux_z", kt, b.F[d].n));
+         if (dim == 1) {
+            KFluxTrans<NQ, H, 1> kt{m, P, dt, b.F[d], b.prim, b.gradp, b.gradr, b.gradz,
+                                    b.piph, b.dphi, b.dF[d], b.emf_f[d]};
+            DH_TRY(x.run("flux_r", kt, b.F[d].n));
+         } else {
+            KFluxTrans<NQ, H, 2> kt{m, P, dt, b.F[d], b.prim, b.gradp, b.gradr, b.gradz,
+                                    b.piph, b.dphi, b.dF[d], b.emf_f[d]};
+            DH_TRY(x.run("flux_z", kt, b.F[d].n));
+         }
       }
       if (mhd) {
          int ct = P.ct;
@@ -318,8 +362,10 @@ struct Engine {
          for (int dim = 1; dim <= 2; ++dim) {
             int d = dim - 1;
             if (!dim_active(m, dim)) continue;
-            KFaceEmf kfe{m, dim, b.F[d], b.piph, b.dphi, b.Epre, b.fE[d], b.fB[d]};
-            DH_TRY(x.run(dim == 1 ? "face_emf_r" : "face_emf_z", kfe, b.F[d].n));
+            if (dim == 1)
+               DH_TRY(x.run("face_emf_r", KFaceEmf<1>{m, b.F[d], b.piph, b.dphi, b.Epre, b.fE[d], b.fB[d]}, b.F[d].n));
+            else
+               DH_TRY(x.run("face_emf_z", KFaceEmf<2>{m, b.F[d], b.piph, b.dphi, b.Epre, b.fE[d], b.fB[d]}, b.F[d].n));
          }
          EdgeArgs ea{m, b.F[0], b.F[1], b.rng[0], b.rng[1], b.Epre, b.fE[0], b.fB[0], b.fE[1], b.fB[1]};
          KEdgesFinal kef{ea, ct, b.wiph, b.emf_f[0], b.emf_f[1], b.Eo, b.Eown, b.Ecarry};
@@ -327,7 +373,10 @@ struct Engine {
          KUpdateB kb{m, RK, dt, b.F[0], b.F[1], b.rng[0], b.rng[1], b.Eo, b.Phi, b.rkPhi};
          DH_TRY(x.run("update_B", kb, m.ntot));
          if (dim_active(m, 2)) {
-            KEdgeAdjust ke{m, dt, b.F[0], b.F[1], b.piph, b.dphi, b.Eown, b.Phi};
+            if (dim_active(m, 1))
+               DH_TRY(x.run("line_steps_r", KLineSteps<1>{m, b.F[0], b.estep[0]}, b.F[0].n));
+            DH_TRY(x.run("line_steps_z", KLineSteps<2>{m, b.F[1], b.estep[1]}, b.F[1].n));
+            KEdgeAdjust ke{m, dt, b.F[0], b.F[1], b.estep[0], b.estep[1], b.piph, b.dphi, b.Eown, b.Phi};
             DH_TRY(x.run("edge_adjust", ke, (long)b.F[0].n + b.F[1].n));
          }
       }
@@ -341,8 +390,10 @@ struct Engine {
             for (int dim = 1; dim <= 2; ++dim) {
                int d = dim - 1;
                if (!dim_active(m, dim)) continue;
-               KFlipFlags kff{m, dim, b.F[d], b.piph, b.wiph, dt, b.flag[d]};
-               DH_TRY(x.run(dim == 1 ? "flip_flags_r" : "flip_flags_z", kff, b.F[d].n));
+               if (dim == 1)
+                  DH_TRY(x.run("flip_flags_r", KFlipFlags<1>{m, b.F[d], b.piph, b.wiph, dt, b.flag[d]}, b.F[d].n));
+               else
+                  DH_TRY(x.run("flip_flags_z", KFlipFlags<2>{m, b.F[d], b.piph, b.wiph, dt, b.flag[d]}, b.F[d].n));
                DH_TRY(x.flips(m, dim, b.F[d], b.flag[d], b.Phi, b.rkPhi));
             }
          }

@@ -1,0 +1,171 @@
+"""Summaries of rocprofv3 runs of the timestep, for profiles/.
+
+    python -m disco_amd.profile_summary kernels --db TRACE.db --out FILE.md [--cmd "..."]
+    python -m disco_amd.profile_summary traffic --fetch F.db --write W.db \\
+        --copy-fetch CF.db --copy-write CW.db --workload NAME --out FILE.json [--cmd "..."]
+    python -m disco_amd.profile_summary copy-probe
+
+`kernels` tabulates a --kernel-trace run per kernel (calls, mean and total
+duration), with each kernel's name in the engine's timers.  `traffic` turns
+two --pmc runs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) into HBM bytes
+per launch of every kernel, scaled by what the same counters read for a
+plain copy of known size (`copy-probe`: one 1 GiB device-to-device copy),
+which bench.py reports next to the algorithmic bytes.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import re
+import sqlite3
+import sys
+from collections import defaultdict
+
+# kernel symbol (as rocprofv3 prints it) -> the engine's timer name
+_NAMES = [
+    (r"KPlmPhi<", "plm_phi"),
+    (r"k_ring_start", "ring_start"),
+    (r"KCellRanges<1>", "ranges_r"), (r"KCellRanges<2>", "ranges_z"),
+    (r"KFaces<1>", "faces_r"), (r"KFaces<2>", "faces_z"),
+    (r"KGrad<\d+, 1>", "grad_r"), (r"KGrad<\d+, 2>", "grad_z"),
+    (r"KFluxPhi<", "flux_phi"),
+    (r"KFluxTrans<.*, 1>", "flux_r"), (r"KFluxTrans<.*, 2>", "flux_z"),
+    (r"KEdgesCollect", "edges_collect"),
+    (r"KFaceEmf<1>", "face_emf_r"), (r"KFaceEmf<2>", "face_emf_z"),
+    (r"KEdgesFinal", "edges_final"),
+    (r"KUpdateB", "update_B"),
+    (r"KLineSteps<1>", "line_steps_r"), (r"KLineSteps<2>", "line_steps_z"),
+    (r"KEdgeAdjust", "edge_adjust"),
+    (r"KUpdate<", "update"),
+    (r"KFlipFlags<1>", "flip_flags_r"), (r"KFlipFlags<2>", "flip_flags_z"),
+    (r"k_flips", "flips"),
+    (r"KBfacesC2p<", "bfaces_c2p"),
+    (r"KBcScan", "bc_scan"), (r"KBcFill<", "bc_fill"),
+    (r"k_mindt<", "mindt"),
+    (r"KWcell<", "wcell"), (r"KWcellAvg", "wcell"),
+    (r"KInitCells<", "init"),
+]
+
+
+def timer_name(kernel: str) -> str:
+    head = kernel.split("(")[0] if kernel.startswith("k_") else kernel
+    for pat, name in _NAMES:
+        if re.search(pat, head):
+            return name
+    return ""
+
+
+def _rows(db: str, sql: str):
+    con = sqlite3.connect(db)
+    try:
+        return con.execute(sql).fetchall()
+    finally:
+        con.close()
+
+
+def kernel_stats(db: str):
+    """{kernel: (calls, total_ns, mean_ns)} of a --kernel-trace database."""
+    out = {}
+    for name, calls, total in _rows(
+            db, "select name, count(*), sum(duration) from kernels group by name"):
+        out[name] = (calls, total, total / calls)
+    return out
+
+
+def counter_means(db: str, counter: str):
+    """{kernel: (dispatches, mean value)} of one --pmc counter."""
+    acc = defaultdict(list)
+    for name, value in _rows(db, "select kernel_name, value from counters_collection "
+                                 f"where counter_name = '{counter}'"):
+        acc[name].append(value)
+    return {k: (len(v), sum(v) / len(v)) for k, v in acc.items()}
+
+
+def cmd_kernels(args):
+    stats = kernel_stats(args.db)
+    total = sum(v[1] for v in stats.values())
+    lines = [f"# Kernel trace: {args.title}", ""]
+    if args.cmd:
+        lines += ["Command:", "", "```", args.cmd, "```", ""]
+    lines += ["| kernel | timer name | calls | mean (ms) | total (ms) | share |",
+              "|---|---|---|---|---|---|"]
+    for name, (calls, tot, mean) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        short = name if len(name) < 90 else name[:87] + "..."
+        lines.append(f"| `{short}` | {timer_name(name)} | {calls} | {mean / 1e6:.4f} | "
+                     f"{tot / 1e6:.3f} | {100 * tot / total:.1f}% |")
+    with open(args.out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print(f"wrote {args.out}")
+
+
+def _copy_factor(fetch_db: str, write_db: str):
+    """(read factor, write factor): true bytes / counter bytes of the copy probe."""
+    nbytes = 1 << 30
+    f = counter_means(fetch_db, "FETCH_SIZE")
+    w = counter_means(write_db, "WRITE_SIZE")
+    # the probe's one large copy is the dispatch with the largest counts
+    fk = max(f, key=lambda k: f[k][1])
+    wk = max(w, key=lambda k: w[k][1])
+    return nbytes / (f[fk][1] * 1024), nbytes / (w[wk][1] * 1024), fk
+
+
+def cmd_traffic(args):
+    rf, wf, probe = _copy_factor(args.copy_fetch, args.copy_write)
+    fetch = counter_means(args.fetch, "FETCH_SIZE")
+    write = counter_means(args.write, "WRITE_SIZE")
+    kernels = {}
+    for name in sorted(set(fetch) | set(write)):
+        tn = timer_name(name)
+        if not tn:
+            continue
+        nf, fv = fetch.get(name, (0, 0.0))
+        nw, wv = write.get(name, (0, 0.0))
+        rd = fv * 1024 * rf
+        wr = wv * 1024 * wf
+        row = kernels.setdefault(tn, {"kernel": name, "dispatches": max(nf, nw),
+                                      "read_bytes_per_launch": 0.0,
+                                      "write_bytes_per_launch": 0.0})
+        row["read_bytes_per_launch"] += rd
+        row["write_bytes_per_launch"] += wr
+    for row in kernels.values():
+        row["hbm_bytes_per_launch"] = row["read_bytes_per_launch"] + row["write_bytes_per_launch"]
+    out = {"workload": args.workload,
+           "counters": "FETCH_SIZE and WRITE_SIZE (KiB per dispatch), one rocprofv3 --pmc run each",
+           "scale": {"read": rf, "write": wf,
+                     "probe": f"1 GiB device-to-device copy ({probe.split('(')[0]})"},
+           "command": args.cmd, "kernels": kernels}
+    with open(args.out, "w") as f:
+        json.dump(out, f, indent=1)
+    print(f"wrote {args.out}: read x{rf:.3f}, write x{wf:.3f}")
+
+
+def cmd_copy_probe(args):
+    import torch
+    n = 1 << 27  # 1 GiB of doubles
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print("copy probe: 1 GiB device-to-device")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="what", required=True)
+    k = sub.add_parser("kernels")
+    k.add_argument("--db", required=True)
+    k.add_argument("--out", required=True)
+    k.add_argument("--title", default="")
+    k.add_argument("--cmd", default="")
+    t = sub.add_parser("traffic")
+    for a in ("--fetch", "--write", "--copy-fetch", "--copy-write", "--workload", "--out"):
+        t.add_argument(a, required=True)
+    t.add_argument("--cmd", default="")
+    sub.add_parser("copy-probe")
+    args = ap.parse_args(argv)
+    {"kernels": cmd_kernels, "traffic": cmd_traffic, "copy-probe": cmd_copy_probe}[args.what](args)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -160,36 +160,8 @@ int dhx_riemann(dh_ctx *c, const double *pL, const double *pR, double r, const d
 }
 }
 
-template <int NQ, class H>
-static int do_fluxes(dh_ctx *c, double dt, double *dFp, double *dFr, double *dFz)
-{
-   Engine<HostExec> e{c->x, c->b};
-   Buffers &b = c->b;
-   DH_TRY(c->x.run("plm_phi", KPlmPhi<NQ>{b.m, b.P.plm, b.prim, b.dphi, b.gradp}, b.m.ntot));
-   DH_TRY(e.template topology<NQ>(true));
-   KFluxPhi<NQ, H> kp{b.m, b.P, dt, b.prim, b.gradp, b.gradr, b.gradz, b.piph, b.dphi, b.wiph,
-                      b.dFp, b.emf_phi};
-   DH_TRY(c->x.run("flux_phi", kp, b.m.ntot));
-   for (int dim = 1; dim <= 2; ++dim) {
-      int d = dim - 1;
-      if (!dim_active(b.m, dim)) continue;
-      KFluxTrans<NQ, H> kt{b.m, b.P, dim, dt, b.F[d], b.prim, b.gradp, b.gradr, b.gradz,
-                           b.piph, b.dphi, b.dF[d], b.emf_f[d]};
-      DH_TRY(c->x.run("flux", kt, b.F[d].n));
-   }
-   memcpy(dFp, b.dFp, sizeof(double) * NQ * b.m.ntot);
-   if (dFr) memcpy(dFr, b.dF[0], sizeof(double) * NQ * b.F[0].n);
-   if (dFz) memcpy(dFz, b.dF[1], sizeof(double) * NQ * b.F[1].n);
-   return DH_OK;
-}
 
 extern "C" long dhx_flip_count(dh_ctx *c) { return c ? c->x.flipped : -1; }
-
-extern "C" int dhx_fluxes(dh_ctx *c, double dt, double *dFp, double *dFr, double *dFz)
-{
-   if (!c) return fail(DH_EINVAL, "null context");
-   return DH_DISPATCH(c->b, do_fluxes, c, dt, dFp, dFr, dFz);
-}
 
 extern "C" {
 

@@ -55,14 +55,56 @@ def rel_l1(x: np.ndarray, y: np.ndarray) -> float:
 PARITY_TOL = 1e-10
 
 
+def cell_r(s: MeshState) -> np.ndarray:
+    """Radial centroid of every cell (cylindrical.c:24-30)."""
+    j = np.repeat(np.arange(len(s.np_)) % s.nr, s.np_)
+    rp, rm = s.r_jph[j + 1], s.r_jph[j]
+    return 2.0 * (rp * rp + rp * rm + rm * rm) / (3.0 * (rp + rm))
+
+
 def field_errors(s: MeshState, ref: MeshState):
-    """Relative L1 over real cells of every prim/cons field, and of piph."""
+    """Relative L1 error over real cells of every primitive and conserved
+    field, and of piph.  A component of a vector (velocity, momentum, B) is
+    measured against the L1 norm of the whole vector, so that a component
+    that vanishes in the solution (v_r of a steady vortex, B_r of a
+    vertical field) is judged on the scale of the field it belongs to."""
     m = ref.real_mask()
+    r = cell_r(ref)[m]
     out = {}
-    for q in range(ref.nq):
-        out[f"prim{q}"] = rel_l1(s.prim[q][m], ref.prim[q][m])
-        out[f"cons{q}"] = rel_l1(s.cons[q][m], ref.cons[q][m])
-    out["piph"] = rel_l1(s.piph, ref.piph)
+
+    def put(key, err, scale):
+        den = float(np.abs(scale).sum())
+        num = float(np.abs(err).sum())
+        out[key] = num / den if den > 0 else num
+
+    P, Ps = ref.prim[:, m], s.prim[:, m]
+    C, Cs = ref.cons[:, m], s.cons[:, m]
+    nc = ref.num_c
+    for q in (0, 1):
+        put(f"prim{q}", Ps[q] - P[q], P[q])
+        put(f"cons{q}", Cs[q] - C[q], C[q])
+    # velocity (v_r, r omega, v_z) and momentum (S_r, L / r, S_z)
+    vmag = np.abs(P[2]) + np.abs(r * P[3]) + np.abs(P[4])
+    put("prim2", Ps[2] - P[2], vmag)
+    put("prim3", r * (Ps[3] - P[3]), vmag)
+    put("prim4", Ps[4] - P[4], vmag)
+    smag = np.abs(C[2]) + np.abs(C[3] / r) + np.abs(C[4])
+    put("cons2", Cs[2] - C[2], smag)
+    put("cons3", (Cs[3] - C[3]) / r, smag)
+    put("cons4", Cs[4] - C[4], smag)
+    if nc >= 8:
+        bmag = np.abs(P[5]) + np.abs(P[6]) + np.abs(P[7])
+        for q in (5, 6, 7):
+            put(f"prim{q}", Ps[q] - P[q], bmag)
+        # conserved B: (B_r dV / r, B_phi dV / r, B_z dV)
+        w = np.array([r, r, np.ones_like(r)])
+        cmag = sum(np.abs(C[q] * w[q - 5]) for q in (5, 6, 7))
+        for q in (5, 6, 7):
+            put(f"cons{q}", (Cs[q] - C[q]) * w[q - 5], cmag)
+    for q in range(nc, ref.nq):
+        put(f"prim{q}", Ps[q] - P[q], P[q])
+        put(f"cons{q}", Cs[q] - C[q], C[q])
+    put("piph", s.piph - ref.piph, ref.piph)
     return out
 
 
