@@ -59,54 +59,70 @@ def b_alg(nq: int, nf: int) -> int:
     return 40 * (nq + nf + 1)
 
 
-def kernel_bytes(name: str, nq: int, mhd: bool, nt: int, nr: int, nfr: int, nfz: int) -> float:
+def kernel_bytes(name: str, nq: int, mhd: bool, nt: int, nr: int, nfr: int, nfz: int,
+                 zlayer: int = 0) -> float:
     """Algorithmic bytes of one launch of a kernel: every array element the
     kernel needs read once and every output element written once (8-byte
-    doubles, 4-byte indices, 1-byte face orientations)."""
+    doubles, 4-byte indices, 1-byte face orientations).  nt counts all cells
+    (ghosts included), nr the real ones, nfr/nfz the r and z faces; zlayer is
+    the cell count of one z ghost layer (0 when no z side is copied).
+
+    Per face the topology holds L, R (int), LR (byte) and up, dA, phic, dphi
+    (double); every cell has two merge steps per direction (int) and its
+    annulus index cjk (int)."""
     d, i = 8, 4
     nf = {"r": nfr, "z": nfz}.get(name[-1:], 0)
-    rng = 2 * 8 * i * nt  # head/tail face ranges of both directions
-    if name == "plm_phi":
-        return d * (2 * nq * nt + nt) + i * nt
-    if name in ("faces_r", "faces_z"):
-        # topology out (L, R, LR, dA, phic, dphi), cell geometry and the
-        # phi-reconstructed states in, one slope per field out
-        return nf * (2 * i + 1 + 3 * d) + d * 2 * nt + d * 2 * nq * nt + d * nq * nf
-    if name in ("ranges_r", "ranges_z"):
-        return d * nt + i * nt + 8 * i * nt
-    if name in ("grad_r", "grad_z"):
-        return 8 * i * nt + d * (nq * nf + nf) + d * 2 * nt + d * nq * nt
-    if name == "flux_phi":
-        return d * (2 * nq * nt + 2 * nt + nq * nt) + (d * 4 * nt if mhd else 0)
-    if name in ("flux_r", "flux_z"):
-        return (2 * i + 2 * d) * nf + d * (3 * nq * nt + 2 * nt) + d * nq * nf + (3 * d * nf if mhd else 0)
-    if name == "edges_collect":
-        return d * (4 * nt + nt) + rng + d * 3 * (nfr + nfz) + d * 16 * nt
-    if name == "edges_avg":
-        return d * 32 * nt
-    if name in ("face_emf_r", "face_emf_z"):
-        return nf * (2 * i + 1) + d * 2 * nt + d * 16 * nt + d * 2 * nf
-    if name == "edges_final":
-        return d * 16 * nt + rng + d * 4 * (nt) + d * 2 * nt * 2 + d * nt + d * 8 * nt
-    if name == "update_B":
-        return d * 15 * nt + rng + d * 4 * nt + (2 * i + 1) * 2 * nt
-    if name == "edge_adjust":
-        return (nfr + nfz) * (2 * i + 1 + 2 * d) + d * 2 * nt + d * 4 * nt
-    if name == "update":
-        # cons and rkcons in, cons out (+ prim out for hydro), face deltas,
-        # ranges, geometry in and out
-        return (d * nq * nr * (3 + (0 if mhd else 1) + 1) + d * nq * (nt + nfr + nfz) + rng
-                + d * 5 * nt)
-    if name == "bfaces_c2p":
-        return rng + (nfr + nfz) * (2 * i + 1 + d) + d * 5 * nt + d * 2 * nt + d * nq * nr * 2
-    if name in ("flip_flags_r", "flip_flags_z"):
-        return nf * (2 * i + 1 + 1) + d * 2 * nt
-    if name == "mindt":
-        return d * (nq * nr + 3 * nt)
-    if name == "wcell":
+    nfa = nfr + nfz
+    steps = 2 * i * nt                  # the two merge steps of one direction
+    if name == "wcell":                 # analytic mesh motion: r_jph in, wiph out
         return d * nt + i * nt
     if name == "ring_start":
         return d * nt
+    if name == "plm_phi":
+        return d * (2 * nq * nt + nt) + i * nt
+    if name in ("ranges_r", "ranges_z"):
+        return d * nt + i * nt + steps
+    if name in ("faces_r", "faces_z"):
+        return steps + i * nt + d * 2 * nt + nf * (2 * i + 1 + 4 * d)
+    if name in ("grad_r", "grad_z"):
+        return steps + i * nt + nf * (2 * i + 2 * d) + d * nt * (2 + 3 * nq)
+    if name == "flux_phi":
+        return d * (3 * nq + 2) * nt + i * nt + (d * 4 * nt if mhd else 0)
+    if name in ("flux_r", "flux_z"):
+        return (nf * (2 * i + 2 * d) + d * nt * (2 + 3 * nq) + d * nq * nf
+                + (3 * d * nf if mhd else 0))
+    if name == "edges_collect":
+        return d * 21 * nt + 2 * steps + i * nt + 3 * d * nfa
+    if name in ("face_emf_r", "face_emf_z"):
+        return nf * (2 * i + 1 + 2 * d) + d * 18 * nt + i * nt
+    if name == "edges_final":
+        return d * 25 * nt + 2 * steps + i * nt + 3 * d * nfa
+    if name == "update_B":
+        # Phi in and out, rkPhi on the second substep only (mean 2.5 per cell)
+        return d * (12.5 + 4) * nt + 2 * steps + i * nt + nfa * (2 * i + 1)
+    if name in ("line_steps_r", "line_steps_z"):
+        return nf * (d + 2 * i)
+    if name == "edge_adjust":
+        return nfa * (2 * i + 1 + 2 * i + 2 * d) + d * 5 * nt
+    if name == "update":
+        # cons in/out, rkcons on the second substep only, prim in for the
+        # sources (+ prim out for hydro), phi/r/z face deltas, geometry in
+        # and moved geometry out
+        return (d * nq * nr * (2.5 + 1 + (0 if mhd else 1)) + d * nq * (nt + nfa)
+                + 2 * steps + i * nt + d * 5 * nt)
+    if name == "bfaces_c2p":
+        return (2 * steps + i * nt + d * 10 * nt + nfa * (2 * i + 1 + d)
+                + d * nq * nr * 2)
+    if name in ("flip_flags_r", "flip_flags_z"):
+        return nf * (2 * i + 2) + 2 * d * nt
+    if name == "flips":
+        return nfa
+    if name == "mindt":
+        return d * (nq * nr + 3 * nt) + i * nt
+    if name == "bc_scan":               # one z side: positions of both layers' rings
+        return 2 * zlayer * (4 * d + i)
+    if name == "bc_fill":               # one z ghost layer from the layer inside it
+        return zlayer * (d * (2 * nq + 4) + i)
     return 0.0
 
 
@@ -201,6 +217,15 @@ def main(argv=None):
         nfaces.append(n.value)
     nfr, nfz = nfaces
 
+    # cells of one z ghost layer (Np does not depend on z); the bc_* models
+    # cover meshes whose copied sides are the z sides only
+    bc = cfg.dh_bc(par)
+    ng = 2  # ghost annuli per side (NUM_G)
+    nz_tot = int(par.get("Num_Z", 1)) + sum(ng for s in (2, 3) if bc[s] != configs.BC_NONE)
+    z_copied = any(bc[s] in (configs.BC_ZEROGRAD, configs.BC_DIODE) for s in (2, 3))
+    r_copied = any(bc[s] in (configs.BC_ZEROGRAD, configs.BC_DIODE) for s in (0, 1))
+    zlayer = ntot // nz_tot if z_copied and not r_copied else 0
+
     roofline = None
     kernels = None
     if not args.no_kernel_timers:
@@ -219,7 +244,7 @@ def main(argv=None):
         kernels = {}
         for k in range(n.value):
             nm = bytes(names[k]).split(b"\0")[0].decode()
-            alg = kernel_bytes(nm, nq, mhd, ntot, ncell, nfr, nfz)
+            alg = kernel_bytes(nm, nq, mhd, ntot, ncell, nfr, nfz, zlayer)
             mean_s = ms[k] / max(calls[k], 1) * 1e-3
             kernels[nm] = {"ms_per_step": ms[k] / tsteps, "launches_per_step": calls[k] / tsteps,
                            "mean_launch_ms": mean_s * 1e3,

@@ -2,9 +2,9 @@
 //
 // The per-thread work of every kernel lives in dh_kernels.h and the launch
 // sequence of a timestep in dh_pipeline.h; this file supplies what only the
-// GPU has: the generic launch, three wavefront kernels (ring starts, face
-// flips, the CFL reduction), HIP memory and streams, and per-kernel event
-// timers.
+// GPU has: the generic launch, four cooperative kernels (ring starts, face
+// flips, the ghost-ring walk, the CFL reduction), HIP memory and streams,
+// and per-kernel event timers.
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -86,6 +86,67 @@ __global__ __launch_bounds__(TPB) void k_flips(Mesh m, int dim, Faces F, const s
          }
       }
    }
+}
+
+// The ghost-ring walk (KBcScan), one workgroup per ring.  The walk's state
+// on entering ghost cell i is the source cell holding the cell's lower edge;
+// every thread guesses it for its cells by bisection (bc_guess), the first
+// cell's by the walk's own start test (the lowest element that passes it),
+// then checks each guess against one step of the walk from the guess before
+// it.  A ring with any miss is walked again sequentially, so start[] is the
+// sequential walk's in every case.
+__global__ __launch_bounds__(TPB) void k_bc_scan(Mesh m, int side, const double *piph,
+                                                 const double *dphi, int *start)
+{
+   __shared__ int s_first, s_miss;
+   __shared__ double s_minv[TPB];
+   __shared__ int s_mini[TPB];
+   int lines = bc_lines(m, side);
+   int t = blockIdx.x;
+   int line = t % lines, g = t / lines;
+   int dst, src;
+   bc_pair(m, side, line, g, &dst, &src);
+   double pm = m.Pmax;
+   int c0 = m.off[dst], s0 = m.off[src];
+   int Np = m.Np[dst], Np1 = m.Np[src];
+   int tid = threadIdx.x;
+   if (tid == 0) {
+      s_first = Np1;
+      s_miss = 0;
+   }
+   __syncthreads();
+   double phi0 = piph[c0] - dphi[c0];
+   double mv = HUGE_VAL;
+   int mi = 0;
+   for (int k = tid; k < Np1; k += TPB) {
+      if (bc_holds_start(piph, dphi, s0, k, phi0, pm)) atomicMin(&s_first, k);
+      double v = piph[s0 + k];
+      if (v < mv) { mv = v; mi = k; }
+   }
+   s_minv[tid] = mv;
+   s_mini[tid] = mi;
+   __syncthreads();
+   for (int o = TPB / 2; o > 0; o >>= 1) {
+      if (tid < o) {
+         double v = s_minv[tid + o];
+         int i = s_mini[tid + o];
+         if (v < s_minv[tid] || (v == s_minv[tid] && i < s_mini[tid])) {
+            s_minv[tid] = v;
+            s_mini[tid] = i;
+         }
+      }
+      __syncthreads();
+   }
+   int first = s_first < Np1 ? s_first : 0;
+   int rot = s_mini[0];
+   for (int i = tid; i < Np; i += TPB)
+      start[c0 + i] = i == 0 ? first : bc_guess(piph, s0, Np1, rot, piph[c0 + i - 1], pm);
+   __syncthreads();
+   for (int i = tid; i < Np - 1; i += TPB)
+      if (bc_walk_cell(piph, dphi, c0 + i, s0, Np1, start[c0 + i], pm) != start[c0 + i + 1])
+         s_miss = 1;
+   __syncthreads();
+   if (s_miss && tid == 0) KBcScan{m, side, piph, dphi, start}(t);
 }
 
 // CFL reduction over real cells (misc.c:39-84).  Positive doubles order like
@@ -188,6 +249,18 @@ struct HipExec {
       {
          Bracket b(*this, "ring_start");
          k_ring_start<<<(unsigned)((m.nann + 3) / 4), TPB, 0, stream>>>(m, piph, I0);
+      }
+      LAUNCH_CHECK();
+      return DH_OK;
+   }
+
+   int bc_scan(const Mesh &m, int side, const double *piph, const double *dphi, int *start)
+   {
+      long rings = (long)bc_lines(m, side) * bc_layers(m, side);
+      if (rings <= 0) return DH_OK;
+      {
+         Bracket b(*this, "bc_scan");
+         k_bc_scan<<<(unsigned)rings, TPB, 0, stream>>>(m, side, piph, dphi, start);
       }
       LAUNCH_CHECK();
       return DH_OK;

@@ -372,20 +372,41 @@ struct KGrad {
    const int *rng;
    const double *piph, *dphi, *prim, *gradp;
    double *grad;
-   // One-sided slope of face f (plm.c:84-110): S = (W_R - W_L)/(dx_L + dx_R)
-   // of the phi-reconstructed states at the face's phi centre.
-   DHD void face_slope(int f, double dxL, double dxR, double *S) const
+   // The limiter of plm.c:160-195 visits the face slopes s in order with
+   //   if (s g < 0) g = 0; else if (|PLM s| < |g|) g = PLM s;
+   // its outcome is 0 if any s has the sign opposite to the mean g0, else
+   // the first PLM s of least magnitude when that is below |g0|, else g0
+   // (NaN slopes never act).  So one pass over the faces keeps, per field,
+   // the area-weighted sum, the first least PLM s, and which signs occurred.
+   struct Acc {
+      double sum[NQ], best[NQ];
+      unsigned pos = 0, neg = 0;
+   };
+   // Face f of a pair whose distances from the centroids are dxL, dxR; the
+   // cell is its L (own_left) or R, with state (pc, gc).  The one-sided slope
+   // is S = (W_R - W_L)/(dx_L + dx_R) of the phi-reconstructed states at the
+   // face's phi centre (plm.c:84-110).
+   DHD void add_face(int f, bool own_left, long c, const double *pc, const double *gc, double dxL,
+                     double dxR, Acc &A) const
    {
       size_t nt = (size_t)m.ntot;
-      int cL = F.L[f], cR = F.R[f];
+      long o = own_left ? F.R[f] : F.L[f];
+      long cL = own_left ? c : o, cR = own_left ? o : c;
       double phic = F.phic[f];
+      double dA = F.dA[f];
       double dpL = get_sdp(phic, piph[cL] - .5 * dphi[cL], m.Pmax);
       double dpR = get_sdp(piph[cR] - .5 * dphi[cR], phic, m.Pmax);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-         double WL = prim[q * nt + cL] + dpL * gradp[q * nt + cL];
-         double WR = prim[q * nt + cR] - dpR * gradp[q * nt + cR];
-         S[q] = (WR - WL) / (dxR + dxL);
+         double po = prim[q * nt + o], go = gradp[q * nt + o];
+         double WL = own_left ? pc[q] + dpL * gc[q] : po + dpL * go;
+         double WR = own_left ? po - dpR * go : pc[q] - dpR * gc[q];
+         double S = (WR - WL) / (dxR + dxL);
+         A.sum[q] += S * dA;
+         double v = PLM * S;
+         if (fabs(v) < fabs(A.best[q])) A.best[q] = v;
+         if (S > 0.0) A.pos |= 1u << q;
+         if (S < 0.0) A.neg |= 1u << q;
       }
    }
    DHD void operator()(long c) const
@@ -397,24 +418,28 @@ struct KGrad {
       double dxLi = 0, dxRi = 0, dxLo = 0, dxRo = 0;
       if (cr.in) pair_dx(m, dim, dim == 1 ? j - 1 : j, dim == 1 ? k : k - 1, &dxLi, &dxRi);
       if (cr.out) pair_dx(m, dim, j, k, &dxLo, &dxRo);
-      double acc[NQ], Sf[NQ];
+      double pc[NQ], gc[NQ];
+      load_q<NQ>(prim, nt, c, pc);
+      load_q<NQ>(gradp, nt, c, gc);
+      Acc A;
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-      auto add_range = [&](int base, int lo, int hi, double dxL, double dxR) {
-         for (int f = base + lo; f <= base + hi; ++f) {
-            double dA = F.dA[f];
-            face_slope(f, dxL, dxR, Sf);
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) acc[q] += Sf[q] * dA;
-         }
-      };
+      for (int q = 0; q < NQ; ++q) {
+         A.sum[q] = 0.0;
+         A.best[q] = HUGE_VAL;
+      }
+      // the reference's order: the inner pair (the cell is R), then the outer
+      // pair (the cell is L), each head before tail
       if (cr.in) {
-         add_range(cr.base_in, cr.in_h0, cr.in_h1, dxLi, dxRi);
-         add_range(cr.base_in, cr.in_t0, cr.in_t1, dxLi, dxRi);
+         for (int f = cr.base_in + cr.in_h0; f <= cr.base_in + cr.in_h1; ++f)
+            add_face(f, false, c, pc, gc, dxLi, dxRi, A);
+         for (int f = cr.base_in + cr.in_t0; f <= cr.base_in + cr.in_t1; ++f)
+            add_face(f, false, c, pc, gc, dxLi, dxRi, A);
       }
       if (cr.out) {
-         add_range(cr.base_out, cr.out_h0, cr.out_h1, dxLo, dxRo);
-         add_range(cr.base_out, cr.out_t0, cr.out_t1, dxLo, dxRo);
+         for (int f = cr.base_out + cr.out_h0; f <= cr.base_out + cr.out_h1; ++f)
+            add_face(f, true, c, pc, gc, dxLo, dxRo, A);
+         for (int f = cr.base_out + cr.out_t0; f <= cr.base_out + cr.out_t1; ++f)
+            add_face(f, true, c, pc, gc, dxLo, dxRo, A);
       }
       double phip = piph[c];
       double phim = phip - dphi[c];
@@ -435,26 +460,10 @@ struct KGrad {
       if (!cr.out) dAtot = dAm;
       double g[NQ];
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) g[q] = acc[q] / dAtot;
-      // Limit against every face slope; the outcome does not depend on order.
-      auto limit_range = [&](int base, int lo, int hi, double dxL, double dxR) {
-         for (int f = base + lo; f <= base + hi; ++f) {
-            face_slope(f, dxL, dxR, Sf);
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-               double s = Sf[q];
-               if (s * g[q] < 0.0) g[q] = 0.0;
-               else if (fabs(PLM * s) < fabs(g[q])) g[q] = PLM * s;
-            }
-         }
-      };
-      if (cr.in) {
-         limit_range(cr.base_in, cr.in_h0, cr.in_h1, dxLi, dxRi);
-         limit_range(cr.base_in, cr.in_t0, cr.in_t1, dxLi, dxRi);
-      }
-      if (cr.out) {
-         limit_range(cr.base_out, cr.out_h0, cr.out_h1, dxLo, dxRo);
-         limit_range(cr.base_out, cr.out_t0, cr.out_t1, dxLo, dxRo);
+      for (int q = 0; q < NQ; ++q) {
+         double g0 = A.sum[q] / dAtot;
+         bool opp = (g0 > 0.0 && (A.neg >> q & 1u)) || (g0 < 0.0 && (A.pos >> q & 1u));
+         g[q] = opp ? 0.0 : fabs(A.best[q]) < fabs(g0) ? A.best[q] : g0;
       }
       if (dim == 1 && j == 0 && m.NgRa == 0) {
          double xp0 = m.r_jph[0], xm0 = m.r_jph[-1];
@@ -463,7 +472,7 @@ struct KGrad {
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                if (q == URR || (num_c > BZZ && q == BRR)) {
-                  double SL = prim[q * nt + c] / r;
+                  double SL = pc[q] / r;
                   double Sg = g[q];
                   if (Sg * SL < 0.0) g[q] = 0.0;
                   else if (fabs(PLM * SL) < fabs(Sg)) g[q] = PLM * SL;
@@ -1441,8 +1450,36 @@ DHD double bc_overlap(double phim, double phip, const double *piph, const double
    return phip1;
 }
 
+// Whether source element i1 holds the lower edge phi0 of a ghost ring's
+// first cell, the test that starts the ring walk.
+DHD bool bc_holds_start(const double *piph, const double *dphi, int s0, int i1, double phi0,
+                        double pm)
+{
+   double dphip = get_sdp(piph[s0 + i1], phi0, pm);
+   double dphim = dphip - dphi[s0 + i1];
+   return dphip > 0 && dphim <= 0.0;
+}
+
+// The walk over one ghost cell c from source element i1: the element of the
+// last source cell it overlaps, which is where the next ghost cell's sum
+// begins.
+DHD int bc_walk_cell(const double *piph, const double *dphi, long c, int s0, int Np1, int i1,
+                     double pm)
+{
+   double phip = piph[c];
+   double phim = piph[c] - dphi[c];
+   double phip1, d;
+   int guard = 0;
+   do {
+      phip1 = bc_overlap(phim, phip, piph, dphi, s0 + i1, pm, &d);
+      i1 = i1 < Np1 - 1 ? i1 + 1 : 0;
+   } while (phip1 < phip && ++guard <= Np1);
+   return i1 > 0 ? i1 - 1 : Np1 - 1;
+}
+
 // The ring walk over positions: start[c] = the source element (index within
-// its annulus) where ghost cell c's overlap sum begins.
+// its annulus) where ghost cell c's overlap sum begins.  Ring t is line
+// t % lines of layer t / lines.
 struct KBcScan {
    Mesh m;
    int side;
@@ -1459,26 +1496,35 @@ struct KBcScan {
       int Np = m.Np[dst], Np1 = m.Np[src];
       int i1;
       double phi0 = piph[c0] - dphi[c0];
-      for (i1 = 0; i1 < Np1; i1++) {
-         double dphip = get_sdp(piph[s0 + i1], phi0, pm);
-         double dphim = dphip - dphi[s0 + i1];
-         if (dphip > 0 && dphim <= 0.0) break;
-      }
+      for (i1 = 0; i1 < Np1; i1++)
+         if (bc_holds_start(piph, dphi, s0, i1, phi0, pm)) break;
       if (i1 >= Np1) i1 = 0;
       for (int i = 0; i < Np; ++i) {
          start[c0 + i] = i1;
-         double phip = piph[c0 + i];
-         double phim = piph[c0 + i] - dphi[c0 + i];
-         double phip1, d;
-         int guard = 0;
-         do {
-            phip1 = bc_overlap(phim, phip, piph, dphi, s0 + i1, pm, &d);
-            i1 = i1 < Np1 - 1 ? i1 + 1 : 0;
-         } while (phip1 < phip && ++guard <= Np1);
-         i1 = i1 > 0 ? i1 - 1 : Np1 - 1;
+         i1 = bc_walk_cell(piph, dphi, c0 + i, s0, Np1, i1, pm);
       }
    }
 };
+
+// A guess of the walk's state on entering a ghost cell whose lower edge is
+// x: the source cell that holds x, found by bisection over the source ring
+// read from its lowest upper edge (element rot) on.  Used by the parallel
+// form of the walk, which checks every guess against one step of the walk.
+DHD int bc_guess(const double *piph, int s0, int Np1, int rot, double x, double pm)
+{
+   if (!(x >= 0.0 && x < pm)) x -= floor(x / pm) * pm;
+   int lo = 0, hi = Np1;  // first t in [0, Np1) with U(t) >= x, else Np1
+   while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      int k = rot + mid;
+      if (k >= Np1) k -= Np1;
+      if (piph[s0 + k] >= x) hi = mid;
+      else lo = mid + 1;
+   }
+   int t = lo == Np1 ? 0 : lo;
+   int k = rot + t;
+   return k >= Np1 ? k - Np1 : k;
+}
 
 // One ghost cell of layer g: the overlap-weighted sum over the source cells
 // from its start, in the walk's order, then the diode.

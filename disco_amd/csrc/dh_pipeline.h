@@ -314,7 +314,7 @@ struct Engine {
          if (bcs[side] != DH_BC_ZEROGRAD && bcs[side] != DH_BC_DIODE) continue;
          int lines = bc_lines(m, side), layers = bc_layers(m, side);
          if (layers == 0) continue;
-         DH_TRY(x.run("bc_scan", KBcScan{m, side, b.piph, b.dphi, b.bcstart}, (long)lines * layers));
+         DH_TRY(x.bc_scan(m, side, b.piph, b.dphi, b.bcstart));
          for (int g = 0; g < layers; ++g) {
             int maxnp = b.bc_maxnp[side][g];
             KBcFill<NQ> k{m, side, g, bcs[side] == DH_BC_DIODE ? 1 : 0, b.bcstart, b.prim,

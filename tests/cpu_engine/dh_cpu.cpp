@@ -21,6 +21,8 @@ using namespace dh;
 struct HostExec {
    std::vector<void *> allocs;
    long flipped = 0;  // faces handed over so far (a check that flips ran)
+   long bc_rings = 0;   // ghost rings walked so far
+   long bc_misses = 0;  // of them, rings where the GPU kernel's guesses miss
 
    template <class K>
    int run(const char *, const K &k, long n)
@@ -31,6 +33,30 @@ struct HostExec {
    int ring_start(const Mesh &m, const double *piph, int *I0)
    {
       for (int a = 0; a < m.nann; ++a) I0[a] = ring_start_of(m, piph, a);
+      return DH_OK;
+   }
+   // The sequential ring walk, and a count of the rings where the guesses
+   // of the GPU's parallel walk (k_bc_scan) would not all hold, so that
+   // the GPU would walk the ring again.
+   int bc_scan(const Mesh &m, int side, const double *piph, const double *dphi, int *start)
+   {
+      long rings = (long)bc_lines(m, side) * bc_layers(m, side);
+      KBcScan k{m, side, piph, dphi, start};
+      for (long t = 0; t < rings; ++t) {
+         k(t);
+         int lines = bc_lines(m, side);
+         int dst, src;
+         bc_pair(m, side, (int)(t % lines), (int)(t / lines), &dst, &src);
+         int c0 = m.off[dst], s0 = m.off[src], Np = m.Np[dst], Np1 = m.Np[src];
+         int rot = 0;
+         for (int i = 1; i < Np1; ++i)
+            if (piph[s0 + i] < piph[s0 + rot]) rot = i;
+         bool miss = false;
+         for (int i = 1; i < Np; ++i)
+            miss |= bc_guess(piph, s0, Np1, rot, piph[c0 + i - 1], m.Pmax) != start[c0 + i];
+         ++bc_rings;
+         bc_misses += miss;
+      }
       return DH_OK;
    }
    int flips(const Mesh &m, int dim, const Faces &F, const signed char *flag, double *Phi,
@@ -162,6 +188,8 @@ int dhx_riemann(dh_ctx *c, const double *pL, const double *pR, double r, const d
 
 
 extern "C" long dhx_flip_count(dh_ctx *c) { return c ? c->x.flipped : -1; }
+extern "C" long dhx_bc_rings(dh_ctx *c) { return c ? c->x.bc_rings : -1; }
+extern "C" long dhx_bc_misses(dh_ctx *c) { return c ? c->x.bc_misses : -1; }
 
 extern "C" {
 

@@ -32,6 +32,9 @@ def cpu_lib():
     lib = capi.load_library(cpu_build.build())
     lib.dhx_flip_count.argtypes = [C.c_void_p]
     lib.dhx_flip_count.restype = C.c_long
+    for f in ("dhx_bc_rings", "dhx_bc_misses"):
+        getattr(lib, f).argtypes = [C.c_void_p]
+        getattr(lib, f).restype = C.c_long
     return lib
 
 
@@ -99,6 +102,11 @@ def test_cpu_build_bit_exact_vs_oracle(cpu_lib, cname, over, steps):
         o.timestep(dt)
     out = g.download(a)
     _assert_same(out, o.get_state(), f"{cname} {over}")
+    if cfg.dh_bc(par)[2] in (configs.BC_ZEROGRAD, configs.BC_DIODE):
+        # the GPU's parallel ring walk guesses every ring right (no ring
+        # needs its sequential fallback)
+        assert cpu_lib.dhx_bc_rings(g.ctx) > 0
+        assert cpu_lib.dhx_bc_misses(g.ctx) == 0
     if a.nf and par.get("Max_DT"):
         assert cpu_lib.dhx_flip_count(g.ctx) > 0
         faces = {d: g.faces(d)[:3] for d in (1, 2) if (a.nr if d == 1 else a.nz) > 1}
