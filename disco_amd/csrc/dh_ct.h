@@ -12,24 +12,26 @@
 // and E_phi on the phi-directed edges where four annuli meet.
 //
 // The reference updates these with sequential scatter loops over faces and
-// edges.  Here every output slot is computed by its owner:
-//   k_edges_collect  edge EMFs after the Riemann solves   (riemann.c:66-211)
-//   k_edges_avg      phi-neighbour averaging              (bfields.c:400-441)
-//   k_face_emf       face-interpolated edge EMFs          (bfields.c:443-531)
-//   k_edges_final    owner edges + advective subtraction  (bfields.c:533-690)
-//   k_update_B       Phi += E dl dt of the face edges     (bfields.c:313-382)
-//   k_edge_adjust    E_phi edges, a merge of two face lists instead of the
-//                    reference's 4-way ring walk           (bfields.c:888-1127)
-//   k_flips          face re-association after the mesh moved (bfields.c:693-814)
-//   k_bfaces_c2p     Phi -> cell B, then cons2prim         (bfields.c:88-309)
+// edges.  Here every output slot is computed by its owner (dh_kernels.h):
+//   KEdgesCollect  edge EMFs after the Riemann solves     (riemann.c:66-211)
+//   KFaceEmf       face-interpolated edge EMFs, with the phi-neighbour
+//                  averages formed where they are read    (bfields.c:400-531)
+//   KEdgesFinal    owner edges + advective subtraction    (bfields.c:533-690)
+//   KUpdateB       Phi += E dl dt of the face edges       (bfields.c:313-382)
+//   KLineSteps, KEdgeAdjust  E_phi edges: each face's merge steps on its two
+//                  edge lines, then its segments; a merge of two face lists
+//                  instead of the reference's 4-way ring walk (bfields.c:888-1127)
+//   KFlipFlags, k_flips  face re-association after the mesh moved (bfields.c:693-814)
+//   KBfacesC2p     Phi -> cell B, then cons2prim           (bfields.c:88-309)
 #pragma once
 #include "dh_mesh.h"
 
 namespace dh {
 
 // The faces of a cell in its inner and outer pair of one direction, as
-// inclusive head/tail ranges of the pair's face list (see ranges_as_a and
-// ranges_as_b in dh_mesh.h).  Empty ranges have lo > hi.
+// inclusive head/tail ranges of the pair's face list: a head range and, for
+// the ring's start cell only, a tail range at the end of the list (the walk
+// ends on the start cell again).  Empty ranges have lo > hi.
 struct CellRanges {
    int in_h0, in_h1, in_t0, in_t1, out_h0, out_h1, out_t0, out_t1;
    int base_in, base_out;   // first face of the inner / outer pair
@@ -139,46 +141,58 @@ DHD int get_which4(double phi, double phiR, double phiU, double phiUR, int *LR_a
    return w4;
 }
 
-// E_phi of the face a cell owns on one side: Eown[s][c], s = 0 r-in, 1 r-out,
-// 2 z-in, 3 z-out (the reference's E_phi[0..3] cell slots).
-struct EdgeCells {
-   int c, cR, cU, cUR;
+// The four cells around a phi-directed edge line at (r_jph[j], z_kph[k]):
+// position 0 = c (j,k), 1 = cR (j+1,k), 2 = cU (j,k+1), 3 = cUR (j+1,k+1).
+// What the edge needs of each: its upper and lower phi edges and the E_phi
+// of the two faces it owns that touch the line, taken from Eown[s][c]
+// (s = 0 r-in, 1 r-out, 2 z-in, 3 z-out, the reference's E_phi[0..3]):
+// ea is the r face (r-out for positions 0 and 2, r-in for 1 and 3), eb the
+// z face (z-out for 0 and 1, z-in for 2 and 3).
+struct EdgeCell {
+   double up, lo, ea, eb;
 };
 
-// One segment of the phi-directed edge line at (r_jph[j], z_kph[k]) where the
-// four cells c (j,k), cR (j+1,k), cU (j,k+1), cUR (j+1,k+1) meet: its EMF and
-// length, exactly as make_edge_adjust forms them (bfields.c:948-1116).
-DHD void edge_segment(const double *piph, const double *dphi, const double *Eown, size_t nt,
-                      const EdgeCells &q, double r, double Pmax, double *E, double *dl)
+DHD EdgeCell edge_cell(const double *piph, const double *dphi, const double *Eown, size_t nt,
+                       long cell, int pos)
+{
+   EdgeCell q;
+   q.up = piph[cell];
+   q.lo = piph[cell] - dphi[cell];
+   q.ea = Eown[(size_t)((pos & 1) ? 0 : 1) * nt + cell];
+   q.eb = Eown[(size_t)((pos & 2) ? 2 : 3) * nt + cell];
+   return q;
+}
+
+
+// One segment of the phi-directed edge line of radius r where the four
+// cells meet: its EMF and length, exactly as make_edge_adjust forms them
+// (bfields.c:948-1116).
+DHD void edge_segment(EdgeCell c, EdgeCell cR, EdgeCell cU,
+                      EdgeCell cUR, double r, double Pmax, double *E, double *dl)
 {
    int LR_alt, UD_alt;
-   int w4 = get_which4(piph[q.c], piph[q.cR], piph[q.cU], piph[q.cUR], &LR_alt, &UD_alt, 0, Pmax);
-   int w4b = get_which4(piph[q.c] - dphi[q.c], piph[q.cR] - dphi[q.cR], piph[q.cU] - dphi[q.cU],
-                        piph[q.cUR] - dphi[q.cUR], nullptr, nullptr, 1, Pmax);
-   // Eown slots: 0 r-in (E_phi[0]), 1 r-out (E_phi[1]), 2 z-in (E_phi[2]), 3 z-out (E_phi[3])
-#define EO(s, cell) Eown[(s) * nt + (cell)]
+   int w4 = get_which4(c.up, cR.up, cU.up, cUR.up, &LR_alt, &UD_alt, 0, Pmax);
+   int w4b = get_which4(c.lo, cR.lo, cU.lo, cUR.lo, nullptr, nullptr, 1, Pmax);
    double e;
    if (w4 == 0) {
-      e = .25 * (EO(3, q.c) + EO(1, q.c));
-      e += LR_alt == 0 ? .25 * EO(1, q.cU) : .25 * EO(0, q.cUR);
-      e += UD_alt == 0 ? .25 * EO(3, q.cR) : .25 * EO(2, q.cUR);
+      e = .25 * (c.eb + c.ea);
+      e += LR_alt == 0 ? .25 * cU.ea : .25 * cUR.ea;
+      e += UD_alt == 0 ? .25 * cR.eb : .25 * cUR.eb;
    } else if (w4 == 1) {
-      e = .25 * (EO(3, q.cR) + EO(0, q.cR));
-      e += LR_alt == 0 ? .25 * EO(1, q.cU) : .25 * EO(0, q.cUR);
-      e += UD_alt == 0 ? .25 * EO(3, q.c) : .25 * EO(2, q.cU);
+      e = .25 * (cR.eb + cR.ea);
+      e += LR_alt == 0 ? .25 * cU.ea : .25 * cUR.ea;
+      e += UD_alt == 0 ? .25 * c.eb : .25 * cU.eb;
    } else if (w4 == 2) {
-      e = .25 * (EO(2, q.cU) + EO(1, q.cU));
-      e += LR_alt == 0 ? .25 * EO(1, q.c) : .25 * EO(0, q.cR);
-      e += UD_alt == 0 ? .25 * EO(3, q.cR) : .25 * EO(2, q.cUR);
+      e = .25 * (cU.eb + cU.ea);
+      e += LR_alt == 0 ? .25 * c.ea : .25 * cR.ea;
+      e += UD_alt == 0 ? .25 * cR.eb : .25 * cUR.eb;
    } else {
-      e = .25 * (EO(2, q.cUR) + EO(0, q.cUR));
-      e += LR_alt == 0 ? .25 * EO(1, q.c) : .25 * EO(0, q.cR);
-      e += UD_alt == 0 ? .25 * EO(3, q.c) : .25 * EO(2, q.cU);
+      e = .25 * (cUR.eb + cUR.ea);
+      e += LR_alt == 0 ? .25 * c.ea : .25 * cR.ea;
+      e += UD_alt == 0 ? .25 * c.eb : .25 * cU.eb;
    }
-#undef EO
-   int cs[4] = {q.c, q.cR, q.cU, q.cUR};
-   double xp1 = piph[cs[w4]];
-   double xm1 = piph[cs[w4b]] - dphi[cs[w4b]];
+   double xp1 = w4 == 0 ? c.up : w4 == 1 ? cR.up : w4 == 2 ? cU.up : cUR.up;
+   double xm1 = w4b == 0 ? c.lo : w4b == 1 ? cR.lo : w4b == 2 ? cU.lo : cUR.lo;
    *E = e;
    *dl = .5 * (r + r) * get_dp(xp1, xm1, Pmax);
 }

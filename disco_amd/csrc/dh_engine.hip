@@ -17,13 +17,22 @@
 using namespace dh;
 
 static constexpr int TPB = 256;
+static constexpr unsigned NXCD = 8;  // accelerator complex dies, each with its own L2
 static inline unsigned nblocks(long n) { return (unsigned)((n + TPB - 1) / TPB); }
+// Grid of the XCD-aware launches: rounded up to a whole number of blocks per XCD.
+static inline unsigned nblocks_xcd(long n) { return (nblocks(n) + NXCD - 1) / NXCD * NXCD; }
 
 // One thread per element: t = cell, face, face track or boundary line.
+// Workgroups are dispatched to the XCDs round-robin (block b to XCD b mod 8);
+// renumbering them so that each XCD runs one contiguous eighth of the
+// elements keeps the neighbouring annuli a kernel gathers from in that
+// XCD's L2.
 template <class K>
 __global__ __launch_bounds__(TPB) void k_run(K k, long n)
 {
-   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+   unsigned per = gridDim.x / NXCD;
+   unsigned lb = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
+   long t = (long)lb * blockDim.x + threadIdx.x;
    if (t < n) k(t);
 }
 
@@ -238,7 +247,7 @@ struct HipExec {
       if (n <= 0) return DH_OK;
       {
          Bracket b(*this, name);
-         k_run<K><<<nblocks(n), TPB, 0, stream>>>(k, n);
+         k_run<K><<<nblocks_xcd(n), TPB, 0, stream>>>(k, n);
       }
       LAUNCH_CHECK();
       return DH_OK;

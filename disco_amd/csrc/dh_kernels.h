@@ -34,7 +34,8 @@ struct IsMHD {
 // One direction's face list: faces of annulus pair P are fI[P] .. fI[P+1]-1.
 struct Faces {
    int n, npairs;
-   const int *fI;
+   const int *fI;   // [npairs+1] first face of every annulus pair
+   const int *pf;   // [n] annulus pair of every face (static, like fI)
    int *L, *R;
    signed char *LR;
    double *dA, *phic, *dphi;
@@ -584,7 +585,7 @@ struct KFluxTrans {
    DHD void operator()(long f) const
    {
       constexpr bool mhd = IsMHD<H>::value;
-      int Pi = pair_of_face(F.fI, F.npairs, (int)f);
+      int Pi = F.pf[f];
       Pair pr = pair_of(m, dim, Pi);
       if (!pair_has_flux(m, mhd && P.ct, dim, pr.j, pr.k)) return;
       size_t nt = (size_t)m.ntot, nfs = (size_t)F.n;
@@ -1188,7 +1189,7 @@ struct KLineSteps {
    DHD void operator()(long t) const
    {
       int f = (int)t;
-      int JK = pair_of_face(F.fI, F.npairs, f);
+      int JK = F.pf[f];
       Pair pr = pair_of(m, DIM, JK);
       int base = F.fI[JK];
       FaceRing self = make_face_ring(F.up, base, F.fI[JK + 1] - base);
@@ -1220,13 +1221,14 @@ struct KEdgeAdjust {
       const Faces &F = dim == 1 ? Fr : Fz;
       const int *estep = dim == 1 ? estep_r : estep_z;
       int f = (int)(dim == 1 ? t : t - Fr.n);
-      int JK = pair_of_face(F.fI, F.npairs, f);
+      int JK = F.pf[f];
       Pair pr = pair_of(m, dim, JK);
       int base = F.fI[JK];
       int N = F.fI[JK + 1] - base;
       int x = f - base;
       int fp = base + (x == 0 ? N - 1 : x - 1);
-      size_t slot = phi_slot(nt, dim, F.LR[f], F.L[f], F.R[f]);
+      int fL = F.L[f], fR = F.R[f];
+      size_t slot = phi_slot(nt, dim, F.LR[f], fL, fR);
       double v = Phi[slot];
       for (int role = 0; role < 2; ++role) {
          EdgeRole er = edge_role(m, dim, pr.j, pr.k, role);
@@ -1236,6 +1238,15 @@ struct KEdgeAdjust {
          int h0, h1 = estep[(size_t)role * F.n + f], t0, t1;
          range_from_steps(h1, estep[(size_t)role * F.n + fp], x == 0, N + oN, &h0, &t0, &t1);
          double rline = m.r_jph[er.lj];
+         // Corner positions of the face's L and R cells and of the facing
+         // face's: an r face (c, cR) is the line's first face, (cU, cUR) its
+         // second; a z face (c, cU) the first, (cR, cUR) the second.
+         int sL = er.self_first ? 0 : (dim == 1 ? 2 : 1);
+         int sR = er.self_first ? (dim == 1 ? 1 : 2) : 3;
+         int oL = er.self_first ? (dim == 1 ? 2 : 1) : 0;
+         int oR = er.self_first ? 3 : (dim == 1 ? 1 : 2);
+         EdgeCell own_L = edge_cell(piph, dphi, Eown, nt, fL, sL);
+         EdgeCell own_R = edge_cell(piph, dphi, Eown, nt, fR, sR);
          for (int part = 0; part < 2; ++part) {
             int lo = part == 0 ? h0 : t0, hi = part == 0 ? h1 : t1;
             // While this face is current, the merge has consumed x of its
@@ -1245,15 +1256,18 @@ struct KEdgeAdjust {
             for (int e = lo; e <= hi; ++e) {
                int y = e - consumed;
                int g = obase + (y == oN ? 0 : y);
-               int f1 = er.self_first ? f : g, f2 = er.self_first ? g : f;
-               EdgeCells q;
-               if (dim == 1) {  // f1 = D face (c, cR), f2 = U face (cU, cUR)
-                  q.c = F.L[f1]; q.cR = F.R[f1]; q.cU = F.L[f2]; q.cUR = F.R[f2];
-               } else {         // f1 = L face (c, cU), f2 = R face (cR, cUR)
-                  q.c = F.L[f1]; q.cU = F.R[f1]; q.cR = F.L[f2]; q.cUR = F.R[f2];
-               }
+               EdgeCell other_L = edge_cell(piph, dphi, Eown, nt, F.L[g], oL);
+               EdgeCell other_R = edge_cell(piph, dphi, Eown, nt, F.R[g], oR);
+               auto at = [&](int p) {
+                  EdgeCell r;
+                  r.up = p == sL ? own_L.up : p == sR ? own_R.up : p == oL ? other_L.up : other_R.up;
+                  r.lo = p == sL ? own_L.lo : p == sR ? own_R.lo : p == oL ? other_L.lo : other_R.lo;
+                  r.ea = p == sL ? own_L.ea : p == sR ? own_R.ea : p == oL ? other_L.ea : other_R.ea;
+                  r.eb = p == sL ? own_L.eb : p == sR ? own_R.eb : p == oL ? other_L.eb : other_R.eb;
+                  return r;
+               };
                double E, dl;
-               edge_segment(piph, dphi, Eown, nt, q, rline, m.Pmax, &E, &dl);
+               edge_segment(at(0), at(1), at(2), at(3), rline, m.Pmax, &E, &dl);
                double Edldt = E * dl * dt;
                if (er.sgn > 0) v += Edldt;
                else v -= Edldt;
@@ -1282,7 +1296,7 @@ struct KFlipFlags {
    }
    DHD void operator()(long f) const
    {
-      int JK = pair_of_face(F.fI, F.npairs, (int)f);
+      int JK = F.pf[f];
       int base = F.fI[JK], N = F.fI[JK + 1] - base;
       int n = (int)f - base;
       int fm = base + (n == 0 ? N - 1 : n - 1);

@@ -102,15 +102,65 @@ DHD bool a_first(const RA &A, int x, const RB &B, int y, double Pmax)
    return D < 0.;
 }
 
-// Number of A elements consumed before step d of the walk (merge path).
-template <class RA, class RB>
-DHD int merge_x(const RA &A, const RB &B, int d, double Pmax)
+// Position of element x (x < N) in its ring's unwrapped order, as a_first
+// reads it.
+template <class R>
+DHD double ring_pos(const R &A, int x, double Pmax)
 {
-   int lo = d - B.N > 0 ? d - B.N : 0;
-   int hi = d < A.N ? d : A.N;
+   double p = ring_key(A, x);
+   return (x > 0 && p < A.p0) ? p + Pmax : p;
+}
+
+// Estimate of how many elements of ring B lie below unwrapped position u:
+// the elements are spread nearly evenly over one period from B's start.
+template <class R>
+DHD int count_below(const R &B, double u, double Pmax)
+{
+   double g = (u - B.p0) * (double)B.N / Pmax;
+   if (!(g > 0.0)) return 0;
+   if (g >= (double)B.N) return B.N;
+   return (int)ceil(g);
+}
+
+// Largest m in [0, N] with P(m), for P monotone (true up to the answer,
+// false after) with P(0) taken as true: a galloping search out from the
+// estimate g, then bisection inside the bracket it finds.  For monotone P
+// the result is the plain bisection's, in a few probes when g is close.
+template <class Pred>
+DHD int last_true(int N, int g, const Pred &P)
+{
+   int lo, hi;
+   if (g < 0) g = 0;
+   if (g > N) g = N;
+   if (g == 0 || P(g)) {
+      lo = g;
+      hi = N;
+      for (int step = 1;; step <<= 1) {
+         int probe = lo + step;
+         if (probe > N) break;
+         if (P(probe)) {
+            lo = probe;
+         } else {
+            hi = probe - 1;
+            break;
+         }
+      }
+   } else {
+      hi = g - 1;
+      lo = 0;
+      for (int step = 1;; step <<= 1) {
+         int probe = g - step;
+         if (probe <= 0) break;
+         if (P(probe)) {
+            lo = probe;
+            break;
+         }
+         hi = probe - 1;
+      }
+   }
    while (lo < hi) {
       int mid = (lo + hi + 1) >> 1;
-      if (a_first(A, mid - 1, B, d - mid, Pmax)) lo = mid;
+      if (P(mid)) lo = mid;
       else hi = mid - 1;
    }
    return lo;
@@ -121,67 +171,16 @@ DHD int merge_x(const RA &A, const RB &B, int d, double Pmax)
 template <class RA, class RB>
 DHD int a_step(const RA &A, int x, const RB &B, double Pmax)
 {
-   int lo = 0, hi = B.N;
-   while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (!a_first(A, x, B, mid - 1, Pmax)) lo = mid;
-      else hi = mid - 1;
-   }
-   return x + lo;
+   int g = count_below(B, ring_pos(A, x, Pmax), Pmax);
+   return x + last_true(B.N, g, [&](int mid) { return !a_first(A, x, B, mid - 1, Pmax); });
 }
 
 // Step at which B's element y (y < N_B) is consumed.
 template <class RA, class RB>
 DHD int b_step(const RA &A, const RB &B, int y, double Pmax)
 {
-   int lo = 0, hi = A.N;
-   while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (a_first(A, mid - 1, B, y, Pmax)) lo = mid;
-      else hi = mid - 1;
-   }
-   return y + lo;
-}
-
-// Faces (indices within the pair's list, inclusive ranges) owned by one cell:
-// a head range and, for the ring's start cell only, a tail range at the end
-// of the list (the walk ends on the start cell again).  Empty: lo > hi.
-struct FaceRange {
-   int h0, h1, t0, t1;
-};
-
-// Ranges of A's element x (the cell is the L side of these faces).
-template <class RA, class RB>
-DHD FaceRange ranges_as_a(const RA &A, int x, const RB &B, double Pmax)
-{
-   FaceRange r;
-   int nf = A.N + B.N;
-   r.h0 = x == 0 ? 0 : a_step(A, x - 1, B, Pmax) + 1;
-   r.h1 = a_step(A, x, B, Pmax);
-   r.t0 = 1;
-   r.t1 = 0;
-   if (x == 0) {
-      r.t0 = (A.N > 1 ? a_step(A, A.N - 1, B, Pmax) : r.h1) + 1;
-      r.t1 = nf - 1;
-   }
-   return r;
-}
-
-// Ranges of B's element y (the cell is the R side of these faces).
-template <class RA, class RB>
-DHD FaceRange ranges_as_b(const RA &A, const RB &B, int y, double Pmax)
-{
-   FaceRange r;
-   int nf = A.N + B.N;
-   r.h0 = y == 0 ? 0 : b_step(A, B, y - 1, Pmax) + 1;
-   r.h1 = b_step(A, B, y, Pmax);
-   r.t0 = 1;
-   r.t1 = 0;
-   if (y == 0) {
-      r.t0 = (B.N > 1 ? b_step(A, B, B.N - 1, Pmax) : r.h1) + 1;
-      r.t1 = nf - 1;
-   }
-   return r;
+   int g = count_below(A, ring_pos(B, y, Pmax), Pmax);
+   return y + last_true(A.N, g, [&](int mid) { return a_first(A, mid - 1, B, y, Pmax); });
 }
 
 // Annulus pair of a face list: dim 1 pairs (j, j+1), dim 2 pairs (k, k+1)
@@ -207,18 +206,6 @@ DHD Pair pair_of(const Mesh &m, int dim, int JK)
       p.b = p.a + m.Nr;
    }
    return p;
-}
-
-// Largest pair index P with fI[P] <= f.
-DHD int pair_of_face(const int *fI, int npairs, int f)
-{
-   int lo = 0, hi = npairs - 1;
-   while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (fI[mid] <= f) lo = mid;
-      else hi = mid - 1;
-   }
-   return lo;
 }
 
 }  // namespace dh

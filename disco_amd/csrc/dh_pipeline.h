@@ -185,6 +185,13 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
       DH_TRY(alloc_n(x, &dfI, sizeof(int) * fI.size()));
       DH_TRY(x.put(dfI, fI.data(), sizeof(int) * fI.size()));
       F.fI = dfI;
+      std::vector<int> pf(n);
+      for (int P = 0; P < F.npairs; ++P)
+         for (int f = fI[P]; f < fI[P + 1]; ++f) pf[f] = P;
+      int *dpf = nullptr;
+      DH_TRY(alloc_n(x, &dpf, sizeof(int) * (size_t)n));
+      DH_TRY(x.put(dpf, pf.data(), sizeof(int) * (size_t)n));
+      F.pf = dpf;
       DH_TRY(alloc_n(x, &F.L, sizeof(int) * n));
       DH_TRY(alloc_n(x, &F.R, sizeof(int) * n));
       DH_TRY(alloc_n(x, &F.LR, sizeof(signed char) * n));

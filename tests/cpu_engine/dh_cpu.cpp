@@ -190,6 +190,11 @@ int dhx_riemann(dh_ctx *c, const double *pL, const double *pR, double r, const d
 extern "C" long dhx_flip_count(dh_ctx *c) { return c ? c->x.flipped : -1; }
 extern "C" long dhx_bc_rings(dh_ctx *c) { return c ? c->x.bc_rings : -1; }
 extern "C" long dhx_bc_misses(dh_ctx *c) { return c ? c->x.bc_misses : -1; }
+// The galloping search of the merge steps on a table of predicate values.
+extern "C" int dhx_last_true(int N, int g, const unsigned char *truth)
+{
+   return last_true(N, g, [&](int m) { return truth[m] != 0; });
+}
 
 extern "C" {
 

@@ -126,3 +126,15 @@ def test_cpu_build_faces_match_oracle(cpu_lib):
         assert np.array_equal(LR, f["LRtype"])
         assert np.array_equal(dA, f["dA"]) and np.array_equal(cm[:, 1], f["phic"])
     g.close()
+
+
+def test_galloping_search_equals_bisection(cpu_lib):
+    """last_true (dh_mesh.h) finds the boundary of a monotone predicate from
+    any estimate: every boundary and estimate of lists up to 40 long."""
+    cpu_lib.dhx_last_true.argtypes = [C.c_int, C.c_int, C.c_char_p]
+    cpu_lib.dhx_last_true.restype = C.c_int
+    for n in range(0, 41):
+        for ans in range(0, n + 1):
+            truth = bytes([1] * (ans + 1) + [0] * (n - ans))
+            for g in range(-2, n + 3):
+                assert cpu_lib.dhx_last_true(n, g, truth) == ans, (n, ans, g)

@@ -10,6 +10,15 @@ below 1e-12 where the shear per step is below one cell.  Dominant
 fields (density, pressure) agree far tighter; fields that vanish in the
 exact solution (v_r of the steady vortex) carry relative round-off of a few
 1e-12.
+
+One fixture is ill-conditioned: on the 8x2 MHD disk (mhd_kep_r8z2) the
+Keplerian shear moves cell edges by more than a cell per step, and a
+one-ulp change of pow or sincos on half their arguments in the reference's
+own CPU build moves B_r after 10 steps by 1.1e-10 to 1.4e-10
+(tests/test_conditioning.py pins that).  The device libm rounds differently
+from glibc in the last bit, so for that case the trajectory is held to
+ILL_TOL and every step, restarted from one state on both sides, to
+STEP_TOL.
 """
 import os
 import subprocess
@@ -29,6 +38,11 @@ TOL = PARITY_TOL
 # The CFL time of a Keplerian mesh involves |omega - w| with w close to
 # omega, which cancels: device and host libm differ there by ~1e-12.
 DT_TOL = 1e-11
+# Trajectory bound of the ill-conditioned fixtures (see above) and the
+# bound of one step from a common state.
+ILL_CONDITIONED = {"mhd_kep_r8z2"}
+ILL_TOL = 1e-9
+STEP_TOL = 1e-12
 
 
 def _run_gpu(cfg, par, state, dts):
@@ -48,7 +62,7 @@ def test_gpu_matches_reference_fixture(name):
     print(f"case {name}: {a.n_real()} cells, {len(dts)} steps", flush=True)
     out, dts_gpu = _run_gpu(cfg, par, a, dts)
     np.testing.assert_allclose(dts_gpu, dts, rtol=DT_TOL)
-    worst = compare_states(out, b, TOL)
+    worst = compare_states(out, b, ILL_TOL if name in ILL_CONDITIONED else TOL)
     np.testing.assert_allclose(out.dphi, b.dphi, rtol=0, atol=1e-13)
     e = field_errors(out, b)
     print(f"  worst rel L1 {worst:.2e}; rho {e['prim0']:.2e}, P {e['prim1']:.2e}", flush=True)
@@ -58,6 +72,35 @@ def test_gpu_matches_reference_fixture(name):
         ephi = rel_l1(out.phi_flux[:, m], b.phi_flux[:, m])
         print(f"  Phi rel L1 {ephi:.2e}", flush=True)
         assert ephi <= TOL
+
+
+@pytest.mark.parametrize("name", sorted(ILL_CONDITIONED))
+def test_gpu_step_matches_oracle_from_common_state(name):
+    """Every step of the ill-conditioned fixture, taken by the device and by
+    the oracle from the same state (the device's trajectory so far), agrees
+    to round-off, as does the CFL dt."""
+    cfg, par, a, b, dts = load_golden(name)
+    print(f"case {name}: {a.n_real()} cells, {len(dts)} steps from common states", flush=True)
+    g = capi.DiscoHip(cfg, par, a)
+    worst = 0.0
+    for n, dt in enumerate(dts):
+        s = g.download(a)
+        o = Oracle(cfg, par, s)
+        h = capi.DiscoHip(cfg, par, s)
+        dto, dth = o.getmindt(), h.getmindt()
+        assert abs(dth - dto) <= DT_TOL * dto, (n, dth, dto)
+        o.timestep(dt)
+        h.timestep(dt)
+        e = field_errors(h.download(a), o.get_state())
+        m = a.real_mask()
+        ephi = rel_l1(h.download(a).phi_flux[:, m], o.get_state().phi_flux[:, m])
+        h.close()
+        k = max(e, key=e.get)
+        worst = max(worst, e[k], ephi)
+        assert e[k] <= STEP_TOL and ephi <= STEP_TOL, (n, k, e[k], ephi)
+        g.timestep(dt)
+    g.close()
+    print(f"  worst one-step rel L1 {worst:.2e}", flush=True)
 
 
 def _ref_initial(cfg, par):
