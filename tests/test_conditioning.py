@@ -8,8 +8,8 @@ the math library moves each case after its steps.  The 8x2 MHD disk, whose
 Keplerian shear moves cell edges by more than a cell per step, moves by more
 than the 1e-10 parity bar, which is why the GPU tests hold that fixture's
 trajectory to a looser bound and each of its steps, from a common state, to
-round-off.  The MHD disks the other GPU parity tests run move by less than
-a tenth of the bar.
+the bar (B) and to round-off (density, pressure, velocity, Phi).  The MHD
+disks the other GPU parity tests run move by less than a tenth of the bar.
 """
 import os
 import subprocess
@@ -101,6 +101,19 @@ def test_unperturbed_shim_is_exact():
 def test_ill_conditioned_fixture_amplifies_one_ulp(perturb):
     cfg, par, a, b, dts = load_golden("mhd_kep_r8z2")
     assert max(field_errors(_perturbed(cfg, par, a, dts, perturb), b).values()) > 1e-10
+
+
+def test_ill_conditioned_fixture_one_step():
+    """One step of the 8x2 disk from its initial state: B moves by more than
+    1e-12 under a one-ulp pow, density, pressure and velocity do not."""
+    cfg, par, a, b, dts = load_golden("mhd_kep_r8z2")
+    g = capi.DiscoHip(cfg, par, a, library=capi.load_library(cpu_build.build()))
+    g.timestep(dts[0])
+    ref = g.download(a)
+    g.close()
+    e = field_errors(_perturbed(cfg, par, a, dts[:1], "pow"), ref)
+    assert max(e[f"prim{q}"] for q in (5, 6, 7)) > 5e-12
+    assert max(e[f"prim{q}"] for q in range(5)) < 1e-13
 
 
 @pytest.mark.parametrize("cname,over,steps", [
