@@ -42,7 +42,7 @@ _PARAM_DOUBLES = ["gamma_law", "plm", "cfl", "max_dt", "rho_floor", "p_floor",
 class DhParams(C.Structure):
     _fields_ = ([(n, C.c_int) for n in _PARAM_INTS] + [("pad_", C.c_int * 3)]
                 + [(n, C.c_double) for n in _PARAM_DOUBLES]
-                + [("init_par", C.c_double * 9)])
+                + [("init_par", C.c_double * 9), ("phi_order", C.c_double)])
 
 
 PLANET_STRIDE = 8
@@ -82,6 +82,7 @@ def make_params(cfg: "cfgmod.Config", par: Dict[str, object]) -> DhParams:
     for i in range(9):
         p.init_par[i] = float(full[f"Init_Par{i}"])
     p.bc_rmin, p.bc_rmax, p.bc_zmin, p.bc_zmax = cfg.dh_bc(full)
+    p.phi_order = 2.0  # PHI_ORDER, planet.c:4
     return p
 
 
@@ -128,6 +129,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "dh_read_timers": ([vp, C.c_int, C.c_void_p, _dp, C.POINTER(C.c_long), _ip], C.c_int),
         "dh_face_count": ([vp, C.c_int, C.POINTER(C.c_long)], C.c_int),
         "dh_get_faces": ([vp, C.c_int, _ip, _ip, _ip, _dp, _dp], C.c_int),
+        "dh_add_diagnostics": ([vp, C.c_int, C.c_double, _dp, C.c_int], C.c_int),
+        "dh_get_diagnostics": ([vp, _dp], C.c_int),
+        "dh_zero_diagnostics": ([vp], C.c_int),
+        "dh_avg_diagnostics": ([vp, C.c_double], C.c_int),
         "dh_last_error": ([], C.c_char_p),
     }
     for name, (args, res) in sig.items():
@@ -140,7 +145,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
 EXPORTED_SYMBOLS = ["dh_create", "dh_destroy", "dh_sizes", "dh_upload_state",
                     "dh_download_state", "dh_init_cells", "dh_set_planets", "dh_getmindt", "dh_timestep",
                     "dh_face_count", "dh_get_faces", "dh_sync", "dh_enable_timers",
-                    "dh_read_timers", "dh_last_error"]
+                    "dh_read_timers", "dh_add_diagnostics", "dh_get_diagnostics",
+                    "dh_zero_diagnostics", "dh_avg_diagnostics", "dh_last_error"]
 
 _LIB: Optional[C.CDLL] = None
 
@@ -207,6 +213,27 @@ class DiscoHip:
         ip = lambda a: a.ctypes.data_as(_ip)  # noqa: E731
         self._check(self.lib.dh_get_faces(self.ctx, dim, ip(L), ip(R), ip(LR), _ptr(cm), _ptr(dA)))
         return L, R, LR, cm, dA
+
+    def add_diagnostics(self, analysis: str, dt: float, planets: np.ndarray):
+        """add_diagnostics (analysis.c:52-106) of the named module with the
+        given planets (the state after the step)."""
+        kind, _ = cfgmod.DIAG_KINDS[analysis]
+        pl = np.ascontiguousarray(planets, dtype=np.float64).reshape(-1, PLANET_STRIDE)
+        self._check(self.lib.dh_add_diagnostics(self.ctx, kind, dt, _ptr(pl), pl.shape[0]))
+        self._diag = analysis
+
+    def diagnostics(self) -> np.ndarray:
+        """The accumulated Qrz as [Nr*Nz, nq]."""
+        nq = cfgmod.DIAG_KINDS[self._diag][1]
+        out = np.zeros((len(self.grid.np_), nq))
+        self._check(self.lib.dh_get_diagnostics(self.ctx, _ptr(out)))
+        return out
+
+    def avg_diagnostics(self, t_avg: float):
+        self._check(self.lib.dh_avg_diagnostics(self.ctx, t_avg))
+
+    def zero_diagnostics(self):
+        self._check(self.lib.dh_zero_diagnostics(self.ctx))
 
     def init_cells(self):
         self._check(self.lib.dh_init_cells(self.ctx))

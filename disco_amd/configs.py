@@ -26,6 +26,7 @@ class Config:
     num_n: int
     ct_mode: int
     par: Tuple[Tuple[str, object], ...]
+    analysis: str = "diag_euler"  # Diagnostics module (ANALYSIS)
 
     def par_dict(self, **overrides) -> Dict[str, object]:
         d = dict(self.par)
@@ -59,9 +60,14 @@ class Config:
         return tuple(BC_NONE if nobc[i] else sides[i] for i in range(4))
 
 
-def _cfg(name, hydro, initial, boundary, planets, num_c, num_n, ct_mode, **par):
+def _cfg(name, hydro, initial, boundary, planets, num_c, num_n, ct_mode, analysis="diag_euler",
+         **par):
     return Config(name, hydro, initial, boundary, planets, num_c, num_n, ct_mode,
-                  tuple(par.items()))
+                  tuple(par.items()), analysis)
+
+
+# DH_DIAG_* (include/disco_hip.h) and the number of quantities of each module
+DIAG_KINDS = {"diag_euler": (1, 8), "diag_disk2d": (2, 5), "diag_mhdmri": (3, 18)}
 
 
 CONFIGS: Dict[str, Config] = {}
@@ -76,7 +82,7 @@ CONFIGS["euler_yee"] = _cfg(
     Exact_Mesh_Omega=1, Adiabatic_Index=5.0 / 3.0, Init_Par1=5.0)
 
 CONFIGS["euler_yee_n1"] = dataclasses.replace(
-    CONFIGS["euler_yee"], name="euler_yee_n1", num_n=1)
+    CONFIGS["euler_yee"], name="euler_yee_n1", num_n=1, analysis="diag_disk2d")
 
 # The reference's own Gaussian vortex (Initial/vortex.c), HLLC, polar axis.
 CONFIGS["euler_vortex"] = _cfg(
@@ -92,12 +98,20 @@ CONFIGS["euler_vortex"] = _cfg(
 # constrained transport, Keplerian mesh motion and energy frame.
 CONFIGS["mhd_kep"] = _cfg(
     "mhd_kep", "mhd", "kep_mhd_disk", "fixed_r_outflow_z", "central", 8, 0, 2,
+    analysis="diag_mhdmri",
     Num_R=256, Num_Z=64, R_Min=0.5, R_Max=2.0, Z_Min=-0.25, Z_Max=0.25,
     NoBC_Rmin=0, NoBC_Rmax=0, NoBC_Zmin=0, NoBC_Zmax=0, CFL=0.5, PLM=1.5,
     Riemann_Solver=2, Mesh_Motion=4, Exact_Mesh_Omega=2, Energy_Omega=2,
     Constrained_Transport=1, Isothermal=1, Cs2_Profile=4, Mach_Number=20.0,
     Metric_Par2=1.0, Grav2D=1, Adiabatic_Index=5.0 / 3.0, Init_Par1=1000.0,
     Noise_Type=3, Noise_Rel=5e-5, Noise_Abs=0.0)
+
+
+# The disk with a companion of mass ratio 1e-3 at r = 1.25 (Planets/central.c
+# reads Mass_Ratio) and the Euler diagnostics, which read planet 1.
+CONFIGS["mhd_kep_p2"] = dataclasses.replace(
+    CONFIGS["mhd_kep"], name="mhd_kep_p2", analysis="diag_euler",
+    par=CONFIGS["mhd_kep"].par + (("Mass_Ratio", 1e-3),))
 
 
 def get(name: str) -> Config:

@@ -23,18 +23,36 @@ static inline unsigned nblocks(long n) { return (unsigned)((n + TPB - 1) / TPB);
 static inline unsigned nblocks_xcd(long n) { return (nblocks(n) + NXCD - 1) / NXCD * NXCD; }
 
 // One thread per element: t = cell, face, face track or boundary line.
-// Workgroups are dispatched to the XCDs round-robin (block b to XCD b mod 8);
-// renumbering them so that each XCD runs one contiguous eighth of the
-// elements keeps the neighbouring annuli a kernel gathers from in that
-// XCD's L2.
 template <class K>
 __global__ __launch_bounds__(TPB) void k_run(K k, long n)
+{
+   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (t < n) k(t);
+}
+
+// The same with the workgroups renumbered so that each XCD runs one
+// contiguous eighth of the elements (workgroups are dispatched to the XCDs
+// round-robin, block b to XCD b mod 8): the neighbouring annuli a kernel
+// gathers from then stay in that XCD's L2.  Measured faster only for the
+// kernels that declare xcd_contiguous (the r slopes and the face lists);
+// streaming kernels run slower with it.
+template <class K>
+__global__ __launch_bounds__(TPB) void k_run_xcd(K k, long n)
 {
    unsigned per = gridDim.x / NXCD;
    unsigned lb = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
    long t = (long)lb * blockDim.x + threadIdx.x;
    if (t < n) k(t);
 }
+
+template <class K, class = void>
+struct XcdContiguous {
+   static constexpr bool value = false;
+};
+template <class K>
+struct XcdContiguous<K, decltype((void)K::xcd_contiguous)> {
+   static constexpr bool value = K::xcd_contiguous;
+};
 
 // Start cell of every annulus (faces_alt.c:59-78): one wavefront per
 // annulus scans 64 cells at a time with ballots for the first cell in the
@@ -158,6 +176,46 @@ __global__ __launch_bounds__(TPB) void k_bc_scan(Mesh m, int side, const double 
    if (s_miss && tid == 0) KBcScan{m, side, piph, dphi, start}(t);
 }
 
+// add_diagnostics (analysis.c:52-106): one wavefront per annulus; each lane
+// sums get_diagnostics x dV and dV over every 64th cell, then the wave adds
+// the lanes' sums.  (The host build sums the cells in order, as the
+// reference does; the two differ by rounding only.)
+template <int KIND>
+__global__ __launch_bounds__(TPB) void k_diag(Mesh m, dh_params P, DiagPlanet p1,
+                                              const double *prim, const double *piph,
+                                              const double *dphi, double dt, double *Qrz)
+{
+   constexpr int NQD = KIND == DH_DIAG_EULER ? 8 : KIND == DH_DIAG_DISK2D ? 5 : 18;
+   int a = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+   int lane = threadIdx.x & 63;
+   if (a >= m.nann) return;
+   size_t nt = (size_t)m.ntot;
+   int j = a % m.Nr, k = a / m.Nr;
+   double sum[NQD], vol = 0.0;
+#pragma unroll
+   for (int q = 0; q < NQD; ++q) sum[q] = 0.0;
+   for (int i = lane; i < m.Np[a]; i += 64) {
+      long c = m.off[a] + i;
+      double x[3], dV, p[BZZ + 1], Q[NQD];
+      diag_geometry(m, piph, dphi, c, j, k, x, &dV);
+#pragma unroll
+      for (int q = 0; q < BZZ + 1; ++q)
+         if (q < diag_fields(KIND)) p[q] = prim[q * nt + c];
+      diag_cell(KIND, P, p1, x[0], x[1], x[2], p, Q);
+#pragma unroll
+      for (int q = 0; q < NQD; ++q) sum[q] += Q[q] * dV;
+      vol += dV;
+   }
+   for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+      for (int q = 0; q < NQD; ++q) sum[q] += __shfl_down(sum[q], o, 64);
+      vol += __shfl_down(vol, o, 64);
+   }
+   if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < NQD; ++q) Qrz[(size_t)a * NQD + q] += sum[q] * dt / vol;
+}
+
 // CFL reduction over real cells (misc.c:39-84).  Positive doubles order like
 // their bit patterns, so the minimum is an unsigned 64-bit atomic min.
 template <int NQ, class H>
@@ -247,7 +305,10 @@ struct HipExec {
       if (n <= 0) return DH_OK;
       {
          Bracket b(*this, name);
-         k_run<K><<<nblocks_xcd(n), TPB, 0, stream>>>(k, n);
+         if constexpr (XcdContiguous<K>::value)
+            k_run_xcd<K><<<nblocks_xcd(n), TPB, 0, stream>>>(k, n);
+         else
+            k_run<K><<<nblocks(n), TPB, 0, stream>>>(k, n);
       }
       LAUNCH_CHECK();
       return DH_OK;
@@ -281,6 +342,23 @@ struct HipExec {
       {
          Bracket b(*this, "flips");
          k_flips<<<(unsigned)((F.npairs + 3) / 4), TPB, 0, stream>>>(m, dim, F, flag, Phi, rkPhi);
+      }
+      LAUNCH_CHECK();
+      return DH_OK;
+   }
+
+   int diag(const Mesh &m, const dh_params &P, int kind, const DiagPlanet &p1,
+            const double *prim, const double *piph, const double *dphi, double dt, double *Qrz)
+   {
+      unsigned nb = (unsigned)((m.nann + 3) / 4);
+      {
+         Bracket b(*this, "diagnostics");
+         if (kind == DH_DIAG_EULER)
+            k_diag<DH_DIAG_EULER><<<nb, TPB, 0, stream>>>(m, P, p1, prim, piph, dphi, dt, Qrz);
+         else if (kind == DH_DIAG_DISK2D)
+            k_diag<DH_DIAG_DISK2D><<<nb, TPB, 0, stream>>>(m, P, p1, prim, piph, dphi, dt, Qrz);
+         else
+            k_diag<DH_DIAG_MHDMRI><<<nb, TPB, 0, stream>>>(m, P, p1, prim, piph, dphi, dt, Qrz);
       }
       LAUNCH_CHECK();
       return DH_OK;
@@ -492,6 +570,30 @@ int dh_sync(dh_ctx *c)
 {
    if (!c) return fail(DH_EINVAL, "null context");
    return c->x.sync();
+}
+
+int dh_add_diagnostics(dh_ctx *c, int kind, double dt, const double *planets, int npl)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return add_diagnostics(c->x, c->b, kind, dt, planets, npl);
+}
+
+int dh_get_diagnostics(dh_ctx *c, double *Qrz)
+{
+   if (!c || !Qrz) return fail(DH_EINVAL, "null argument");
+   return get_diagnostics(c->x, c->b, Qrz);
+}
+
+int dh_zero_diagnostics(dh_ctx *c)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return zero_diagnostics(c->x, c->b);
+}
+
+int dh_avg_diagnostics(dh_ctx *c, double t_avg)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return avg_diagnostics(c->x, c->b, t_avg);
 }
 
 int dh_enable_timers(dh_ctx *c, int on)

@@ -26,6 +26,13 @@ struct PlanetSet {
    int n;
 };
 
+// Planet 1 as the Diagnostics modules read it (theDomain->thePlanets + 1,
+// used when Npl > 1).
+struct DiagPlanet {
+   double v[DH_PLANET_STRIDE];
+   int on;
+};
+
 template <class H>
 struct IsMHD {
    static constexpr bool value = H::NC == 8;
@@ -285,6 +292,7 @@ template <int DIM>
 struct KFaces {
    Mesh m;
    static constexpr int dim = DIM;
+   static constexpr bool xcd_contiguous = true;
    Faces F;
    const int *rng;
    const double *piph, *dphi;
@@ -367,6 +375,7 @@ template <int NQ, int DIM>
 struct KGrad {
    Mesh m;
    static constexpr int dim = DIM;
+   static constexpr bool xcd_contiguous = DIM == 1;
    double PLM;
    int num_c;
    Faces F;
@@ -648,10 +657,12 @@ struct KFluxTrans {
 // Point-mass gravity (planet.c:56-196)
 // ---------------------------------------------------------------------------
 
-DHD double fgrav(double M, double r, double eps, int type)
+// n is PHI_ORDER (planet.c:4), read at run time as the reference does: with
+// a constant the compiler would turn pow(r, 2.0) into r * r, which is not
+// always glibc's pow.
+DHD double fgrav(double M, double r, double eps, int type, double n)
 {
    if (type == 0) {
-      double n = 2.0;
       return M * pow(r, n - 1.) / pow(pow(r, n) + pow(eps, n), 1. + 1. / n);
    } else if (type == 1) {
       return M / ((r - 2 * M) * (r - 2 * M));
@@ -682,7 +693,7 @@ DHD void planet_src(const dh_params &P, const double *pl, const double *prim, do
    double dy = r * sinp - rpl * sinpl;
    double script_r = sqrt(dx * dx + dy * dy + z * z);
    double script_r_perp = sqrt(dx * dx + dy * dy);
-   double f1 = -fgrav(pl[2], script_r, pl[5], (int)pl[6]);
+   double f1 = -fgrav(pl[2], script_r, pl[5], (int)pl[6], P.phi_order);
    double cosa = dx / script_r_perp;
    double sina = dy / script_r_perp;
    double cosap = cosa * cosp + sina * sinp;
@@ -1619,6 +1630,134 @@ struct KInitCells {
          prim[q * nt + c] = pr[q];
       }
    }
+};
+
+// ---------------------------------------------------------------------------
+// Diagnostics (analysis.c:52-106 with the shipped Diagnostics modules)
+// ---------------------------------------------------------------------------
+
+// planetaryForce, mode 0 (planet.c:99-149): force per unit mass of planet pl
+// (DH_PLANET_STRIDE layout: r, phi, M, omega, vr, eps, type) at (r, phi, z).
+DHD void planetary_force(const dh_params &P, DiagPlanet pl, double r, double phi, double z,
+                         double *fr, double *fp, double *fz)
+{
+   double rp = pl.v[0], pp = pl.v[1];
+   if (P.grav2d == 1) z = 0.0;
+   else if (P.grav2d == 2) { rp = r; pp = phi; z = 1.0; }
+   double sinp, cosp, sinpp, cospp;
+   sincos(phi, &sinp, &cosp);
+   sincos(pp, &sinpp, &cospp);
+   double dx = r * cosp - rp * cospp;
+   double dy = r * sinp - rp * sinpp;
+   double script_r = sqrt(dx * dx + dy * dy + z * z);
+   double script_r_perp = sqrt(dx * dx + dy * dy);
+   double f1 = -fgrav(pl.v[2], script_r, pl.v[5], (int)pl.v[6], P.phi_order);
+   double cosa = dx / script_r_perp;
+   double sina = dy / script_r_perp;
+   double cosap = cosa * cosp + sina * sinp;
+   double sinap = sina * cosp - cosa * sinp;
+   if (script_r_perp <= 0.0) { cosap = 0.0; sinap = 0.0; }
+   double sint = script_r_perp / script_r;
+   double cost = z / script_r;
+   *fr = cosap * f1 * sint;
+   *fp = sinap * f1 * sint;
+   *fz = f1 * cost;
+}
+
+// Primitive fields a module reads: rho..u_z, and B for the MHD module.
+DHD int diag_fields(int kind) { return kind == DH_DIAG_MHDMRI ? BZZ + 1 : UZZ + 1; }
+
+DHD int diag_count(int kind)
+{
+   return kind == DH_DIAG_EULER ? 8 : kind == DH_DIAG_DISK2D ? 5 : kind == DH_DIAG_MHDMRI ? 18 : 0;
+}
+
+// get_diagnostics of the module `kind` at x = {r, phi, z} (Diagnostics/
+// diag_euler.c:17-45, diag_disk2d.c:17-39, diag_mhdmri.c:17-59); p1 is
+// planet 1 (p1.on when Npl > 1).
+DHD void diag_cell(int kind, const dh_params &P, DiagPlanet p1, double r, double phi, double z,
+                   const double *prim, double *Q)
+{
+   double rho = prim[RHO], Pp = prim[PPP], vr = prim[URR], omega = prim[UPP], vz = prim[UZZ];
+   if (kind == DH_DIAG_MHDMRI) {
+      double gam = P.gamma_law;
+      double Br = prim[BRR], Bp = prim[BPP], Bz = prim[BZZ];
+      double cs2 = gam * Pp / rho;
+      double B2 = Br * Br + Bp * Bp + Bz * Bz;
+      double sr = rho * vr;
+      double sp = r * r * rho * omega;
+      double sz = rho * vz;
+      double e = 0.5 * rho * (vr * vr + r * r * omega * omega + vz * vz) + Pp / (gam - 1.0) +
+                 0.5 * B2;
+      Q[0] = rho; Q[1] = Pp; Q[2] = vr; Q[3] = omega; Q[4] = vz;
+      Q[5] = Br; Q[6] = Bp; Q[7] = Bz;
+      Q[8] = rho; Q[9] = e; Q[10] = sr; Q[11] = sp; Q[12] = sz;
+      Q[13] = rho * vr * omega;
+      Q[14] = Br * Bp;
+      Q[15] = B2;
+      Q[16] = sqrt(cs2);
+      Q[17] = sqrt(B2 / rho);
+      return;
+   }
+   double Fr, Fp = 0.0, Fz;
+   if (p1.on) planetary_force(P, p1, r, phi, kind == DH_DIAG_DISK2D ? 0.0 : z, &Fr, &Fp, &Fz);
+   Q[0] = rho;
+   Q[1] = 2. * M_PI * r * rho * vr;
+   if (kind == DH_DIAG_DISK2D) {
+      Q[2] = 2. * M_PI * r * rho * (r * Fp);
+      Q[3] = 2. * M_PI * r * rho * (r * r * omega * vr);
+      Q[4] = omega;
+      return;
+   }
+   Q[2] = 2. * M_PI * r * rho * vz;
+   Q[3] = 2. * M_PI * r * rho * (r * Fp);
+   Q[4] = 2. * M_PI * r * rho * (r * r * omega * vr);
+   Q[5] = omega;
+   Q[6] = 2 * M_PI * r * rho * r * r * omega;
+   Q[7] = Pp;
+}
+
+// Position and volume of cell c as add_diagnostics forms them: the midpoint
+// of the cell's extents (not the centroid) and get_dV.
+DHD void diag_geometry(const Mesh &m, const double *piph, const double *dphi, long c, int j,
+                       int k, double *x, double *dV)
+{
+   double phip = piph[c];
+   double phim = phip - dphi[c];
+   double rp = m.r_jph[j], rm = m.r_jph[j - 1], zp = m.z_kph[k], zm = m.z_kph[k - 1];
+   x[0] = .5 * (rp + rm);
+   x[1] = .5 * (phip + phim);
+   x[2] = .5 * (zp + zm);
+   *dV = cell_dV(rp, rm, phip, phim, zp, zm, m.Pmax);
+}
+
+// One annulus, in the reference's order: Qrz[a*nq + q] += (sum_i Q_q dV) dt / sum_i dV.
+DHD void diag_annulus(const Mesh &m, const dh_params &P, int kind, DiagPlanet p1,
+                      const double *prim, const double *piph, const double *dphi, double dt,
+                      int a, double *Qrz)
+{
+   int nq = diag_count(kind);
+   size_t nt = (size_t)m.ntot;
+   int j = a % m.Nr, k = a / m.Nr;
+   double sum[DH_DIAG_MAXQ], vol = 0.0;
+   for (int q = 0; q < nq; ++q) sum[q] = 0.0;
+   for (int i = 0; i < m.Np[a]; ++i) {
+      long c = m.off[a] + i;
+      double x[3], dV, p[BZZ + 1], Q[DH_DIAG_MAXQ];
+      diag_geometry(m, piph, dphi, c, j, k, x, &dV);
+      for (int q = 0; q < diag_fields(kind); ++q) p[q] = prim[q * nt + c];
+      diag_cell(kind, P, p1, x[0], x[1], x[2], p, Q);
+      for (int q = 0; q < nq; ++q) sum[q] += Q[q] * dV;
+      vol += dV;
+   }
+   for (int q = 0; q < nq; ++q) Qrz[(size_t)a * nq + q] += sum[q] * dt / vol;
+}
+
+// avg_diagnostics (analysis.c:28-49): every accumulated value over t_avg.
+struct KDiagAvg {
+   double *Qrz;
+   double t_avg;
+   DHD void operator()(long i) const { Qrz[i] /= t_avg; }
 };
 
 // CFL time of one real cell (misc.c:39-84); HUGE_VAL for ghosts.

@@ -9,6 +9,7 @@
 //   ring_start(m, piph, I0)       start cell of every annulus
 //   flips(m, dim, F, flag, Phi, rkPhi)   hand-overs of flagged faces per track
 //   min_dt<NQ,H>(m, P, prim, piph, dphi, wiph, &dt)
+//   diag(m, P, kind, pl1, prim, piph, dphi, dt, Qrz)   per-annulus diagnostics
 //   alloc/put/get/copy/zero/sync  memory
 #pragma once
 #include <cstdarg>
@@ -62,6 +63,10 @@ struct Buffers {
    int *bcstart = nullptr;          // where each ghost cell's overlap sum starts
    int bc_maxnp[4][8] = {{0}};      // largest Np of each side's ghost layers
    PlanetSet pl[2];
+   // diagnostics (analysis.c:52-106): Qrz[nann * DH_DIAG_MAXQ], allocated on
+   // first use; the kind that accumulated it
+   double *diagQ = nullptr;
+   int diag_kind = 0;
 };
 
 // Checks a grid and parameter set against what the engine implements.
@@ -89,6 +94,7 @@ inline int validate(const dh_grid *g, const dh_params &P, long *ntot_out)
    if (P.rot_frame) return fail(DH_EINVAL, "RotFrame is not supported");
    if (P.nozzle_type) return fail(DH_EINVAL, "Nozzle_Type is not supported");
    if (P.r_periodic || P.z_periodic) return fail(DH_EINVAL, "periodic boundaries are not supported");
+   if (!(P.phi_order > 0.0)) return fail(DH_EINVAL, "phi_order must be positive (PHI_ORDER is 2)");
    if (g->Nr < 1 || g->Nz < 1) return fail(DH_EINVAL, "empty grid");
    if (!(g->phi_max > 0.0)) return fail(DH_EINVAL, "phi_max must be positive");
    int bcs[4] = {P.bc_rmin, P.bc_rmax, P.bc_zmin, P.bc_zmax};
@@ -531,6 +537,58 @@ int download_state(X &x, Buffers &b, double *prim, double *cons, double *piph, d
    if (wiph) DH_TRY(x.get(wiph, b.wiph, sizeof(double) * nt));
    if (Phi && b.nf) DH_TRY(x.get(Phi, b.Phi, sizeof(double) * nt * b.nf));
    return DH_OK;
+}
+
+// add_diagnostics (analysis.c:52-106) of a shipped module into the device
+// Qrz; a different kind than the one accumulated so far starts from zero.
+template <class X>
+int add_diagnostics(X &x, Buffers &b, int kind, double dt, const double *planets, int npl)
+{
+   if (!diag_count(kind)) return fail(DH_EINVAL, "unknown diagnostics kind %d", kind);
+   if (kind == DH_DIAG_MHDMRI && !b.mhd)
+      return fail(DH_EINVAL, "diag_mhdmri needs the MHD fields (NUM_C 8)");
+   if (npl < 0 || npl > DH_MAX_PLANETS) return fail(DH_EINVAL, "npl %d out of range", npl);
+   if (npl > 1 && !planets) return fail(DH_EINVAL, "null planet list");
+   size_t bytes = sizeof(double) * (size_t)b.m.nann * DH_DIAG_MAXQ;
+   if (!b.diagQ) {
+      DH_TRY(x.alloc((void **)&b.diagQ, bytes));
+      DH_TRY(x.zero(b.diagQ, bytes));
+   } else if (b.diag_kind != kind) {
+      DH_TRY(x.zero(b.diagQ, bytes));
+   }
+   b.diag_kind = kind;
+   DiagPlanet p1{};
+   p1.on = npl > 1;
+   if (p1.on)
+      for (int q = 0; q < DH_PLANET_STRIDE; ++q) p1.v[q] = planets[DH_PLANET_STRIDE + q];
+   return x.diag(b.m, b.P, kind, p1, b.prim, b.piph, b.dphi, dt, b.diagQ);
+}
+
+template <class X>
+int get_diagnostics(X &x, Buffers &b, double *Qrz)
+{
+   int nq = diag_count(b.diag_kind);
+   size_t n = (size_t)b.m.nann * nq;
+   if (!b.diagQ || !nq) {
+      for (size_t i = 0; i < n; ++i) Qrz[i] = 0.0;
+      return DH_OK;
+   }
+   return x.get(Qrz, b.diagQ, sizeof(double) * n);
+}
+
+template <class X>
+int avg_diagnostics(X &x, Buffers &b, double t_avg)
+{
+   if (!b.diagQ) return DH_OK;
+   long n = (long)b.m.nann * diag_count(b.diag_kind);
+   return x.run("diag_avg", KDiagAvg{b.diagQ, t_avg}, n);
+}
+
+template <class X>
+int zero_diagnostics(X &x, Buffers &b)
+{
+   if (!b.diagQ) return DH_OK;
+   return x.zero(b.diagQ, sizeof(double) * (size_t)b.m.nann * DH_DIAG_MAXQ);
 }
 
 // Planets seen by the sources of one RK substep.

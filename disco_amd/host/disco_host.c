@@ -14,7 +14,10 @@
  * The boundary module is named at compile time (-DBOUNDARY="polar", as the
  * reference's Makefile passes it) and mapped to a device boundary kind; sides
  * whose ghost values come from initial() are held on the device after the
- * host has checked that initial() does not depend on phi there.
+ * host has checked that initial() does not depend on phi there.  The
+ * Analysis module (-DANALYSIS="diag_euler") is likewise one of the device
+ * diagnostics kinds (diag_euler, diag_disk2d, diag_mhdmri), accumulated on
+ * the device every step; theTools.Qrz is its host mirror.
  */
 #include "disco_host.h"
 
@@ -24,6 +27,9 @@
 
 #ifndef BOUNDARY
 #define BOUNDARY "polar"
+#endif
+#ifndef ANALYSIS
+#define ANALYSIS "diag_euler"
 #endif
 
 /* Plugin symbols (reference plugin ABI). */
@@ -40,7 +46,21 @@ struct disco_accel {
    long *off;     /* [Nr*Nz+1] */
    int bc[4];     /* DH_BC_* per side */
    int host_dirty; /* mirror older than the device state */
+   int diag_kind;  /* DH_DIAG_* of the Analysis module */
 };
+
+static int diag_kind_of_module(const char *name)
+{
+   if (!strcmp(name, "diag_euler")) return DH_DIAG_EULER;
+   if (!strcmp(name, "diag_disk2d")) return DH_DIAG_DISK2D;
+   if (!strcmp(name, "diag_mhdmri")) return DH_DIAG_MHDMRI;
+   return -1;
+}
+
+static int diag_count_of_kind(int kind)
+{
+   return kind == DH_DIAG_EULER ? 8 : kind == DH_DIAG_DISK2D ? 5 : 18;
+}
 
 static char g_host_err[512];
 
@@ -343,6 +363,7 @@ static void fill_params(struct domain *d, dh_params *P, const int *bc)
    double ip[9] = {(double)L->initPar0, L->initPar1, L->initPar2, L->initPar3, L->initPar4,
                    L->initPar5, L->initPar6, L->initPar7, L->initPar8};
    memcpy(P->init_par, ip, sizeof ip);
+   P->phi_order = 2.0; /* PHI_ORDER, planet.c:4 */
 }
 
 static void planets_pack(const struct domain *d, double *out)
@@ -439,6 +460,15 @@ int disco_setup(struct domain *d)
    d->thePlanets = (struct planet *)calloc(d->Npl > 0 ? d->Npl : 1, sizeof(struct planet));
    initializePlanets(d->thePlanets);
    if (d->Npl > DH_MAX_PLANETS) return host_fail("at most %d planets", DH_MAX_PLANETS);
+   /* diagnostics (domain.c:46-51) */
+   A->diag_kind = diag_kind_of_module(ANALYSIS);
+   if (A->diag_kind < 0)
+      return host_fail("analysis module '%s' has no device implementation", ANALYSIS);
+   if (A->diag_kind == DH_DIAG_MHDMRI && NUM_C < 8)
+      return host_fail("analysis module '%s' needs the MHD fields", ANALYSIS);
+   d->num_tools = diag_count_of_kind(A->diag_kind);
+   d->theTools.Qrz = (double *)calloc((size_t)Nr * Nz * d->num_tools, sizeof(double));
+   d->theTools.t_avg = 0.0;
    double Pmax = d->phi_max;
    /* One rand() per annulus of the global mesh, ghost layers included, in
     * (k, j) order; annuli this mesh does not hold still consume theirs
@@ -572,7 +602,57 @@ int disco_sync_to_host(struct domain *d)
       A->host_dirty = 0;
    }
    free(prim); free(cons); free(piph); free(dphi); free(wiph);
-   return rc ? host_fail("dh_download_state: %s", dh_last_error()) : 0;
+   if (rc) return host_fail("dh_download_state: %s", dh_last_error());
+   if (d->theTools.Qrz && dh_get_diagnostics(A->ctx, d->theTools.Qrz))
+      return host_fail("dh_get_diagnostics: %s", dh_last_error());
+   return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Diagnostics with the signatures of analysis.c:8-106; the sums live on the */
+/* device and theTools.Qrz mirrors them after disco_sync_to_host or         */
+/* avg_diagnostics.                                                          */
+/* ------------------------------------------------------------------------ */
+
+void zero_diagnostics(struct domain *d)
+{
+   struct disco_accel *A = (struct disco_accel *)d->accel;
+   if (dh_zero_diagnostics(A->ctx)) {
+      host_fail("dh_zero_diagnostics: %s", dh_last_error());
+      exit(1);
+   }
+   memset(d->theTools.Qrz, 0, sizeof(double) * (size_t)d->Nr * d->Nz * d->num_tools);
+   d->theTools.t_avg = 0.0;
+}
+
+void avg_diagnostics(struct domain *d)
+{
+   struct disco_accel *A = (struct disco_accel *)d->accel;
+   if (dh_avg_diagnostics(A->ctx, d->theTools.t_avg) ||
+       dh_get_diagnostics(A->ctx, d->theTools.Qrz)) {
+      host_fail("avg_diagnostics: %s", dh_last_error());
+      exit(1);
+   }
+   d->theTools.t_avg = 0.0;
+}
+
+int disco_get_diagnostics(struct domain *d, double *Qrz)
+{
+   struct disco_accel *A = (struct disco_accel *)d->accel;
+   return dh_get_diagnostics(A->ctx, Qrz) ? host_fail("dh_get_diagnostics: %s", dh_last_error())
+                                          : 0;
+}
+
+void add_diagnostics(struct domain *d, double dt)
+{
+   struct disco_accel *A = (struct disco_accel *)d->accel;
+   double pl[DH_MAX_PLANETS * DH_PLANET_STRIDE];
+   planets_pack(d, pl);
+   if (dh_add_diagnostics(A->ctx, A->diag_kind, dt, pl, d->Npl)) {
+      host_fail("dh_add_diagnostics: %s", dh_last_error());
+      exit(1);
+   }
+   d->theTools.t_avg += dt;
 }
 
 int disco_export_state(struct domain *d, double *prim, double *cons, double *piph,
@@ -663,6 +743,7 @@ void timestep(struct domain *d, double dt)
       planets_clean(d);
    }
    A->host_dirty = 1;
+   add_diagnostics(d, dt);
    d->count_steps += 1;
 }
 
@@ -697,6 +778,7 @@ void disco_free(struct domain *d)
    if (d->r_jph) free(d->r_jph - 1);
    if (d->z_kph) free(d->z_kph - 1);
    free(d->thePlanets);
+   free(d->theTools.Qrz);
    memset(d, 0, sizeof *d);
 }
 

@@ -6,6 +6,8 @@
  *   double getmindt(struct domain *)            misc.c:39-84
  *   void   timestep(struct domain *, double)    timestep.c:7-45
  *   void   check_dt(struct domain *, double *)  domain.c:284-311
+ *   void   add_diagnostics / avg_diagnostics / zero_diagnostics
+ *                                               analysis.c:8-106
  * Set-up (readpar.c:44-188, gridsetup.c:15-144, domain.c:26-242):
  *   disco_read_par, disco_setup_grid, disco_setup
  * Device state <-> host mirror: disco_sync_to_host, disco_export_state.
@@ -37,6 +39,11 @@ int disco_write_dstate(struct domain *d, const char *path);
 double getmindt(struct domain *d);
 void timestep(struct domain *d, double dt);
 void check_dt(struct domain *d, double *dt);
+void add_diagnostics(struct domain *d, double dt);
+void avg_diagnostics(struct domain *d);
+void zero_diagnostics(struct domain *d);
+/* theTools.Qrz after the last step (disco_sync_to_host refreshes it). */
+int disco_get_diagnostics(struct domain *d, double *Qrz);
 
 #ifdef __cplusplus
 }

@@ -3,7 +3,9 @@
  * reference, serial): reads ./in.par, sets the mesh up, then loops
  *     dt = getmindt(); check_dt(&dt); timestep(dt);
  * until T_End or an `abort` file, and writes the final state to
- * output.dstate (DSTATE01 layout, disco_amd/state_io.py).
+ * output.dstate (DSTATE01 layout, disco_amd/state_io.py) and the Analysis
+ * module's accumulated sums to output.diag (int32 nq, int32 Nr*Nz, double
+ * t_avg, double Qrz[Nr*Nz*nq], as the reference keeps theTools).
  *
  * Usage: disco [max_steps]
  */
@@ -43,6 +45,14 @@ int main(int argc, char **argv)
           ncells, D.count_steps, D.t, secs,
           secs > 0 ? (double)ncells * D.count_steps / secs : 0.0);
    if (disco_write_dstate(&D, "output.dstate")) return 1;
+   if (disco_sync_to_host(&D)) return 1;
+   FILE *fq = fopen("output.diag", "wb");
+   if (!fq) return 1;
+   int qhdr[2] = {D.num_tools, D.Nr * D.Nz};
+   fwrite(qhdr, sizeof(int), 2, fq);
+   fwrite(&D.theTools.t_avg, sizeof(double), 1, fq);
+   fwrite(D.theTools.Qrz, sizeof(double), (size_t)qhdr[0] * qhdr[1], fq);
+   fclose(fq);
    disco_free(&D);
    return 0;
 }

@@ -120,6 +120,16 @@ def read_state(path: str) -> MeshState:
                      step=step, num_c=num_c)
 
 
+def read_diag(path: str):
+    """(t_avg, Qrz[Nr*Nz, nq]) of a diag.bin: int32 nq, int32 Nr*Nz, double
+    t_avg, double Qrz (the Analysis module's sums, analysis.c:52-106)."""
+    with open(path, "rb") as f:
+        nq, nann = np.frombuffer(f.read(8), dtype=np.int32)
+        t_avg = float(np.frombuffer(f.read(8), dtype=np.float64)[0])
+        q = np.frombuffer(f.read(8 * int(nq) * int(nann)), dtype=np.float64)
+    return t_avg, q.reshape(int(nann), int(nq)).copy()
+
+
 def write_state(path: str, s: MeshState) -> None:
     hdr = np.zeros(16, dtype=np.int32)
     hdr[:12] = [s.nq, s.nf, s.nr, s.nz, *s.ng, s.ntot, s.planets.shape[0], s.step, s.num_c]

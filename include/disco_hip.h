@@ -101,6 +101,9 @@ typedef struct dh_params {
    double metric_par2;          /* Metric_Par2 (M in the Cs2 profiles) */
    double rot_omega, rot_d;     /* RotOmega, RotD */
    double init_par[9];          /* Init_Par0..8 (Init_Par0 as a double) */
+   double phi_order;            /* PHI_ORDER of the softened point-mass force,
+                                   2.0 (planet.c:4); a run-time value as there,
+                                   so pow() is evaluated as the reference does */
 } dh_params;
 
 /* Planet record passed to dh_set_planets: 8 doubles per planet,
@@ -159,6 +162,26 @@ int dh_timestep(dh_ctx *ctx, double dt);
 
 /* Waits for all work queued on the context's stream. */
 int dh_sync(dh_ctx *ctx);
+
+/* Per-annulus diagnostics (the Analysis module): every step the reference
+ * adds, for each annulus jk and quantity q, dt times the volume-weighted
+ * mean of get_diagnostics(x, prim) over the annulus's cells to
+ * Qrz[jk*nq + q] (analysis.c:52-106, called at timestep.c:42).  The device
+ * accumulates into its own Qrz for the shipped modules below; planets are
+ * the state after the step (the modules read planet 1, when Npl > 1). */
+enum {
+   DH_DIAG_EULER = 1,   /* Diagnostics/diag_euler.c,   8 quantities */
+   DH_DIAG_DISK2D = 2,  /* Diagnostics/diag_disk2d.c,  5 quantities */
+   DH_DIAG_MHDMRI = 3   /* Diagnostics/diag_mhdmri.c, 18 quantities */
+};
+#define DH_DIAG_MAXQ 18
+int dh_add_diagnostics(dh_ctx *ctx, int kind, double dt, const double *planets, int npl);
+/* Copies the accumulated Qrz[Nr*Nz*nq] out (nq of the kind last added). */
+int dh_get_diagnostics(dh_ctx *ctx, double *Qrz);
+/* Zeroes the accumulated Qrz (zero_diagnostics, analysis.c:8-26). */
+int dh_zero_diagnostics(dh_ctx *ctx);
+/* Divides the accumulated Qrz by t_avg (avg_diagnostics, analysis.c:28-49). */
+int dh_avg_diagnostics(dh_ctx *ctx, double t_avg);
 
 /* Average device time (ms) of the kernels of the last `dh_timestep` calls,
  * accumulated per kernel family since dh_reset_timers; used by the benchmark

@@ -952,6 +952,7 @@ or_ctx *or_create(const dh_grid *g, const dh_params *par)
    o->nq = par->num_c + par->num_n;
    o->nf = par->ct_mode == 1 ? 3 : (par->ct_mode == 2 ? 5 : 0);
    if (o->nq > MAXQ) { fprintf(stderr, "oracle: NQ too large\n"); abort(); }
+   if (!(par->phi_order > 0.0)) { fprintf(stderr, "oracle: phi_order must be 2\n"); abort(); }
    if (par->hydro == DH_HYDRO_MHD && (par->ct_mode != 2 || par->riemann_solver != R_HLLD)) {
       fprintf(stderr, "oracle: MHD is restated with CT_MODE 2 and HLLD only\n");
       abort();
@@ -1567,10 +1568,9 @@ static void trans_flux(or_ctx *o, double dt, int dim)
 }
 
 /* fgrav, planet.c:56-76 */
-static double fgrav(double M, double r, double eps, int type)
+static double fgrav(double M, double r, double eps, int type, double n)
 {
    if (type == 0) {
-      double n = 2.0;
       return M * pow(r, n - 1.) / pow(pow(r, n) + pow(eps, n), 1. + 1. / n);
    } else if (type == 1) {
       return M / ((r - 2 * M) * (r - 2 * M));
@@ -1593,7 +1593,7 @@ static void planetary_force(const or_ctx *o, const double *pl, double r, double 
    double dy = r * sinp - rp * sin(pp);
    double script_r = sqrt(dx * dx + dy * dy + z * z);
    double script_r_perp = sqrt(dx * dx + dy * dy);
-   double f1 = -fgrav(pl[2], script_r, pl[5], (int)pl[6]);
+   double f1 = -fgrav(pl[2], script_r, pl[5], (int)pl[6], o->p.phi_order);
    double cosa = dx / script_r_perp;
    double sina = dy / script_r_perp;
    double cosap = cosa * cosp + sina * sinp;

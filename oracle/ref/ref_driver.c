@@ -13,7 +13,9 @@
  *   -> dump state_0000.bin
  *   -> N x { dt = getmindt ; timestep(dt) }   (main.c:69-82; check_dt is
  *      skipped: T_End is never reached in these runs)
- *   -> dump state_final.bin, dt.txt
+ *   -> dump state_final.bin, dt.txt, diag.bin (the Analysis module's
+ *      accumulated Qrz: int32 nq, int32 Nr*Nz, double t_avg,
+ *      double Qrz[Nr*Nz*nq], analysis.c:52-106)
  *
  * Usage: disco_ref_<cfg> <outdir> <nsteps> [dump_every]
  *   in.par is read from the current directory (readpar.c:60).
@@ -152,6 +154,15 @@ int main(int argc, char *argv[])
 
    snprintf(path, sizeof path, "%s/state_final.bin", outdir);
    dump_state(&theDomain, path, nsteps);
+
+   snprintf(path, sizeof path, "%s/diag.bin", outdir);
+   FILE *fq = fopen(path, "wb");
+   if (!fq) { perror(path); return 2; }
+   int qhdr[2] = {theDomain.num_tools, theDomain.Nr * theDomain.Nz};
+   fwrite(qhdr, sizeof(int), 2, fq);
+   fwrite(&theDomain.theTools.t_avg, sizeof(double), 1, fq);
+   fwrite(theDomain.theTools.Qrz, sizeof(double), (size_t)qhdr[0] * qhdr[1], fq);
+   fclose(fq);
 
    /* Real cells, as profiler.c:12-37 counts them. */
    long ncells = 0;

@@ -47,12 +47,13 @@ def build_driver(config: str) -> str:
     srcs = [os.path.join(host, "main.c"), os.path.join(host, "disco_host.c"),
             os.path.join(plug, "Initial", c["initial"] + ".c"),
             os.path.join(plug, "Planets", c["planets"] + ".c")]
-    if os.path.exists(out) and all(os.path.getmtime(out) >= os.path.getmtime(s)
-                                   for s in srcs + [lib]):
+    deps = srcs + [lib, os.path.join(host, "disco_host.h"), os.path.join(REPO, "disco_amd", "build.py")]
+    if os.path.exists(out) and all(os.path.getmtime(out) >= os.path.getmtime(s) for s in deps):
         return out
     cmd = ["gcc", "-O2", "-Wall", "-Wno-unused-parameter", f"-DNUM_C={c['num_c']}",
            f"-DNUM_N={c['num_n']}", f"-DCT_MODE={c['ct_mode']}", f'-DBOUNDARY="{c["boundary"]}"',
-           f'-DINITIAL="{c["initial"]}"', f"-I{os.path.join(REPO, 'include', 'disco')}",
+           f'-DINITIAL="{c["initial"]}"', f'-DANALYSIS="{c["analysis"]}"',
+           f"-I{os.path.join(REPO, 'include', 'disco')}",
            f"-I{os.path.join(REPO, 'include')}", f"-I{host}", *srcs, "-o", out + ".tmp",
            f"-L{os.path.dirname(lib)}", "-ldh_cpu", "-lm",
            f"-Wl,-rpath,{os.path.dirname(lib)}"]

@@ -70,6 +70,13 @@ struct HostExec {
       }
       return DH_OK;
    }
+   int diag(const Mesh &m, const dh_params &P, int kind, const DiagPlanet &p1,
+            const double *prim, const double *piph, const double *dphi, double dt, double *Qrz)
+   {
+      for (int a = 0; a < m.nann; ++a)
+         diag_annulus(m, P, kind, p1, prim, piph, dphi, dt, a, Qrz);
+      return DH_OK;
+   }
    template <int NQ, class H>
    int min_dt(const Mesh &m, const dh_params &P, const double *prim, const double *piph,
               const double *dphi, const double *wiph, double *out)
@@ -284,6 +291,30 @@ int dh_timestep(dh_ctx *c, double dt)
 }
 
 int dh_sync(dh_ctx *c) { return c ? DH_OK : fail(DH_EINVAL, "null context"); }
+
+int dh_add_diagnostics(dh_ctx *c, int kind, double dt, const double *planets, int npl)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return add_diagnostics(c->x, c->b, kind, dt, planets, npl);
+}
+
+int dh_get_diagnostics(dh_ctx *c, double *Qrz)
+{
+   if (!c || !Qrz) return fail(DH_EINVAL, "null argument");
+   return get_diagnostics(c->x, c->b, Qrz);
+}
+
+int dh_zero_diagnostics(dh_ctx *c)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return zero_diagnostics(c->x, c->b);
+}
+
+int dh_avg_diagnostics(dh_ctx *c, double t_avg)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return avg_diagnostics(c->x, c->b, t_avg);
+}
 
 int dh_enable_timers(dh_ctx *c, int) { return c ? DH_OK : fail(DH_EINVAL, "null context"); }
 

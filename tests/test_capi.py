@@ -3,13 +3,15 @@ point that include/disco_hip.h declares (no compute calls are made here)."""
 import ctypes
 import os
 import re
+import subprocess
+import tempfile
 
 import pytest
 
 from disco_amd import capi
 
-HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
-                      "include", "disco_hip.h")
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "disco_hip.h")
 
 
 def declared_symbols():
@@ -31,8 +33,27 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
-def test_param_struct_layout_matches_header():
-    # dh_params: 30 ints (27 named + 3 padding) then 11 + 9 doubles.
-    assert ctypes.sizeof(capi.DhParams) == 30 * 4 + 20 * 8
-    assert capi.DhParams.gamma_law.offset == 120
-    assert ctypes.sizeof(capi.DhGrid) == 6 * 4 + 3 * 8 + 8
+def _c_layout(struct, fields):
+    """sizeof and offsetof of the header's struct, as gcc lays it out."""
+    src = ["#include <stddef.h>", "#include <stdio.h>", '#include "disco_hip.h"',
+           "int main(void) {", f'printf("%zu\\n", sizeof({struct}));']
+    src += [f'printf("%zu\\n", offsetof({struct}, {f}));' for f in fields]
+    src += ["return 0; }"]
+    d = tempfile.mkdtemp(prefix="layout_")
+    c, exe = os.path.join(d, "l.c"), os.path.join(d, "l")
+    with open(c, "w") as f:
+        f.write("\n".join(src))
+    subprocess.run(["gcc", f"-I{os.path.join(REPO, 'include')}", c, "-o", exe], check=True)
+    out = [int(v) for v in subprocess.run([exe], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    return out[0], out[1:]
+
+
+@pytest.mark.parametrize("pystruct,cstruct", [(capi.DhParams, "dh_params"),
+                                              (capi.DhGrid, "dh_grid")])
+def test_struct_layout_matches_header(pystruct, cstruct):
+    fields = [f for f, _ in pystruct._fields_ if not f.endswith("_")]
+    size, offs = _c_layout(cstruct, fields)
+    assert ctypes.sizeof(pystruct) == size
+    for f, off in zip(fields, offs):
+        assert getattr(pystruct, f).offset == off, f

@@ -3,7 +3,9 @@
 the Initial/Planets plugins, here linked to the host build of the engine --
 gives the same run bit for bit: mesh set-up with the reference's rand()
 sequence, initial data and noise, the initial magnetic fluxes
-(set_B_fields), CFL steps and the timesteps themselves.  The GPU version of
+(set_B_fields), CFL steps, the timesteps themselves and the Analysis
+module's per-annulus sums (diag_euler with and without a companion planet,
+diag_disk2d, diag_mhdmri).  The GPU version of
 this test (test_gpu_dropin.py) runs the driver on libdisco_hip.so."""
 import os
 import subprocess
@@ -24,6 +26,7 @@ import build as cpu_build  # noqa: E402
     ("euler_yee_n1", {"Num_R": 16, "Riemann_Solver": 0}, 4),
     ("mhd_kep", {"Num_R": 8, "Num_Z": 4}, 5),
     ("mhd_kep", {"Num_R": 12, "Num_Z": 6, "Max_DT": 0.02}, 6),
+    ("mhd_kep_p2", {"Num_R": 12, "Num_Z": 4, "Max_DT": 0.02}, 4),
 ])
 def test_driver_bit_exact_vs_reference(cname, over, steps):
     cfg = configs.get(cname)
@@ -47,3 +50,9 @@ def test_driver_bit_exact_vs_reference(cname, over, steps):
     assert np.array_equal(got.piph, want.piph)
     if want.nf:
         assert np.array_equal(got.phi_flux, want.phi_flux)
+    t_want, q_want = state_io.read_diag(os.path.join(d, "diag.bin"))
+    t_got, q_got = state_io.read_diag(os.path.join(d, "output.diag"))
+    assert q_got.shape == q_want.shape == (got.nr * got.nz, configs.DIAG_KINDS[cfg.analysis][1])
+    assert t_got == t_want and np.array_equal(q_got, q_want)
+    if len(want.planets) > 1:
+        assert np.any(q_want[:, 3] != 0.0)  # the torque column reads planet 1

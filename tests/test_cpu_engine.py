@@ -138,3 +138,23 @@ def test_galloping_search_equals_bisection(cpu_lib):
             truth = bytes([1] * (ans + 1) + [0] * (n - ans))
             for g in range(-2, n + 3):
                 assert cpu_lib.dhx_last_true(n, g, truth) == ans, (n, ans, g)
+
+
+def test_diagnostics_accumulate_average_zero(cpu_lib):
+    """add_diagnostics adds dt-weighted annulus means, avg_diagnostics divides
+    by the accumulated time, zero_diagnostics clears (analysis.c:8-106)."""
+    cfg, par, a = _ref_initial("mhd_kep", {"Num_R": 8, "Num_Z": 4})
+    g = capi.DiscoHip(cfg, par, a, library=cpu_lib)
+    g.add_diagnostics("diag_mhdmri", 0.25, a.planets)
+    one = g.diagnostics()
+    g.add_diagnostics("diag_mhdmri", 0.75, a.planets)
+    two = g.diagnostics()
+    np.testing.assert_allclose(two, 4.0 * one, rtol=1e-14, atol=0)
+    g.avg_diagnostics(1.0)
+    np.testing.assert_allclose(g.diagnostics(), two, rtol=1e-15, atol=0)
+    # column 0 is the annulus mean density; column 15 the mean B^2
+    m = a.real_mask()
+    assert np.all(one[:, 0] > 0) and np.all(one[:, 15] >= 0)
+    g.zero_diagnostics()
+    assert not g.diagnostics().any()
+    g.close()

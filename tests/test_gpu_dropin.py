@@ -3,7 +3,11 @@ and to this project's `disco` driver (host C layer + libdisco_hip.so, with the
 same Initial/Planets plugins) gives the same run.
 
 Both sides set the mesh up themselves (random ring offsets from srand(314159),
-initial() on every cell), take their own CFL steps and stop after N steps."""
+initial() on every cell), take their own CFL steps and stop after N steps.
+The Analysis module's per-annulus sums are compared as well: each column
+against its own L1 norm, floored at 1e-6 of the largest column's, since
+columns such as the radial mass flux are small differences of the disk's
+motion."""
 import os
 import subprocess
 import tempfile
@@ -23,6 +27,7 @@ HOST = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
     ("euler_yee", {}, 20),
     ("euler_yee_n1", {"Num_R": 64, "Riemann_Solver": 0}, 10),
     ("mhd_kep", {"Num_R": 32, "Num_Z": 8}, 6),
+    ("mhd_kep_p2", {"Num_R": 32, "Num_Z": 8, "Max_DT": 0.01}, 6),
 ])
 def test_disco_driver_matches_reference(cname, over, steps):
     cfg = configs.get(cname)
@@ -51,3 +56,11 @@ def test_disco_driver_matches_reference(cname, over, steps):
         ephi = rel_l1(got.phi_flux[:, m], want.phi_flux[:, m])
         print(f"  magnetic fluxes rel L1 {ephi:.2e}", flush=True)
         assert ephi <= PARITY_TOL
+    t_want, q_want = state_io.read_diag(os.path.join(d, "diag.bin"))
+    t_got, q_got = state_io.read_diag(os.path.join(d, "output.diag"))
+    assert abs(t_got - t_want) <= 1e-12 * t_want and q_got.shape == q_want.shape
+    scale = np.abs(q_want).sum(axis=0)
+    scale = np.maximum(scale, 1e-6 * scale.max())
+    err = np.abs(q_got - q_want).sum(axis=0) / scale
+    print(f"  diagnostics ({cfg.analysis}) worst column {err.max():.2e}", flush=True)
+    assert err.max() <= PARITY_TOL
