@@ -48,17 +48,20 @@ def build_hip(extra=()) -> str:
 # library and one `disco` driver per selection, as the reference links one
 # binary per Makefile_opt.in.
 HOST_CONFIGS = {
-    "euler_yee": dict(initial="yee_vortex", planets="none", boundary="polar",
+    "euler_yee": dict(hydro="euler", initial="yee_vortex", planets="none", boundary="polar",
                       analysis="diag_euler", num_c=5, num_n=0, ct_mode=0),
-    "euler_yee_n1": dict(initial="yee_vortex", planets="none", boundary="polar",
+    "euler_yee_n1": dict(hydro="euler", initial="yee_vortex", planets="none", boundary="polar",
                          analysis="diag_disk2d", num_c=5, num_n=1, ct_mode=0),
-    "mhd_kep": dict(initial="kep_mhd_disk", planets="central", boundary="fixed_r_outflow_z",
+    "mhd_kep": dict(hydro="mhd", initial="kep_mhd_disk", planets="central", boundary="fixed_r_outflow_z",
                     analysis="diag_mhdmri", num_c=8, num_n=0, ct_mode=2),
     # the MHD disk with a companion (Mass_Ratio > 0) and the Euler diagnostics,
     # which read planet 1
-    "mhd_kep_p2": dict(initial="kep_mhd_disk", planets="central",
+    "mhd_kep_p2": dict(hydro="mhd", initial="kep_mhd_disk", planets="central",
                        boundary="fixed_r_outflow_z", analysis="diag_euler",
                        num_c=8, num_n=0, ct_mode=2),
+    "gr_disk": dict(hydro="grmhd2", initial="gr_disk", planets="none",
+                    boundary="fixed_r_outflow_z", analysis="diag_euler",
+                    num_c=8, num_n=0, ct_mode=2),
 }
 
 
@@ -72,7 +75,7 @@ def build_host(configs=None) -> None:
             continue
         defs = [f"-DNUM_C={c['num_c']}", f"-DNUM_N={c['num_n']}", f"-DCT_MODE={c['ct_mode']}",
                 f'-DBOUNDARY="{c["boundary"]}"', f'-DINITIAL="{c["initial"]}"',
-                f'-DANALYSIS="{c["analysis"]}"']
+                f'-DANALYSIS="{c["analysis"]}"', f'-DHYDRO="{c["hydro"]}"']
         srcs = [os.path.join(host, "disco_host.c"),
                 os.path.join(plug, "Initial", c["initial"] + ".c"),
                 os.path.join(plug, "Planets", c["planets"] + ".c")]

@@ -73,7 +73,8 @@ def make_params(cfg: "cfgmod.Config", par: Dict[str, object]) -> DhParams:
     full.update(par)
     p = DhParams()
     p.num_c, p.num_n, p.ct_mode = cfg.num_c, cfg.num_n, cfg.ct_mode
-    p.hydro = cfgmod.HYDRO_MHD if cfg.hydro == "mhd" else cfgmod.HYDRO_EULER
+    p.hydro = {"euler": cfgmod.HYDRO_EULER, "mhd": cfgmod.HYDRO_MHD,
+               "grmhd2": cfgmod.HYDRO_GRMHD}[cfg.hydro]
     for k, f in _PAR_INT_KEYS.items():
         # readpar.c:31-36 parses integers through %lf and a cast.
         setattr(p, f, int(float(full[k])))

@@ -12,13 +12,13 @@ from typing import Dict, Optional, Tuple
 
 # DH_BC_* (include/disco_hip.h)
 BC_NONE, BC_HELD, BC_ZEROGRAD, BC_DIODE = 0, 1, 2, 3
-HYDRO_EULER, HYDRO_MHD = 0, 1
+HYDRO_EULER, HYDRO_MHD, HYDRO_GRMHD = 0, 1, 2
 
 
 @dataclasses.dataclass(frozen=True)
 class Config:
     name: str
-    hydro: str                 # "euler" | "mhd"
+    hydro: str                 # "euler" | "mhd" | "grmhd2"
     initial: str               # plugin name (disco_amd/plugins/Initial or reference Initial/)
     boundary: str              # "polar" | "fixed" | "outflow3d" | "fixed_r_outflow_z"
     planets: str               # "none" | "sun" | ...
@@ -106,6 +106,18 @@ CONFIGS["mhd_kep"] = _cfg(
     Metric_Par2=1.0, Grav2D=1, Adiabatic_Index=5.0 / 3.0, Init_Par1=1000.0,
     Noise_Type=3, Noise_Rel=5e-5, Noise_Abs=0.0)
 
+
+# GRMHD: a magnetised gas disk on the equator of a Schwarzschild black hole
+# (M = 1) in Kerr-Schild coordinates, r in [4, 12], z in [-0.5, 0.5], HLLC,
+# PLM 1.5, gamma = 4/3, constrained transport, Keplerian mesh motion, the
+# Noble et al. cons2prim (Hydro/grmhd2.c, Metric/schw_ks_eq_rpz.c).
+CONFIGS["gr_disk"] = _cfg(
+    "gr_disk", "grmhd2", "gr_disk", "fixed_r_outflow_z", "none", 8, 0, 2,
+    Num_R=64, Num_Z=16, R_Min=4.0, R_Max=12.0, Z_Min=-0.5, Z_Max=0.5,
+    NoBC_Rmin=0, NoBC_Rmax=0, NoBC_Zmin=0, NoBC_Zmax=0, CFL=0.5, PLM=1.5,
+    Riemann_Solver=1, Mesh_Motion=4, Exact_Mesh_Omega=2, Energy_Omega=0,
+    Constrained_Transport=1, Isothermal=0, Metric_Par2=1.0, Adiabatic_Index=4.0 / 3.0,
+    Init_Par1=100.0, Density_Floor=1e-8, Pressure_Floor=1e-10)
 
 # The disk with a companion of mass ratio 1e-3 at r = 1.25 (Planets/central.c
 # reads Mass_Ratio) and the Euler diagnostics, which read planet 1.

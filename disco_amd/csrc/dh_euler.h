@@ -258,7 +258,7 @@ struct Euler {
    }
 
    // get_omega (euler.c:32-34): the cell's own angular velocity.
-   static DHD double omega(const double *prim) { return prim[UPP]; }
+   static DHD double omega(const dh_params &, const double *prim, double) { return prim[UPP]; }
 
    // solve_riemann (riemann.c:216-318), HLL/HLLC branch: the interface flux
    // F and the state U* it is associated with (the caller forms F - w U*).

@@ -1,0 +1,742 @@
+// dh_grmhd.h -- general-relativistic MHD on the moving cylindrical mesh
+// (device side of the reference Hydro/grmhd2.c) in the Schwarzschild metric
+// in Kerr-Schild coordinates evaluated on the equator
+// (Hydro/Metric/schw_ks_eq_rpz.c) with the Eulerian observer frame
+// (Hydro/Frame/normal.c).
+//
+// Primitive layout {rho, P, u_r, u_phi, u_z, B^r, B^phi, B^z, passives...}
+// with covariant four-velocity components u_i and the densitised field;
+// conserved {D, tau, S_r, S_phi, S_z, B^r dV / r, B^phi dV, B^z dV, ...}
+// (grmhd2.c:88-180).  Every metric quantity of this metric depends on the
+// cylindrical radius only, so a module function needs r and nothing else of
+// the position.
+//
+// Riemann solvers: HLL and HLLC (riemann.c:216-292 with grmhd2.c's vel,
+// flux, prim2cons and getUstar); cons2prim: the 2-D Newton scheme of Noble
+// et al. (grmhd2.c:1096-1314).  The expressions keep the reference's
+// association so that the host build reproduces the reference bit for bit.
+#pragma once
+#include "dh_geom.h"
+
+namespace dh {
+
+// Metric (schw_ks_eq_rpz.c:19-189) and frame (normal.c:9-42) at radius r,
+// with M = Metric_Par2.  gam / igam are the spatial metric and its inverse,
+// be the shift, al the lapse, U the frame four-velocity.
+struct GRMetric {
+   double al, be[3], gam[9], igam[9], jac, sqrtgam, U[4];
+};
+
+DHD double gr_lapse(double M, double r) { return 1.0 / sqrt(1.0 + 2 * M / r); }
+
+DHD void gr_shift(double M, double r, double *b)
+{
+   double z = 0.0;
+   double R = sqrt(r * r + z * z);
+   double a2 = 1.0 / (1.0 + 2 * M / R);
+   b[0] = 2 * M * r / (R * R) * a2;
+   b[1] = 0.0;
+   b[2] = 2 * M * z / (R * R) * a2;
+}
+
+DHD void gr_gam(double M, double r, double *g)
+{
+   double z = 0.0;
+   double R = sqrt(r * r + z * z);
+   double H = 2 * M / R;
+   double st = r / R, ct = z / R;
+   g[0] = 1.0 + H * st * st;
+   g[1] = 0.0;
+   g[2] = H * st * ct;
+   g[3] = 0.0;
+   g[4] = r * r;
+   g[5] = 0.0;
+   g[6] = H * st * ct;
+   g[7] = 0.0;
+   g[8] = 1.0 + H * ct * ct;
+}
+
+DHD void gr_igam(double M, double r, double *g)
+{
+   double z = 0.0;
+   double R = sqrt(r * r + z * z);
+   double H = 2 * M / R;
+   double st = r / R, ct = z / R;
+   double a2 = 1.0 / (1.0 + H);
+   g[0] = (1.0 + H * ct * ct) * a2;
+   g[1] = 0.0;
+   g[2] = -H * st * ct * a2;
+   g[3] = 0.0;
+   g[4] = 1.0 / (r * r);
+   g[5] = 0.0;
+   g[6] = -H * st * ct * a2;
+   g[7] = 0.0;
+   g[8] = (1.0 + H * st * st) * a2;
+}
+
+// d g_{mu nu} / d x^i for i = 1 (r) and 3 (z); zero otherwise.
+DHD void gr_der_g(double M, double r, int i, double *dg)
+{
+   for (int mu = 0; mu < 16; ++mu) dg[mu] = 0.0;
+   if (i != 1 && i != 3) return;
+   double z = 0.0;
+   double R = sqrt(r * r + z * z);
+   double st = r / R, ct = z / R;
+   double MoR2 = M / (R * R), MoR3 = M / (R * R * R), MoR4 = M / (R * R * R * R);
+   if (i == 1) {
+      dg[0] = -2 * MoR2 * st;
+      dg[1] = 2 * MoR2 * (1.0 - 2 * st * st);
+      dg[3] = -4 * MoR3 * z * st;
+      dg[5] = 2 * MoR3 * r * (2.0 - 3.0 * st * st);
+      dg[7] = 2 * MoR3 * z * (1.0 - 3.0 * st * st);
+      dg[10] = 2 * r;
+      dg[15] = -6 * MoR4 * z * z * st;
+   } else {
+      dg[0] = -2 * MoR2 * ct;
+      dg[1] = -4 * MoR3 * r * ct;
+      dg[3] = 2 * MoR2 * (1.0 - 2 * ct * ct);
+      dg[5] = -6 * MoR4 * r * r * ct;
+      dg[7] = 2 * MoR3 * r * (1.0 - 3.0 * ct * ct);
+      dg[15] = 2 * MoR3 * z * (2.0 - 3.0 * ct * ct);
+   }
+   dg[4] = dg[1];
+   dg[12] = dg[3];
+   dg[13] = dg[7];
+}
+
+// Killing directions of the metric: t and phi (x^0, x^2).
+DHD bool gr_killing(int mu) { return !(mu == 1 || mu == 3); }
+
+DHD void gr_der_lapse(double M, double r, double *da)
+{
+   double z = 0.0;
+   double R = sqrt(r * r + z * z);
+   double st = r / R, ct = z / R;
+   double MoR2 = M / (R * R);
+   double a = 1.0 / sqrt(1.0 + 2 * M / R);
+   da[0] = 0.0;
+   da[1] = a * a * a * MoR2 * st;
+   da[2] = 0.0;
+   da[3] = a * a * a * MoR2 * ct;
+}
+
+DHD void gr_der_shift(double M, double r, double *db)
+{
+   double z = 0.0;
+   double R = sqrt(r * r + z * z);
+   double st = r / R, ct = z / R;
+   double denom = 1.0 / (R * R + 2 * M * R);
+   double d_denom = -denom * denom * 2 * (R + M);
+   for (int i = 0; i < 3; ++i) db[i] = 0;
+   db[3] = 2 * M * denom + 2 * M * r * d_denom * st;
+   db[4] = 0.0;
+   db[5] = 2 * M * z * d_denom * st;
+   for (int i = 0; i < 3; ++i) db[6 + i] = 0;
+   db[9] = 2 * M * r * d_denom * ct;
+   db[10] = 0.0;
+   db[11] = 2 * M * denom + 2 * M * z * d_denom * ct;
+}
+
+// The Eulerian observer (normal.c:9-42).
+DHD void gr_frame_U(double M, double r, double *U)
+{
+   double b[3];
+   double ia = 1.0 / gr_lapse(M, r);
+   gr_shift(M, r, b);
+   U[0] = ia;
+   U[1] = -ia * b[0];
+   U[2] = -ia * b[1];
+   U[3] = -ia * b[2];
+}
+
+DHD void gr_frame_der_U(double M, double r, double *dU)
+{
+   double da[4], b[3], db[12];
+   double ia = 1.0 / gr_lapse(M, r);
+   gr_der_lapse(M, r, da);
+   gr_shift(M, r, b);
+   gr_der_shift(M, r, db);
+   for (int i = 0; i < 16; ++i) dU[i] = 0.0;
+   for (int i = 0; i < 4; ++i) {
+      if (gr_killing(i)) continue;
+      dU[4 * i] = -ia * ia * da[i];
+      for (int j = 0; j < 3; ++j) dU[4 * i + j + 1] = b[j] * ia * ia * da[i] - db[3 * i + j] * ia;
+   }
+}
+
+DHD GRMetric gr_metric(double M, double r)
+{
+   GRMetric g;
+   g.al = gr_lapse(M, r);
+   gr_shift(M, r, g.be);
+   gr_gam(M, r, g.gam);
+   gr_igam(M, r, g.igam);
+   g.jac = r / r;  // metric_jacobian (= r) over r
+   g.sqrtgam = g.jac / g.al;
+   gr_frame_U(M, r, g.U);
+   return g;
+}
+
+// The fluid of a primitive state in a metric: the quantities prim2cons,
+// flux and source share (grmhd2.c:88-140, 272-320).
+struct GRState {
+   double rho, Pp, l[3], B[3];
+   double igaml[3], u2, w, u0, u[3], l0, uU;
+   double uB, b0, bd[3], B2, UB, b2, Ub;
+};
+
+DHD GRState gr_state(double gamma_law, const GRMetric &g, double r, const double *prim)
+{
+   GRState s;
+   s.rho = prim[RHO];
+   s.Pp = prim[PPP];
+   s.l[0] = prim[URR];
+   s.l[1] = prim[UPP];
+   s.l[2] = prim[UZZ];
+   s.B[0] = prim[BRR] / g.sqrtgam;
+   s.B[1] = prim[BPP] / (r * g.sqrtgam);
+   s.B[2] = prim[BZZ] / g.sqrtgam;
+   for (int i = 0; i < 3; ++i) {
+      s.igaml[i] = 0.0;
+      for (int j = 0; j < 3; ++j) s.igaml[i] += g.igam[3 * i + j] * s.l[j];
+   }
+   s.u2 = s.l[0] * s.igaml[0] + s.l[1] * s.igaml[1] + s.l[2] * s.igaml[2];
+   s.w = sqrt(1.0 + s.u2);
+   s.u0 = s.w / g.al;
+   for (int i = 0; i < 3; ++i) s.u[i] = s.igaml[i] - g.be[i] * s.u0;
+   s.l0 = -g.al * s.w + g.be[0] * s.l[0] + g.be[1] * s.l[1] + g.be[2] * s.l[2];
+   s.uU = g.U[0] * s.l0 + g.U[1] * s.l[0] + g.U[2] * s.l[1] + g.U[3] * s.l[2];
+   s.uB = s.l[0] * s.B[0] + s.l[1] * s.B[1] + s.l[2] * s.B[2];
+   s.b0 = s.uB / g.al;
+   for (int i = 0; i < 3; ++i)
+      s.bd[i] = (g.gam[3 * i] * s.B[0] + g.gam[3 * i + 1] * s.B[1] + g.gam[3 * i + 2] * s.B[2] +
+                 s.l[i] * s.uB) / s.w;
+   s.B2 = 0.0;
+   s.UB = 0.0;
+   for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+         s.B2 += g.gam[3 * i + j] * s.B[i] * s.B[j];
+         s.UB += g.gam[3 * i + j] * s.B[i] * (g.U[0] * g.be[j] + g.U[j + 1]);
+      }
+   s.b2 = (s.B2 + s.uB * s.uB) / (s.w * s.w);
+   s.Ub = (s.UB + s.uU * s.uB) / s.w;
+   (void)gamma_law;
+   return s;
+}
+
+template <int NQ>
+struct GRMHD {
+   static constexpr int NC = 8;
+
+   // bfield_scale_factor (grmhd2.c:899-909)
+   static DHD double bscale_r(double r) { return 1.0 / r; }
+
+   // prim2cons (grmhd2.c:88-180)
+   static DHD void prim2cons(const dh_params &P, const double *prim, double *cons, double r,
+                             double dV)
+   {
+      double gl = P.gamma_law;
+      GRMetric g = gr_metric(P.metric_par2, r);
+      GRState s = gr_state(gl, g, r, prim);
+      double rhoh = s.rho + gl / (gl - 1.0) * s.Pp + s.b2;
+      double rhoe = s.Pp / (gl - 1.0);
+      double jac = g.jac;
+      cons[DDD] = jac * s.rho * s.u0 * dV;
+      cons[SRR] = jac * (rhoh * s.u0 * s.l[0] - s.b0 * s.bd[0]) * dV;
+      cons[LLL] = jac * (rhoh * s.u0 * s.l[1] - s.b0 * s.bd[1]) * dV;
+      cons[SZZ] = jac * (rhoh * s.u0 * s.l[2] - s.b0 * s.bd[2]) * dV;
+      cons[TAU] = jac * (-(rhoe + 0.5 * s.b2) * s.uU * s.u0 - (s.Pp + 0.5 * s.b2) * (s.uU * s.u0 + g.U[0]) -
+                         s.rho * (s.uU + 1.0) * s.u0 + s.Ub * s.b0) * dV;
+      cons[BRR] = g.sqrtgam * s.B[0] / r * dV;
+      cons[BPP] = g.sqrtgam * s.B[1] * dV;
+      cons[BZZ] = g.sqrtgam * s.B[2] * dV;
+#pragma unroll
+      for (int q = NC; q < NQ; ++q) cons[q] = prim[q] * cons[DDD];
+   }
+
+   // flux (grmhd2.c:272-381) through a face of unit normal n at radius r.
+   static DHD void flux(const dh_params &P, const GRMetric &g, double r, const double *prim,
+                        const double *n, double *F)
+   {
+      double gl = P.gamma_law;
+      GRState s = gr_state(gl, g, r, prim);
+      double v[3];
+      for (int i = 0; i < 3; ++i) v[i] = s.u[i] / s.u0;
+      double un = s.u[0] * n[0] + s.u[1] * n[1] + s.u[2] * n[2];
+      double Bn = s.B[0] * n[0] + s.B[1] * n[1] + s.B[2] * n[2];
+      double Un = g.U[1] * n[0] + g.U[2] * n[1] + g.U[3] * n[2];
+      double vn = v[0] * n[0] + v[1] * n[1] + v[2] * n[2];
+      double hn = n[0] + r * n[1] + n[2];
+      double bn = (Bn + s.uB * un) / s.w;
+      double rhoh = s.rho + gl / (gl - 1.0) * s.Pp + s.b2;
+      double rhoe = s.Pp / (gl - 1.0);
+      double Pt = s.Pp + 0.5 * s.b2;
+      double jac = g.jac;
+      F[DDD] = jac * hn * s.rho * un;
+      F[SRR] = jac * hn * (rhoh * un * s.l[0] + Pt * n[0] - s.bd[0] * bn);
+      F[LLL] = jac * hn * (rhoh * un * s.l[1] + Pt * n[1] - s.bd[1] * bn);
+      F[SZZ] = jac * hn * (rhoh * un * s.l[2] + Pt * n[2] - s.bd[2] * bn);
+      F[TAU] = jac * hn * (-(rhoe + 0.5 * s.b2) * s.uU * un - Pt * (s.uU * un + Un) -
+                           s.rho * (s.uU + 1.0) * un + s.Ub * bn);
+      F[BRR] = g.sqrtgam * hn * (s.B[0] * vn - Bn * v[0]) / r;
+      F[BPP] = g.sqrtgam * hn * (s.B[1] * vn - Bn * v[1]);
+      F[BZZ] = g.sqrtgam * hn * (s.B[2] * vn - Bn * v[2]);
+#pragma unroll
+      for (int q = NC; q < NQ; ++q) F[q] = prim[q] * F[DDD];
+   }
+
+   // getUstar (grmhd2.c:182-251): the HLLC star state on side k.
+   static DHD void ustar(const dh_params &P, const GRMetric &g, double r, const double *prim,
+                         double Sk, double Ss, const double *n, double *Us)
+   {
+      double gl = P.gamma_law;
+      double rho = prim[RHO], Pp = prim[PPP];
+      double l[3] = {prim[URR], prim[UPP], prim[UZZ]};
+      double al = g.al;
+      const double *be = g.be, *igam = g.igam, *U = g.U;
+      double bn = n[0] * be[0] + n[1] * be[1] + n[2] * be[2];
+      double ign = n[0] * igam[0] + n[1] * igam[4] + n[2] * igam[8];
+      double Un = n[0] * U[1] + n[1] * U[2] + n[2] * U[3];
+      double hn = n[0] + r * n[1] + n[2];
+      double ss = Ss / hn, sk = Sk / hn;
+      double uS[3], u2 = 0.0;
+      for (int i = 0; i < 3; ++i) {
+         uS[i] = 0.0;
+         for (int j = 0; j < 3; ++j) uS[i] += igam[3 * i + j] * l[j];
+         u2 += l[i] * uS[i];
+      }
+      double w = sqrt(1 + u2);
+      double u0 = w / al;
+      double l0 = -al * w + be[0] * l[0] + be[1] * l[1] + be[2] * l[2];
+      double vn = al * (n[0] * uS[0] + n[1] * uS[1] + n[2] * uS[2]) / w - bn;
+      double uU = l0 * U[0] + l[0] * U[1] + l[1] * U[2] + l[2] * U[3];
+      double rhoh = rho + gl / (gl - 1.0) * Pp;
+      double qE = rhoh * w * w * (vn - sk) + Pp * (sk + bn);
+      double mn = rhoh * w * (n[0] * uS[0] + n[1] * uS[1] + n[2] * uS[2]);
+      double qMn = mn * (vn - sk) + al * Pp * ign;
+      double ssS = (ss + bn) / al;
+      double skS = (sk + bn) / al;
+      double Pstar = (qMn - ssS * qE) / (al * (ign - ssS * skS));
+      double kappa = (vn - sk) / (ss - sk);
+      double alpha1 = (Pp - Pstar) / (ss - sk);
+      double alpha2 = (vn * Pp - ss * Pstar) / (ss - sk);
+      double rhoe = Pp / (gl - 1.0);
+      double tau = -rhoe * uU * u0 - Pp * (u0 * uU + U[0]) - rho * (uU + 1.0) * u0;
+      double jac = g.jac;
+      Us[DDD] = jac * rho * u0 * kappa;
+      Us[SRR] = jac * (rhoh * u0 * l[0] * kappa + alpha1 * n[0]);
+      Us[LLL] = jac * (rhoh * u0 * l[1] * kappa + alpha1 * n[1]);
+      Us[SZZ] = jac * (rhoh * u0 * l[2] * kappa + alpha1 * n[2]);
+      Us[TAU] = jac * (tau * kappa - Un * alpha1 + U[0] * alpha2);
+      // the field components of the star state stay zero (grmhd2.c:182-251
+      // leaves them unset; riemann.c:222-226 zeroes Ustr)
+      Us[BRR] = Us[BPP] = Us[BZZ] = 0.0;
+#pragma unroll
+      for (int q = NC; q < NQ; ++q) Us[q] = prim[q] * Us[DDD];
+   }
+
+   // vel (grmhd2.c:542-746, EXACT_DISP 0): outer wave speeds Sl, Sr and the
+   // contact speed Ss of the face.
+   static DHD void vel(const dh_params &P, const GRMetric &g, double r, const double *prim1,
+                       const double *prim2, const double *n, double *Sl, double *Sr, double *Ss)
+   {
+      double gl = P.gamma_law;
+      double al = g.al;
+      const double *be = g.be, *gam = g.gam, *igam = g.igam;
+      double sqrtgam = r / (al * r);  // metric_jacobian / (al r)
+      double rho1 = prim1[RHO], P1 = prim1[PPP];
+      double l1[3] = {prim1[URR], prim1[UPP], prim1[UZZ]};
+      double B1[3] = {prim1[BRR] / sqrtgam, prim1[BPP] / (r * sqrtgam), prim1[BZZ] / sqrtgam};
+      double cs21 = gl * P1 / (rho1 + gl / (gl - 1.0) * P1);
+      double rho2 = prim2[RHO], P2 = prim2[PPP];
+      double l2[3] = {prim2[URR], prim2[UPP], prim2[UZZ]};
+      double B2[3] = {prim2[BRR] / sqrtgam, prim2[BPP] / (r * sqrtgam), prim2[BZZ] / sqrtgam};
+      double cs22 = gl * P2 / (rho2 + gl / (gl - 1.0) * P2);
+      double u21 = 0.0, u22 = 0.0, uS1[3], uS2[3];
+      for (int i = 0; i < 3; ++i) {
+         uS1[i] = 0.0;
+         uS2[i] = 0.0;
+         for (int j = 0; j < 3; ++j) {
+            uS1[i] += igam[3 * i + j] * l1[j];
+            uS2[i] += igam[3 * i + j] * l2[j];
+         }
+         u21 += uS1[i] * l1[i];
+         u22 += uS2[i] * l2[i];
+      }
+      double w1 = sqrt(1.0 + u21), w2 = sqrt(1.0 + u22);
+      double v21 = u21 / (w1 * w1), v22 = u22 / (w2 * w2);
+      double uB1 = l1[0] * B1[0] + l1[1] * B1[1] + l1[2] * B1[2];
+      double uB2 = l2[0] * B2[0] + l2[1] * B2[1] + l2[2] * B2[2];
+      double B21 = 0.0, B22 = 0.0;
+      for (int i = 0; i < 3; ++i)
+         for (int j = 0; j < 3; ++j) {
+            B21 += gam[3 * i + j] * B1[i] * B1[j];
+            B22 += gam[3 * i + j] * B2[i] * B2[j];
+         }
+      double b21 = (B21 + uB1 * uB1) / (w1 * w1);
+      double b22 = (B22 + uB2 * uB2) / (w2 * w2);
+      double va21 = b21 / (rho1 + gl / (gl - 1.0) * P1 + b21);
+      double va22 = b22 / (rho2 + gl / (gl - 1.0) * P2 + b22);
+      double cf21 = cs21 + va21 - cs21 * va21;
+      double cf22 = cs22 + va22 - cs22 * va22;
+      double vSn1 = (uS1[0] * n[0] + uS1[1] * n[1] + uS1[2] * n[2]) / w1;
+      double vSn2 = (uS2[0] * n[0] + uS2[1] * n[1] + uS2[2] * n[2]) / w2;
+      double bn = (be[0] * n[0] + be[1] * n[1] + be[2] * n[2]);
+      double ign = igam[3 * 0 + 0] * n[0] + igam[3 * 1 + 1] * n[1] + igam[3 * 2 + 2] * n[2];
+      double dv1 = sqrt(cf21 * (ign - vSn1 * vSn1 - cf21 * (ign * v21 - vSn1 * vSn1))) / w1;
+      double dv2 = sqrt(cf22 * (ign - vSn2 * vSn2 - cf22 * (ign * v22 - vSn2 * vSn2))) / w2;
+      double hn = n[0] + r * n[1] + n[2];
+      double sl1 = hn * (al * (vSn1 * (1.0 - cf21) - dv1) / (1.0 - v21 * cf21) - bn);
+      double sr1 = hn * (al * (vSn1 * (1.0 - cf21) + dv1) / (1.0 - v21 * cf21) - bn);
+      double sl2 = hn * (al * (vSn2 * (1.0 - cf22) - dv2) / (1.0 - v22 * cf22) - bn);
+      double sr2 = hn * (al * (vSn2 * (1.0 - cf22) + dv2) / (1.0 - v22 * cf22) - bn);
+      *Sr = sr1 > sr2 ? sr1 : sr2;
+      *Sl = sl1 < sl2 ? sl1 : sl2;
+
+      // contact speed from the HLL state of energy and normal momentum
+      double sL = *Sl / hn, sR = *Sr / hn;
+      double rhohL = rho1 + gl / (gl - 1) * P1;
+      double rhohR = rho2 + gl / (gl - 1) * P2;
+      double vnL = al * vSn1 - bn, vnR = al * vSn2 - bn;
+      double ML[3] = {rhohL * w1 * l1[0], rhohL * w1 * l1[1], rhohL * w1 * l1[2]};
+      double MR[3] = {rhohR * w2 * l2[0], rhohR * w2 * l2[1], rhohR * w2 * l2[2]};
+      double EL = rhohL * w1 * w1 - P1;
+      double ER = rhohR * w2 * w2 - P2;
+      double FML[3], FMR[3];
+      for (int i = 0; i < 3; ++i) {
+         FML[i] = ML[i] * vnL + n[i] * al * P1;
+         FMR[i] = MR[i] * vnR + n[i] * al * P2;
+      }
+      double FEL = EL * vnL + P1 * (vnL + bn);
+      double FER = ER * vnR + P2 * (vnR + bn);
+      double UE = (sR * ER - sL * EL - FER + FEL) / (sR - sL);
+      double FE = ((sR * FEL - sL * FER + sL * sR * (ER - EL)) / (sR - sL) - bn * UE) / al;
+      double UM = 0.0, FM = 0.0;
+      for (int i = 0; i < 3; ++i) {
+         double UM_hll = (sR * MR[i] - sL * ML[i] - FMR[i] + FML[i]) / (sR - sL);
+         double FM_hll = ((sR * FML[i] - sL * FMR[i] + sL * sR * (MR[i] - ML[i])) / (sR - sL) -
+                          bn * UM_hll) / al;
+         double igi = n[0] * igam[3 * i] + n[1] * igam[3 * i + 1] + n[2] * igam[3 * i + 2];
+         UM += igi * UM_hll;
+         FM += igi * FM_hll;
+      }
+      double A = FE;
+      double B = -FM - ign * UE;
+      double C = ign * UM;
+      double sS;
+      if (fabs(4 * A * C / (B * B)) < 1.0e-7)
+         sS = -C / B * (1.0 + A * C / (B * B) + 2 * A * A * C * C / (B * B * B * B));
+      else
+         sS = (-B - sqrt(B * B - 4 * A * C)) / (2 * A);
+      *Ss = hn * (al * sS - bn);
+   }
+
+   // solve_riemann with HLL or HLLC (riemann.c:216-292): flux and fan state
+   // at face speed w.  r is the face's radius (the phi scale factor).
+   static DHD void riemann(const dh_params &P, const double *pL, const double *pR, double r,
+                           const double *n, double w, double *F, double *U)
+   {
+      GRMetric g = gr_metric(P.metric_par2, r);
+      double Sl, Sr, Ss;
+      vel(P, g, r, pL, pR, n, &Sl, &Sr, &Ss);
+      if (w < Sl) {
+         flux(P, g, r, pL, n, F);
+         prim2cons(P, pL, U, r, 1.0);
+      } else if (w > Sr) {
+         flux(P, g, r, pR, n, F);
+         prim2cons(P, pR, U, r, 1.0);
+      } else if (P.riemann_solver == RS_HLL) {
+         double Fl[NQ], Fr[NQ], Ul[NQ], Ur[NQ];
+         double aL = Sr, aR = -Sl;
+         prim2cons(P, pL, Ul, r, 1.0);
+         prim2cons(P, pR, Ur, r, 1.0);
+         flux(P, g, r, pL, n, Fl);
+         flux(P, g, r, pR, n, Fr);
+#pragma unroll
+         for (int q = 0; q < NQ; ++q) {
+            F[q] = (aL * Fl[q] + aR * Fr[q] + aL * aR * (Ul[q] - Ur[q])) / (aL + aR);
+            U[q] = (aR * Ul[q] + aL * Ur[q] + Fl[q] - Fr[q]) / (aL + aR);
+         }
+      } else {
+         const double *pk = w < Ss ? pL : pR;
+         double Sk = w < Ss ? Sl : Sr;
+         double Uk[NQ], Fk[NQ];
+         prim2cons(P, pk, Uk, r, 1.0);
+         ustar(P, g, r, pk, Sk, Ss, n, U);
+         flux(P, g, r, pk, n, Fk);
+#pragma unroll
+         for (int q = 0; q < NQ; ++q) F[q] = Fk[q] + Sk * (U[q] - Uk[q]);
+      }
+   }
+
+   // cons2prim (grmhd2.c:253-270) with the Noble et al. 2-D Newton solve
+   // (grmhd2.c:1096-1314); prim holds the previous primitives on entry (the
+   // initial guess) and the result on exit.  r is the cell centroid radius.
+   static DHD void cons2prim(const dh_params &P, const double *cons_in, double *prim, double r,
+                             double dV)
+   {
+      double gl = P.gamma_law;
+      double cons[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) cons[q] = cons_in[q] / dV;
+      const double prec = 1.0e-12;
+      const double max_iter = 30;
+      const double Nextra = 10;
+      double D = cons[DDD];
+      double S[3] = {cons[SRR], cons[LLL], cons[SZZ]};
+      double tau = cons[TAU];
+      GRMetric g = gr_metric(P.metric_par2, r);
+      double al = g.al, jac = g.jac, sqrtgam = g.sqrtgam;
+      const double *be = g.be, *gam = g.gam, *igam = g.igam, *U = g.U;
+      double B[3] = {r * cons[BRR] / sqrtgam, cons[BPP] / sqrtgam, cons[BZZ] / sqrtgam};
+      double s2 = 0.0, Us = 0.0, BS = 0.0, B2 = 0.0;
+      for (int i = 0; i < 3; ++i)
+         for (int j = 0; j < 3; ++j) s2 += igam[3 * i + j] * S[i] * S[j];
+      s2 /= D * D;
+      for (int i = 0; i < 3; ++i) Us += S[i] * (be[i] * U[0] + U[i + 1]);
+      Us /= D;
+      for (int i = 0; i < 3; ++i)
+         for (int j = 0; j < 3; ++j) B2 += gam[3 * i + j] * B[i] * B[j];
+      double Q = sqrtgam * B2 / D;
+      for (int i = 0; i < 3; ++i) BS += B[i] * S[i];
+      double psi = sqrtgam * BS * BS / (D * D * D);
+      double e = (tau / D + Us + 1.0) / (al * U[0]);
+      double n = (gl - 1.0) / gl;
+
+      // initial guess: the previous v^2 and eta
+      double u2 = 0.0;
+      double l[3] = {prim[URR], prim[UPP], prim[UZZ]};
+      for (int i = 0; i < 3; ++i)
+         for (int j = 0; j < 3; ++j) u2 += igam[3 * i + j] * l[i] * l[j];
+      double w = sqrt(1.0 + u2);
+      double v21 = u2 / (1.0 + u2);
+      double eta1 = w * (1.0 + gl / (gl - 1.0) * prim[PPP] / prim[RHO]);
+      double v2, eta;
+      int it = 0, clean = -1;
+      while (1) {
+         v2 = v21;
+         eta = eta1;
+         w = 1.0 / sqrt(1.0 - v2);
+         double fa = (eta + Q) * (eta + Q) * v2 - psi * (2 * eta + Q) / (eta * eta) - s2;
+         double fb = eta - n * (1 - v2) * (eta - w) + 0.5 * Q * (1 + v2) - 0.5 * psi / (eta * eta) - e;
+         double dfadv2 = (eta + Q) * (eta + Q);
+         double dfadet = 2 * (eta + Q) * (v2 + psi / (eta * eta * eta));
+         double dfbdv2 = n * (eta - 0.5 * w) + 0.5 * Q;
+         double dfbdet = 1 - n * (1 - v2) + psi / (eta * eta * eta);
+         double detj = dfadv2 * dfbdet - dfbdv2 * dfadet;
+         v21 = v2 - (dfbdet * fa - dfadet * fb) / detj;
+         eta1 = eta - (-dfbdv2 * fa + dfadv2 * fb) / detj;
+         it++;
+         if (v21 > 1.0) v21 = 0.5 * (v2 + 1.0);
+         if (v21 < 0.0) v21 = 0.5 * v2;
+         if (eta1 < 1.0) eta1 = 0.5 * (eta + 1.0);
+         double err = (eta1 - eta) / eta;
+         if (fabs(err) < prec && clean < 0) clean = (int)Nextra + 1;
+         if (clean >= 0) clean--;
+         if (clean == 0 || it == max_iter) break;
+      }
+      v2 = v21;
+      eta = eta1;
+      w = 1.0 / sqrt(1 - v2);
+      double u0 = w / al;
+      double h = eta / w;
+      double rho = D / (jac * u0);
+      if (rho < P.rho_floor) rho = P.rho_floor;
+      double Pp = n * rho * (h - 1.0);
+      if (Pp < P.p_floor * rho) Pp = P.p_floor * rho;
+      h = 1.0 + gl / (gl - 1.0) * Pp / rho;
+      double Bd[3];
+      for (int i = 0; i < 3; ++i)
+         Bd[i] = gam[3 * i] * B[0] + gam[3 * i + 1] * B[1] + gam[3 * i + 2] * B[2];
+      for (int i = 0; i < 3; ++i)
+         l[i] = (S[i] + sqrtgam * BS * Bd[i] / (D * h * w)) / (D * h + sqrtgam * B2 / w);
+      prim[RHO] = rho;
+      prim[URR] = l[0];
+      prim[UPP] = l[1];
+      prim[UZZ] = l[2];
+      prim[PPP] = Pp;
+      prim[BRR] = sqrtgam * B[0];
+      prim[BPP] = sqrtgam * r * B[1];
+      prim[BZZ] = sqrtgam * B[2];
+#pragma unroll
+      for (int q = NC; q < NQ; ++q) prim[q] = cons[q] / cons[DDD];
+   }
+
+   // source (grmhd2.c:383-500): 1/2 T^{mu nu} d_i g_{mu nu} in the momenta
+   // and the frame term in the energy, at the cell's mid radius.
+   static DHD void source(const dh_params &P, const double *prim, double *cons, double rp,
+                          double rm, double phip, double phim, double zp, double zm,
+                          double dVdt, int polar, double Pmax)
+   {
+      double gl = P.gamma_law, M = P.metric_par2;
+      double r = 0.5 * (rm + rp);
+      GRMetric g = gr_metric(M, r);
+      double al = g.al, jac = g.jac, sqrtgam = g.sqrtgam;
+      const double *be = g.be, *gam = g.gam, *igam = g.igam, *U = g.U;
+      double dU[16];
+      gr_frame_der_U(M, r, dU);
+      double rho = prim[RHO], Pp = prim[PPP];
+      double l[4] = {0.0, prim[URR], prim[UPP], prim[UZZ]};
+      double B[3] = {prim[BRR] / sqrtgam, prim[BPP] / (r * sqrtgam), prim[BZZ] / sqrtgam};
+      double ig[16];
+      double ia2 = 1.0 / (al * al);
+      ig[0] = -ia2;
+      for (int mu = 0; mu < 3; ++mu) {
+         ig[mu + 1] = be[mu] * ia2;
+         ig[4 * (mu + 1)] = ig[mu + 1];
+         for (int nu = 0; nu < 3; ++nu) ig[4 * (mu + 1) + nu + 1] = igam[3 * mu + nu] - be[mu] * be[nu] * ia2;
+      }
+      double igaml[3];
+      for (int i = 0; i < 3; ++i)
+         igaml[i] = igam[3 * i] * l[1] + igam[3 * i + 1] * l[2] + igam[3 * i + 2] * l[3];
+      double u2 = l[1] * igaml[0] + l[2] * igaml[1] + l[3] * igaml[2];
+      double w = sqrt(1.0 + u2);
+      double u[4];
+      u[0] = w / al;
+      for (int i = 0; i < 3; ++i) u[i + 1] = igaml[i] - be[i] * u[0];
+      l[0] = -al * w + be[0] * l[1] + be[1] * l[2] + be[2] * l[3];
+      double uB = l[1] * B[0] + l[2] * B[1] + l[3] * B[2];
+      double b[4], bd[4];
+      b[0] = uB / al;
+      for (int i = 0; i < 3; ++i) b[i + 1] = (B[i] + u[i + 1] * uB) / w;
+      for (int i = 0; i < 3; ++i)
+         bd[i + 1] = (gam[3 * i] * B[0] + gam[3 * i + 1] * B[1] + gam[3 * i + 2] * B[2] + l[i + 1] * uB) / w;
+      double B2 = 0.0, beB = 0.0;
+      for (int i = 0; i < 3; ++i)
+         for (int j = 0; j < 3; ++j) {
+            beB += gam[3 * i + j] * be[i] * B[j];
+            B2 += gam[3 * i + j] * B[i] * B[j];
+         }
+      bd[0] = (beB + l[0] * uB) / w;
+      double b2 = (B2 + uB * uB) / (w * w);
+      double rhoh = rho + gl / (gl - 1.0) * Pp + b2;
+      double Pt = Pp + 0.5 * b2;
+      const double rcorr = 1.0, pcorr = 1.0;
+      double Sk[3];
+      for (int i = 0; i < 3; ++i) {
+         Sk[i] = 0.0;
+         if (gr_killing(i + 1)) continue;
+         double dg[16];
+         gr_der_g(M, r, i + 1, dg);
+         for (int mu = 0; mu < 4; ++mu)
+            for (int nu = 0; nu < 4; ++nu)
+               Sk[i] += (rhoh * u[mu] * u[nu] * rcorr * pcorr + ig[4 * mu + nu] * Pt -
+                         b[mu] * b[nu] * rcorr * pcorr) * dg[4 * mu + nu];
+         Sk[i] *= 0.5;
+      }
+      double S0 = -U[1] * Sk[0] - U[2] * Sk[1] - U[3] * Sk[2];
+      for (int mu = 0; mu <= 3; ++mu)
+         for (int nu = 0; nu <= 3; ++nu) {
+            if (mu == nu)
+               S0 += -(rhoh * u[mu] * l[nu] + Pt - b[mu] * bd[nu]) * dU[4 * mu + nu];
+            else
+               S0 += -(rhoh * u[mu] * l[nu] - b[mu] * bd[nu]) * dU[4 * mu + nu];
+         }
+      cons[SRR] += jac * Sk[0] * dVdt;
+      cons[LLL] += jac * Sk[1] * dVdt;
+      cons[SZZ] += jac * Sk[2] * dVdt;
+      cons[TAU] += jac * S0 * dVdt;
+      (void)phip; (void)phim; (void)zp; (void)zm; (void)polar; (void)Pmax;
+   }
+
+   static DHD void visc_source(const dh_params &, const double *, const double *,
+                               const double *, const double *, double *, double, double)
+   {
+   }
+   static DHD void visc_flux(const dh_params &, const double *, const double *,
+                             const double *, const double *, double *, double,
+                             const double *)
+   {
+   }
+
+   // mindt (grmhd2.c:748-832) at the cell's mid radius; w is the mean
+   // angular velocity of the cell's two phi faces.
+   static DHD double mindt(const dh_params &P, const double *prim, double wc, double rp,
+                           double rm, double phip, double phim, double zp, double zm,
+                           double Pmax)
+   {
+      double gl = P.gamma_law;
+      double r = 0.5 * (rm + rp);
+      GRMetric g = gr_metric(P.metric_par2, r);
+      double al = g.al;
+      const double *be = g.be, *gam = g.gam, *igam = g.igam;
+      double sqrtgam = r / (r * al);  // metric_jacobian / (r al)
+      double rho = prim[RHO], Pp = prim[PPP];
+      double l[3] = {prim[URR], prim[UPP], prim[UZZ]};
+      double B[3] = {prim[BRR] / sqrtgam, prim[BPP] / (r * sqrtgam), prim[BZZ] / sqrtgam};
+      double cs = sqrt(gl * Pp / (rho + gl / (gl - 1) * Pp));
+      double uS[3], u2 = 0.0;
+      for (int i = 0; i < 3; ++i) {
+         uS[i] = 0.0;
+         for (int j = 0; j < 3; ++j) uS[i] += igam[3 * i + j] * l[j];
+         u2 += uS[i] * l[i];
+      }
+      double w = sqrt(1.0 + u2);
+      double vS[3];
+      for (int i = 0; i < 3; ++i) vS[i] = uS[i] / w;
+      double v2 = u2 / (w * w);
+      double uB = l[0] * B[0] + l[1] * B[1] + l[2] * B[2];
+      double B2 = 0.0;
+      for (int i = 0; i < 3; ++i)
+         for (int j = 0; j < 3; ++j) B2 += gam[3 * i + j] * B[i] * B[j];
+      double b2 = (B2 + uB * uB) / (w * w);
+      double va2 = b2 / (rho + gl / (gl - 1.0) * Pp + b2);
+      double cf = sqrt(cs * cs + va2 - cs * cs * va2);
+      double sig = 1 - cf * cf;
+      double dvr = cf * sqrt(igam[0] * (1 - cf * cf * v2) - sig * vS[0] * vS[0]) / w;
+      double vrl = fabs(al * (vS[0] * sig - dvr) / (1 - v2 * cf * cf) - be[0]);
+      double vrr = fabs(al * (vS[0] * sig + dvr) / (1 - v2 * cf * cf) - be[0]);
+      double dvp = cf * sqrt(igam[4] * (1 - cf * cf * v2) - sig * vS[1] * vS[1]) / w;
+      double vpl = fabs(r * (al * (vS[1] * sig - dvp) / (1 - v2 * cf * cf) - be[1] - wc));
+      double vpr = fabs(r * (al * (vS[1] * sig + dvp) / (1 - v2 * cf * cf) - be[1] - wc));
+      double dvz = cf * sqrt(igam[8] * (1 - cf * cf * v2) - sig * vS[2] * vS[2]) / w;
+      double vzl = fabs(al * (vS[2] * sig - dvz) / (1 - v2 * cf * cf) - be[2]);
+      double vzr = fabs(al * (vS[2] * sig + dvz) / (1 - v2 * cf * cf) - be[2]);
+      double maxvr = vrr > vrl ? vrr : vrl;
+      double maxvp = vpr > vpl ? vpr : vpl;
+      double maxvz = vzr > vzl ? vzr : vzl;
+      double dL0 = .5 * (rp + rm) * get_dp(phip, phim, Pmax);
+      double dtr = (rp - rm) / maxvr;
+      double dtp = dL0 / maxvp;
+      double dtz = (zp - zm) / maxvz;
+      double dt = dtr;
+      dt = dt < dtp ? dt : dtp;
+      dt = dt < dtz ? dt : dtz;
+      return dt;
+   }
+
+   // get_omega (grmhd2.c:64-86): the coordinate angular velocity at radius r.
+   static DHD double omega(const dh_params &P, const double *prim, double r)
+   {
+      GRMetric g = gr_metric(P.metric_par2, r);
+      double l[3] = {prim[URR], prim[UPP], prim[UZZ]};
+      double u2 = 0.0;
+      for (int i = 0; i < 3; ++i)
+         for (int j = 0; j < 3; ++j) u2 += g.igam[3 * i + j] * l[i] * l[j];
+      double w = sqrt(1.0 + u2);
+      return g.al * (g.igam[3] * l[0] + g.igam[4] * l[1] + g.igam[5] * l[2]) / w - g.be[1];
+   }
+
+   // flux_to_E (grmhd2.c:509-540): {E1, B1, E2, B2}; dim 0 phi, 1 r, 2 z.
+   static DHD void flux_to_E(const double *F, const double *U, double r, int dim, double *emf)
+   {
+      if (dim == 0) {
+         emf[0] = F[BRR] * r;
+         emf[1] = U[BRR] * r * r;
+         emf[2] = F[BZZ];
+         emf[3] = U[BZZ] * r;
+      } else if (dim == 1) {
+         emf[0] = -F[BPP] * r;
+         emf[1] = U[BRR] * r * r;
+         emf[2] = 1.0 * F[BZZ];
+         emf[3] = 0.0;
+      } else {
+         emf[0] = -F[BPP] * r;
+         emf[1] = U[BZZ] * r;
+         emf[2] = 1.0 * -F[BRR] * r;
+         emf[3] = 0.0;
+      }
+   }
+};
+
+}  // namespace dh

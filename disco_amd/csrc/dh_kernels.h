@@ -16,6 +16,7 @@
 #include "dh_ct.h"
 #include "dh_euler.h"
 #include "dh_geom.h"
+#include "dh_grmhd.h"
 #include "dh_mesh.h"
 #include "dh_mhd.h"
 
@@ -172,7 +173,9 @@ struct KWcell {
          double pc[NQ], pp[NQ];
          load_q<NQ>(prim, nt, c, pc);
          load_q<NQ>(prim, nt, cp, pp);
-         w = .5 * (H::omega(pc) + H::omega(pp));
+         int j = a % m.Nr;
+         double r = centroid_r(m.r_jph[j], m.r_jph[j - 1]);
+         w = .5 * (H::omega(P, pc, r) + H::omega(P, pp, r));
       }
       wiph[c] = w;
    }
@@ -1423,6 +1426,7 @@ struct KBfacesC2p {
       double cs[NQ], out[NQ];
       load_q<NQ>(cons, nt, c, cs);
       double dVc = cell_dV(rp, rm, piph_new[c], piph_new[c] - dphi_new[c], zp, zm, m.Pmax);
+      load_q<NQ>(prim, nt, c, out);  // the previous primitives: an iterative solve's guess
       H::cons2prim(P, cs, out, x0, dVc);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) prim[q * nt + c] = out[q];

@@ -151,7 +151,7 @@ struct MHD {
       return dt;
    }
 
-   static DHD double omega(const double *prim) { return prim[UPP]; }
+   static DHD double omega(const dh_params &, const double *prim, double) { return prim[UPP]; }
 
    // ---- HLLD (hlld.c) ----
 

@@ -88,6 +88,19 @@ inline int validate(const dh_grid *g, const dh_params &P, long *ntot_out)
       if (nq < 8 || nq > 9) return fail(DH_EINVAL, "NUM_C+NUM_N = %d not supported", nq);
       if (g->NgRa == 0 || g->NgRb == 0 || (g->Nz > 1 && (g->NgZa == 0 || g->NgZb == 0)))
          return fail(DH_EINVAL, "mhd needs ghost annuli on every side");
+   } else if (P.hydro == DH_HYDRO_GRMHD) {
+      if (P.num_c != 8) return fail(DH_EINVAL, "grmhd2 needs NUM_C = 8 (got %d)", P.num_c);
+      if (P.ct_mode != 2 || !P.ct)
+         return fail(DH_EINVAL, "grmhd2 is supported with CT_MODE 2 and Constrained_Transport 1");
+      if (P.riemann_solver != 0 && P.riemann_solver != 1)
+         return fail(DH_EINVAL, "grmhd2 is supported with HLL and HLLC (Riemann_Solver 0, 1)");
+      if (P.isothermal) return fail(DH_EINVAL, "grmhd2 is supported with the adiabatic cons2prim");
+      if (P.visc_flag) return fail(DH_EINVAL, "grmhd2 has no viscous terms (Use_Viscosity 0)");
+      if (nq < 8 || nq > 9) return fail(DH_EINVAL, "NUM_C+NUM_N = %d not supported", nq);
+      if (g->NgRa == 0 || g->NgRb == 0 || (g->Nz > 1 && (g->NgZa == 0 || g->NgZb == 0)))
+         return fail(DH_EINVAL, "grmhd2 needs ghost annuli on every side");
+      if (P.mesh_motion == 3 || P.energy_omega)
+         return fail(DH_EINVAL, "grmhd2 takes Mesh_Motion 0, 1 or 4 and no energy frame");
    } else {
       return fail(DH_EINVAL, "hydro module %d not supported", P.hydro);
    }
@@ -129,7 +142,7 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
    DH_TRY(validate(g, P, &ntot));
    b.P = P;
    b.nq = P.num_c + P.num_n;
-   b.mhd = P.hydro == DH_HYDRO_MHD;
+   b.mhd = P.hydro == DH_HYDRO_MHD || P.hydro == DH_HYDRO_GRMHD;
    b.nf = b.mhd ? 5 : 0;
    int nq = b.nq;
    int nann = g->Nr * g->Nz;
@@ -480,6 +493,13 @@ struct Engine {
 // Dispatch on the hydro module and the number of fields (NUM_Q).
 #define DH_DISPATCH(b, FN, ...)                                               \
    ([&]() -> int {                                                            \
+      if ((b).P.hydro == DH_HYDRO_GRMHD) {                                    \
+         switch ((b).nq) {                                                    \
+         case 8: return FN<8, GRMHD<8>>(__VA_ARGS__);                         \
+         case 9: return FN<9, GRMHD<9>>(__VA_ARGS__);                         \
+         default: return fail(DH_EINVAL, "unsupported NQ %d", (b).nq);       \
+         }                                                                    \
+      }                                                                       \
       if ((b).mhd) {                                                          \
          switch ((b).nq) {                                                    \
          case 8: return FN<8, MHD<8>>(__VA_ARGS__);                           \

@@ -31,6 +31,9 @@
 #ifndef ANALYSIS
 #define ANALYSIS "diag_euler"
 #endif
+#ifndef HYDRO
+#define HYDRO (NUM_C > 5 ? "mhd" : "euler")
+#endif
 
 /* Plugin symbols (reference plugin ABI). */
 void setICparams(struct domain *);
@@ -325,7 +328,9 @@ static void fill_params(struct domain *d, dh_params *P, const int *bc)
    P->num_c = NUM_C;
    P->num_n = NUM_N;
    P->ct_mode = CT_MODE;
-   P->hydro = NUM_C > 5 ? DH_HYDRO_MHD : DH_HYDRO_EULER;
+   P->hydro = !strcmp(HYDRO, "grmhd2") ? DH_HYDRO_GRMHD
+              : !strcmp(HYDRO, "mhd")  ? DH_HYDRO_MHD
+                                       : DH_HYDRO_EULER;
    P->riemann_solver = L->Riemann_Solver;
    P->mesh_motion = L->Mesh_Motion;
    P->exact_mesh_omega = L->Exact_Mesh_Omega;

@@ -47,7 +47,12 @@ enum {
 };
 
 /* Hydro modules (reference Hydro/<name>.c). */
-enum { DH_HYDRO_EULER = 0, DH_HYDRO_MHD = 1 };
+enum {
+   DH_HYDRO_EULER = 0,  /* Hydro/euler.c */
+   DH_HYDRO_MHD = 1,    /* Hydro/mhd.c */
+   DH_HYDRO_GRMHD = 2   /* Hydro/grmhd2.c with Metric/schw_ks_eq_rpz.c (M = metric_par2)
+                           and Frame/normal.c */
+};
 
 /* Boundary treatment of one side of the mesh, applied to ghost annuli after
  * every substep (reference onestep.c:194-210 -> Boundary/<name>.c).

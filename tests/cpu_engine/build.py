@@ -53,6 +53,7 @@ def build_driver(config: str) -> str:
     cmd = ["gcc", "-O2", "-Wall", "-Wno-unused-parameter", f"-DNUM_C={c['num_c']}",
            f"-DNUM_N={c['num_n']}", f"-DCT_MODE={c['ct_mode']}", f'-DBOUNDARY="{c["boundary"]}"',
            f'-DINITIAL="{c["initial"]}"', f'-DANALYSIS="{c["analysis"]}"',
+           f'-DHYDRO="{c["hydro"]}"',
            f"-I{os.path.join(REPO, 'include', 'disco')}",
            f"-I{os.path.join(REPO, 'include')}", f"-I{host}", *srcs, "-o", out + ".tmp",
            f"-L{os.path.dirname(lib)}", "-ldh_cpu", "-lm",

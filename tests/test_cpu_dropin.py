@@ -5,7 +5,7 @@ gives the same run bit for bit: mesh set-up with the reference's rand()
 sequence, initial data and noise, the initial magnetic fluxes
 (set_B_fields), CFL steps, the timesteps themselves and the Analysis
 module's per-annulus sums (diag_euler with and without a companion planet,
-diag_disk2d, diag_mhdmri).  The GPU version of
+diag_disk2d, diag_mhdmri) -- for Euler, Newtonian MHD and GRMHD.  The GPU version of
 this test (test_gpu_dropin.py) runs the driver on libdisco_hip.so."""
 import os
 import subprocess
@@ -27,6 +27,11 @@ import build as cpu_build  # noqa: E402
     ("mhd_kep", {"Num_R": 8, "Num_Z": 4}, 5),
     ("mhd_kep", {"Num_R": 12, "Num_Z": 6, "Max_DT": 0.02}, 6),
     ("mhd_kep_p2", {"Num_R": 12, "Num_Z": 4, "Max_DT": 0.02}, 4),
+    # GRMHD on the equator of a Schwarzschild black hole: HLLC, HLL, and mesh
+    # motion from the fluid's angular velocity (get_omega)
+    ("gr_disk", {"Num_R": 24, "Num_Z": 6}, 8),
+    ("gr_disk", {"Num_R": 16, "Num_Z": 4, "Riemann_Solver": 0}, 4),
+    ("gr_disk", {"Num_R": 16, "Num_Z": 4, "Mesh_Motion": 1}, 4),
 ])
 def test_driver_bit_exact_vs_reference(cname, over, steps):
     cfg = configs.get(cname)

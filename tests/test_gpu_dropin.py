@@ -28,6 +28,8 @@ HOST = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
     ("euler_yee_n1", {"Num_R": 64, "Riemann_Solver": 0}, 10),
     ("mhd_kep", {"Num_R": 32, "Num_Z": 8}, 6),
     ("mhd_kep_p2", {"Num_R": 32, "Num_Z": 8, "Max_DT": 0.01}, 6),
+    ("gr_disk", {"Num_R": 48, "Num_Z": 12}, 8),
+    ("gr_disk", {"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 0}, 6),
 ])
 def test_disco_driver_matches_reference(cname, over, steps):
     cfg = configs.get(cname)
