@@ -12,11 +12,13 @@
 // the position.
 //
 // Riemann solvers: HLL and HLLC (riemann.c:216-292 with grmhd2.c's vel,
-// flux, prim2cons and getUstar); cons2prim: the 2-D Newton scheme of Noble
-// et al. (grmhd2.c:1096-1314).  The expressions keep the reference's
+// flux, prim2cons and getUstar) and the relativistic HLLD of hlld_grmhd.c
+// (dh_grhlld.h); cons2prim: the 2-D Newton scheme of Noble et al.
+// (grmhd2.c:1096-1314).  The expressions keep the reference's
 // association so that the host build reproduces the reference bit for bit.
 #pragma once
 #include "dh_geom.h"
+#include "dh_grhlld.h"
 
 namespace dh {
 
@@ -431,11 +433,126 @@ struct GRMHD {
       *Ss = hn * (al * sS - bn);
    }
 
-   // solve_riemann with HLL or HLLC (riemann.c:216-292): flux and fan state
-   // at face speed w.  r is the face's radius (the phi scale factor).
+   // get_Ustar_HLLD (hlld_grmhd.c:90-368): the relativistic HLLD fan solved
+   // in the face's orthonormal frame, moved back to coordinate components.
+   static DHD void riemann_hlld(const dh_params &P, const double *pL, const double *pR,
+                                double r, const double *n, double w, double *F, double *U)
+   {
+      double gl = P.gamma_law;
+      GRMetric g = gr_metric(P.metric_par2, r);
+      double al = g.al, jac = g.jac, sqrtgam = g.sqrtgam;
+      const double *be = g.be, *gam = g.gam, *igam = g.igam, *Uf = g.U;
+      double rhoL = pL[RHO], PL = pL[PPP];
+      double lL[3] = {pL[URR], pL[UPP], pL[UZZ]};
+      double BL[3] = {pL[BRR] / sqrtgam, pL[BPP] / (r * sqrtgam), pL[BZZ] / sqrtgam};
+      double rhoR = pR[RHO], PR = pR[PPP];
+      double lR[3] = {pR[URR], pR[UPP], pR[UZZ]};
+      double BR[3] = {pR[BRR] / sqrtgam, pR[BPP] / (r * sqrtgam), pR[BZZ] / sqrtgam};
+      int xx = (n[0] > n[1] && n[0] > n[2]) ? 0 : (n[1] > n[2] ? 1 : 2);
+      // the face's four-velocity
+      if (xx == 1) w /= r;
+      double be2 = 0.0, bex = 0.0;
+      for (int i = 0; i < 3; ++i)
+         for (int j = 0; j < 3; ++j) be2 += gam[3 * i + j] * be[i] * be[j];
+      for (int i = 0; i < 3; ++i) bex += gam[3 * xx + i] * be[i];
+      double uw[4] = {0.0, 0.0, 0.0, 0.0};
+      uw[0] = 1.0 / sqrt(al * al - be2 - 2 * bex * w - gam[3 * xx + xx] * w * w);
+      uw[xx + 1] = uw[0] * w;
+      double e[16], ie[16];
+      grh::tetrads(al, be, gam, igam, r * sqrtgam, xx, e, ie);
+      // the states in the orthonormal frame (e_(t) is the normal observer)
+      double uLo[4], uRo[4], BLo[3], BRo[3], uwo[4];
+      for (int i = 0; i < 3; ++i) {
+         uLo[i + 1] = 0.0;
+         uRo[i + 1] = 0.0;
+         BLo[i] = 0.0;
+         BRo[i] = 0.0;
+         for (int j = 0; j < 3; ++j) {
+            uLo[i + 1] += e[4 * (i + 1) + j + 1] * lL[j];
+            uRo[i + 1] += e[4 * (i + 1) + j + 1] * lR[j];
+            BLo[i] += ie[4 * (i + 1) + j + 1] * BL[j];
+            BRo[i] += ie[4 * (i + 1) + j + 1] * BR[j];
+         }
+      }
+      uLo[0] = sqrt(1.0 + uLo[1] * uLo[1] + uLo[2] * uLo[2] + uLo[3] * uLo[3]);
+      uRo[0] = sqrt(1.0 + uRo[1] * uRo[1] + uRo[2] * uRo[2] + uRo[3] * uRo[3]);
+      for (int k = 0; k < 4; ++k) uwo[k] = ie[4 * k] * uw[0] + ie[4 * k + xx + 1] * uw[xx + 1];
+      double Bx = 0.5 * (BLo[0] + BRo[0]);
+      BLo[0] = Bx;
+      BRo[0] = Bx;
+      double wo = uwo[1] / uwo[0];
+      // outer wave speeds in the orthonormal frame
+      double rhohL = rhoL + gl / (gl - 1) * PL;
+      double rhohR = rhoR + gl / (gl - 1) * PR;
+      double cs2L = gl * PL / rhohL, cs2R = gl * PR / rhohR;
+      double b0Lo = uLo[1] * BLo[0] + uLo[2] * BLo[1] + uLo[3] * BLo[2];
+      double b0Ro = uRo[1] * BRo[0] + uRo[2] * BRo[1] + uRo[3] * BRo[2];
+      double b2L = (BLo[0] * BLo[0] + BLo[1] * BLo[1] + BLo[2] * BLo[2] + b0Lo * b0Lo) / (uLo[0] * uLo[0]);
+      double b2R = (BRo[0] * BRo[0] + BRo[1] * BRo[1] + BRo[2] * BRo[2] + b0Ro * b0Ro) / (uRo[0] * uRo[0]);
+      double sL1, sL2, sR1, sR2;
+      grh::fast_speeds(cs2L, uLo[0], uLo[1] / uLo[0], b2L, rhohL, 1.0, 0.0, 1.0, &sL1, &sL2);
+      grh::fast_speeds(cs2R, uRo[0], uRo[1] / uRo[0], b2R, rhohR, 1.0, 0.0, 1.0, &sR1, &sR2);
+      double sLo = sL1 < sR1 ? sL1 : sR1;
+      double sRo = sL2 > sR2 ? sL2 : sR2;
+      double ULo[8], FLo[8], URo[8], FRo[8], Uo[8], Fo[8];
+      grh::state_UF(gl, rhoL, PL, uLo, BLo, ULo, FLo);
+      grh::state_UF(gl, rhoR, PR, uRo, BRo, URo, FRo);
+      double primLo[5] = {rhoL, PL, uLo[1], uLo[2], uLo[3]};
+      double primRo[5] = {rhoR, PR, uRo[1], uRo[2], uRo[3]};
+      int zone = grh::solve_fan(gl, sLo, sRo, Bx, ULo, FLo, URo, FRo, wo, primLo, primRo, Uo, Fo);
+      // back to coordinate components: the energy-momentum covector and the
+      // field vector of the fan state and flux
+      using namespace grh;
+      double Uc[4], Fc[4], UB[3], FB[3];
+      for (int mu = 0; mu < 4; ++mu) {
+         Uc[mu] = -ie[mu] * Uo[EN] + ie[4 + mu] * Uo[MX] + ie[8 + mu] * Uo[MY] + ie[12 + mu] * Uo[MZ];
+         Fc[mu] = -ie[mu] * Fo[EN] + ie[4 + mu] * Fo[MX] + ie[8 + mu] * Fo[MY] + ie[12 + mu] * Fo[MZ];
+      }
+      for (int i = 0; i < 3; ++i) {
+         UB[i] = e[4 + i + 1] * Uo[BX] + e[8 + i + 1] * Uo[BY] + e[12 + i + 1] * Uo[BZ];
+         FB[i] = e[4 + i + 1] * Fo[BX] + e[8 + i + 1] * Fo[BY] + e[12 + i + 1] * Fo[BZ];
+      }
+      // Uc = {tau, S_r, S_phi, S_z} before the frame subtraction
+      double hn = 1.0 * n[0] + r * n[1] + 1.0 * n[2];
+      double t0 = e[0], t1 = e[4], x0 = e[xx + 1], x1 = e[4 + xx + 1];
+      U[DDD] = jac * (t0 * Uo[DD] + t1 * Fo[DD]);
+      U[SRR] = jac * (t0 * Uc[1] + t1 * Fc[1]);
+      U[LLL] = jac * (t0 * Uc[2] + t1 * Fc[2]);
+      U[SZZ] = jac * (t0 * Uc[3] + t1 * Fc[3]);
+      U[TAU] = jac * (t0 * Uc[0] + t1 * Fc[0]);
+      U[BRR] = sqrtgam * (t0 * UB[0] + t1 * FB[0]);
+      U[BPP] = sqrtgam * (t0 * UB[1] + t1 * FB[1]);
+      U[BZZ] = sqrtgam * (t0 * UB[2] + t1 * FB[2]);
+      F[DDD] = jac * hn * (x0 * Uo[DD] + x1 * Fo[DD]);
+      F[SRR] = jac * hn * (x0 * Uc[1] + x1 * Fc[1]);
+      F[LLL] = jac * hn * (x0 * Uc[2] + x1 * Fc[2]);
+      F[SZZ] = jac * hn * (x0 * Uc[3] + x1 * Fc[3]);
+      F[TAU] = jac * hn * (x0 * Uc[0] + x1 * Fc[0]);
+      F[BRR] = sqrtgam * hn * (x0 * UB[0] + x1 * FB[0]);
+      F[BPP] = sqrtgam * hn * (x0 * UB[1] + x1 * FB[1]);
+      F[BZZ] = sqrtgam * hn * (x0 * UB[2] + x1 * FB[2]);
+      // the frame's energy and the rest mass out of tau
+      U[TAU] = -Uf[0] * U[TAU] - Uf[1] * U[SRR] - Uf[2] * U[LLL] - Uf[3] * U[SZZ] - U[DDD];
+      F[TAU] = -Uf[0] * F[TAU] - Uf[1] * F[SRR] - Uf[2] * F[LLL] - Uf[3] * F[SZZ] - F[DDD];
+      U[BRR] /= r;
+      F[BRR] /= r;
+      const double *pk = (zone == 0 || zone == 1 || zone == 2) ? pL : pR;
+#pragma unroll
+      for (int q = NC; q < NQ; ++q) {
+         U[q] = pk[q] * U[DDD];
+         F[q] = pk[q] * F[DDD];
+      }
+   }
+
+   // solve_riemann (riemann.c:216-292): HLL, HLLC or HLLD flux and fan
+   // state at face speed w.  r is the face's radius (the phi scale factor).
    static DHD void riemann(const dh_params &P, const double *pL, const double *pR, double r,
                            const double *n, double w, double *F, double *U)
    {
+      if (P.riemann_solver == RS_HLLD) {
+         riemann_hlld(P, pL, pR, r, n, w, F, U);
+         return;
+      }
       GRMetric g = gr_metric(P.metric_par2, r);
       double Sl, Sr, Ss;
       vel(P, g, r, pL, pR, n, &Sl, &Sr, &Ss);

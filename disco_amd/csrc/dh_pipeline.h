@@ -92,8 +92,8 @@ inline int validate(const dh_grid *g, const dh_params &P, long *ntot_out)
       if (P.num_c != 8) return fail(DH_EINVAL, "grmhd2 needs NUM_C = 8 (got %d)", P.num_c);
       if (P.ct_mode != 2 || !P.ct)
          return fail(DH_EINVAL, "grmhd2 is supported with CT_MODE 2 and Constrained_Transport 1");
-      if (P.riemann_solver != 0 && P.riemann_solver != 1)
-         return fail(DH_EINVAL, "grmhd2 is supported with HLL and HLLC (Riemann_Solver 0, 1)");
+      if (P.riemann_solver < 0 || P.riemann_solver > 2)
+         return fail(DH_EINVAL, "Riemann_Solver %d not supported by grmhd2", P.riemann_solver);
       if (P.isothermal) return fail(DH_EINVAL, "grmhd2 is supported with the adiabatic cons2prim");
       if (P.visc_flag) return fail(DH_EINVAL, "grmhd2 has no viscous terms (Use_Viscosity 0)");
       if (nq < 8 || nq > 9) return fail(DH_EINVAL, "NUM_C+NUM_N = %d not supported", nq);

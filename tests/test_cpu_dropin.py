@@ -32,6 +32,11 @@ import build as cpu_build  # noqa: E402
     ("gr_disk", {"Num_R": 24, "Num_Z": 6}, 8),
     ("gr_disk", {"Num_R": 16, "Num_Z": 4, "Riemann_Solver": 0}, 4),
     ("gr_disk", {"Num_R": 16, "Num_Z": 4, "Mesh_Motion": 1}, 4),
+    # the relativistic HLLD: a smooth run, one where the reference falls back
+    # to the HLL state at some faces, and a strong field with many fallbacks
+    ("gr_disk", {"Num_R": 16, "Num_Z": 4, "Riemann_Solver": 2, "CFL": 0.2}, 8),
+    ("gr_disk", {"Num_R": 24, "Num_Z": 6, "Riemann_Solver": 2}, 4),
+    ("gr_disk", {"Num_R": 24, "Num_Z": 6, "Riemann_Solver": 2, "Init_Par1": 10.0, "CFL": 0.3}, 6),
 ])
 def test_driver_bit_exact_vs_reference(cname, over, steps):
     cfg = configs.get(cname)

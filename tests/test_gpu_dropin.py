@@ -30,6 +30,8 @@ HOST = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
     ("mhd_kep_p2", {"Num_R": 32, "Num_Z": 8, "Max_DT": 0.01}, 6),
     ("gr_disk", {"Num_R": 48, "Num_Z": 12}, 8),
     ("gr_disk", {"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 0}, 6),
+    ("gr_disk", {"Num_R": 16, "Num_Z": 4, "Riemann_Solver": 2, "CFL": 0.2}, 8),
+    ("gr_disk", {"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 2, "CFL": 0.2}, 4),
 ])
 def test_disco_driver_matches_reference(cname, over, steps):
     cfg = configs.get(cname)
