@@ -826,6 +826,70 @@ int disco_write_dstate(struct domain *d, const char *path)
    return 0;
 }
 
+/* Restart from a DSTATE01 file that disco_write_dstate wrote for the same
+ * mesh (the checkpoint of this host layer; the reference's HDF5 restart,
+ * Restart/h5in.c, needs libhdf5): after disco_setup, the whole device state
+ * (prim, cons, piph, dphi, wiph, Phi), the time, the step count and the
+ * planets are replaced by the file's.  The diagnostics start from zero, as
+ * after the reference's restart. */
+int disco_restart_dstate(struct domain *d, const char *path)
+{
+   struct disco_accel *A = (struct disco_accel *)d->accel;
+   long nt = A->ntot;
+   int Nr = d->Nr, Nz = d->Nz;
+   int nf = NUM_FACES == 5 ? 5 : 0;
+   FILE *f = fopen(path, "rb");
+   if (!f) return host_fail("cannot open %s", path);
+   char magic[8];
+   int hdr[16];
+   double dhdr[8];
+   int ok = fread(magic, 1, 8, f) == 8 && !memcmp(magic, "DSTATE01", 8) &&
+            fread(hdr, sizeof(int), 16, f) == 16 && fread(dhdr, sizeof(double), 8, f) == 8;
+   ok = ok && hdr[0] == NUM_Q && hdr[1] == nf && hdr[2] == Nr && hdr[3] == Nz &&
+        hdr[4] == d->NgRa && hdr[5] == d->NgRb && hdr[6] == d->NgZa && hdr[7] == d->NgZb &&
+        hdr[8] == (int)nt && hdr[9] == d->Npl && hdr[11] == NUM_C && dhdr[1] == d->phi_max;
+   int *Np = (int *)malloc(sizeof(int) * Nr * Nz);
+   double *rz = (double *)malloc(sizeof(double) * (Nr + Nz + 2));
+   ok = ok && fread(Np, sizeof(int), Nr * Nz, f) == (size_t)(Nr * Nz) &&
+        !memcmp(Np, d->Np, sizeof(int) * Nr * Nz) &&
+        fread(rz, sizeof(double), Nr + Nz + 2, f) == (size_t)(Nr + Nz + 2) &&
+        !memcmp(rz, d->r_jph - 1, sizeof(double) * (Nr + 1)) &&
+        !memcmp(rz + Nr + 1, d->z_kph - 1, sizeof(double) * (Nz + 1));
+   free(Np);
+   free(rz);
+   if (!ok) {
+      fclose(f);
+      return host_fail("%s is not a checkpoint of this mesh and module selection", path);
+   }
+   size_t nbuf = (size_t)nt * (2 * NUM_Q + 3 + nf);
+   double *buf = (double *)malloc(sizeof(double) * nbuf);
+   double pl[DH_MAX_PLANETS * DH_PLANET_STRIDE];
+   if (!buf || fread(buf, sizeof(double), nbuf, f) != nbuf ||
+       fread(pl, sizeof(double), (size_t)d->Npl * DH_PLANET_STRIDE, f) !=
+           (size_t)d->Npl * DH_PLANET_STRIDE) {
+      free(buf);
+      fclose(f);
+      return host_fail("%s is truncated", path);
+   }
+   fclose(f);
+   double *prim = buf, *cons = buf + nt * NUM_Q, *piph = cons + nt * NUM_Q;
+   double *dphi = piph + nt, *wiph = dphi + nt, *Phi = wiph + nt;
+   int rc = dh_upload_state(A->ctx, prim, cons, piph, dphi, wiph, nf ? Phi : NULL);
+   free(buf);
+   if (rc) return host_fail("dh_upload_state: %s", dh_last_error());
+   for (int p = 0; p < d->Npl; ++p) {
+      struct planet *P = d->thePlanets + p;
+      const double *v = pl + p * DH_PLANET_STRIDE;
+      P->r = v[0]; P->phi = v[1]; P->M = v[2]; P->omega = v[3]; P->vr = v[4];
+      P->eps = v[5]; P->type = (int)v[6];
+   }
+   d->t = dhdr[0];
+   d->count_steps = hdr[10];
+   zero_diagnostics(d);
+   A->host_dirty = 1;
+   return 0;
+}
+
 /* Heap-allocated, zeroed domain for foreign callers (ctypes). */
 struct domain *disco_new(void) { return (struct domain *)calloc(1, sizeof(struct domain)); }
 

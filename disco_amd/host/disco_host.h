@@ -11,6 +11,8 @@
  * Set-up (readpar.c:44-188, gridsetup.c:15-144, domain.c:26-242):
  *   disco_read_par, disco_setup_grid, disco_setup
  * Device state <-> host mirror: disco_sync_to_host, disco_export_state.
+ * Checkpoint and restart (raw DSTATE01 files): disco_write_dstate,
+ * disco_restart_dstate.
  */
 #ifndef DISCO_HOST_H
 #define DISCO_HOST_H
@@ -35,6 +37,7 @@ struct domain *disco_new(void);
 void disco_delete(struct domain *d);
 long disco_real_cells(const struct domain *d);
 int disco_write_dstate(struct domain *d, const char *path);
+int disco_restart_dstate(struct domain *d, const char *path);
 
 double getmindt(struct domain *d);
 void timestep(struct domain *d, double dt);

@@ -7,7 +7,10 @@
  * module's accumulated sums to output.diag (int32 nq, int32 Nr*Nz, double
  * t_avg, double Qrz[Nr*Nz*nq], as the reference keeps theTools).
  *
- * Usage: disco [max_steps]
+ * Usage: disco [max_steps [checkpoint]]
+ *   max_steps   stop after this many steps in all (counting those before a
+ *               restart); -1 runs to T_End
+ *   checkpoint  an output.dstate of an earlier run to restart from
  */
 #include <sys/time.h>
 
@@ -29,6 +32,7 @@ int main(int argc, char **argv)
    memset(&D, 0, sizeof D);
    if (disco_read_par(&D, "in.par")) return 1;
    if (disco_setup(&D)) return 1;
+   if (argc > 2 && disco_restart_dstate(&D, argv[2])) return 1;
    remove("abort");
    double t0 = wall();
    while (!D.final_step && (max_steps < 0 || D.count_steps < max_steps)) {

@@ -66,3 +66,46 @@ def test_driver_bit_exact_vs_reference(cname, over, steps):
     assert t_got == t_want and np.array_equal(q_got, q_want)
     if len(want.planets) > 1:
         assert np.any(q_want[:, 3] != 0.0)  # the torque column reads planet 1
+
+
+@pytest.mark.parametrize("cname,over,n1,n2", [
+    ("euler_yee", {"Num_R": 24}, 3, 3),
+    ("mhd_kep", {"Num_R": 12, "Num_Z": 6, "Max_DT": 0.02}, 3, 4),
+    ("gr_disk", {"Num_R": 16, "Num_Z": 4, "Riemann_Solver": 2, "CFL": 0.2}, 3, 3),
+])
+def test_checkpoint_restart_continues_the_run(cname, over, n1, n2):
+    """n1 + n2 steps in one run equal n1 steps, a checkpoint
+    (disco_write_dstate) and n2 more steps after disco_restart_dstate in a
+    new process: the checkpoint holds the whole state of the step."""
+    cfg = configs.get(cname)
+    exe = cpu_build.build_driver(cname)
+    par = cfg.par_dict(**over)
+    straight, split = tempfile.mkdtemp(prefix="ckA_"), tempfile.mkdtemp(prefix="ckB_")
+    for d in (straight, split):
+        state_io.write_par(d, par)
+    subprocess.run([exe, str(n1 + n2)], cwd=straight, check=True, capture_output=True, timeout=300)
+    subprocess.run([exe, str(n1)], cwd=split, check=True, capture_output=True, timeout=300)
+    os.rename(os.path.join(split, "output.dstate"), os.path.join(split, "ckpt.dstate"))
+    subprocess.run([exe, str(n1 + n2), "ckpt.dstate"], cwd=split, check=True, capture_output=True,
+                   timeout=300)
+    a = state_io.read_state(os.path.join(straight, "output.dstate"))
+    b = state_io.read_state(os.path.join(split, "output.dstate"))
+    assert a.step == b.step == n1 + n2 and a.t == b.t
+    m = a.real_mask()
+    for key in ("prim", "cons"):
+        assert np.array_equal(getattr(a, key)[:, m], getattr(b, key)[:, m]), key
+    assert np.array_equal(a.piph, b.piph)
+    if a.nf:
+        assert np.array_equal(a.phi_flux[:, m], b.phi_flux[:, m])
+
+
+def test_restart_refuses_a_foreign_checkpoint():
+    cfg = configs.get("euler_yee")
+    exe = cpu_build.build_driver("euler_yee")
+    d1, d2 = tempfile.mkdtemp(prefix="ckC_"), tempfile.mkdtemp(prefix="ckD_")
+    state_io.write_par(d1, cfg.par_dict(Num_R=24))
+    state_io.write_par(d2, cfg.par_dict(Num_R=16))
+    subprocess.run([exe, "1"], cwd=d1, check=True, capture_output=True, timeout=300)
+    r = subprocess.run([exe, "2", os.path.join(d1, "output.dstate")], cwd=d2,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0 and "not a checkpoint of this mesh" in r.stderr
