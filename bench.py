@@ -102,8 +102,13 @@ def kernel_bytes(name: str, nq: int, mhd: bool, nt: int, nr: int, nfr: int, nfz:
         return d * (12.5 + 4) * nt + 2 * steps + i * nt + nfa * (2 * i + 1)
     if name in ("line_steps_r", "line_steps_z"):
         return nf * (d + 2 * i)
+    if name == "edge_segments":
+        # face list, merge steps and line offsets in, the four cells' phi
+        # edges and E_phi in, one E dl dt per line step (about two per face) out
+        return nfa * (2 * i + 2 * i) + d * 6 * nt + d * 2 * nfa
     if name == "edge_adjust":
-        return nfa * (2 * i + 1 + 2 * i + 2 * d) + d * 5 * nt
+        # face list and merge steps in, the stored segments in, Phi in and out
+        return nfa * (2 * i + 1 + 2 * i + 2 * d) + d * 4 * nfa
     if name == "update":
         # cons in/out, rkcons on the second substep only, prim in for the
         # sources (+ prim out for hydro), phi/r/z face deltas, geometry in
@@ -119,6 +124,8 @@ def kernel_bytes(name: str, nq: int, mhd: bool, nt: int, nr: int, nfr: int, nfz:
         return nfa
     if name == "mindt":
         return d * (nq * nr + 3 * nt) + i * nt
+    if name == "diagnostics":          # prim (rho..B), piph, dphi in; Qrz out
+        return d * (8 + 2) * nt + i * nt
     if name == "bc_scan":               # one z side: positions of both layers' rings
         return 2 * zlayer * (4 * d + i)
     if name == "bc_fill":               # one z ghost layer from the layer inside it

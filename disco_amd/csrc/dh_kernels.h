@@ -1221,13 +1221,17 @@ struct KLineSteps {
    }
 };
 
-struct KEdgeAdjust {
+// The segments of every edge line, each formed once: a line's steps are
+// shared out among the faces of its first list (each step has exactly one
+// current face in each list), and each of those faces stores E dl dt of its
+// steps at seg[lbase[JK] + step], JK the first list's pair.
+struct KEdgeSegments {
    Mesh m;
    double dt;
    Faces Fr, Fz;
-   const int *estep_r, *estep_z;
+   const int *estep_r, *estep_z, *lbase_r, *lbase_z;
    const double *piph, *dphi, *Eown;
-   double *Phi;
+   double *seg_r, *seg_z;
    DHD void operator()(long t) const
    {
       size_t nt = (size_t)m.ntot;
@@ -1237,54 +1241,82 @@ struct KEdgeAdjust {
       int f = (int)(dim == 1 ? t : t - Fr.n);
       int JK = F.pf[f];
       Pair pr = pair_of(m, dim, JK);
+      EdgeRole er = edge_role(m, dim, pr.j, pr.k, 1);  // the role where F's list is first
+      if (!er.ok) return;
+      double *seg = (dim == 1 ? seg_r : seg_z) + (dim == 1 ? lbase_r : lbase_z)[JK];
       int base = F.fI[JK];
       int N = F.fI[JK + 1] - base;
       int x = f - base;
       int fp = base + (x == 0 ? N - 1 : x - 1);
-      int fL = F.L[f], fR = F.R[f];
-      size_t slot = phi_slot(nt, dim, F.LR[f], fL, fR);
+      int obase = F.fI[er.oJK];
+      int oN = F.fI[er.oJK + 1] - obase;
+      int h0, h1 = estep[(size_t)F.n + f], t0, t1;
+      range_from_steps(h1, estep[(size_t)F.n + fp], x == 0, N + oN, &h0, &t0, &t1);
+      double rline = m.r_jph[er.lj];
+      // Corner positions: an r face (c, cR) is the line's first face and
+      // the facing one (cU, cUR); a z face (c, cU), the facing one (cR, cUR).
+      int oL = dim == 1 ? 2 : 1;
+      EdgeCell c0 = edge_cell(piph, dphi, Eown, nt, F.L[f], 0);
+      EdgeCell c1 = edge_cell(piph, dphi, Eown, nt, F.R[f], dim == 1 ? 1 : 2);
+      for (int part = 0; part < 2; ++part) {
+         int lo = part == 0 ? h0 : t0, hi = part == 0 ? h1 : t1;
+         // while this face is current the merge has consumed x of its list
+         // (all N in the tail): the facing face at step e is element e - x
+         // (e - N) of the other list
+         int consumed = part == 0 ? x : N;
+         for (int e = lo; e <= hi; ++e) {
+            int y = e - consumed;
+            int g = obase + (y == oN ? 0 : y);
+            EdgeCell c2 = edge_cell(piph, dphi, Eown, nt, F.L[g], oL);
+            EdgeCell c3 = edge_cell(piph, dphi, Eown, nt, F.R[g], 3);
+            double E, dl;
+            if (dim == 1) edge_segment(c0, c1, c2, c3, rline, m.Pmax, &E, &dl);
+            else edge_segment(c0, c2, c1, c3, rline, m.Pmax, &E, &dl);
+            seg[e] = E * dl * dt;
+         }
+      }
+   }
+};
+
+// make_edge_adjust (bfields.c:888-1127) per face: Phi of the face's owner
+// slot takes +-E dl dt of each segment of its two edge lines while it is
+// current, in the reference's line and step order, from the stored
+// segments.
+struct KEdgeAdjust {
+   Mesh m;
+   Faces Fr, Fz;
+   const int *estep_r, *estep_z, *lbase_r, *lbase_z;
+   const double *seg_r, *seg_z;
+   double *Phi;
+   DHD void operator()(long t) const
+   {
+      size_t nt = (size_t)m.ntot;
+      int dim = t < Fr.n ? 1 : 2;
+      const Faces &F = dim == 1 ? Fr : Fz;
+      const int *estep = dim == 1 ? estep_r : estep_z;
+      const int *lbase = dim == 1 ? lbase_r : lbase_z;
+      const double *segd = dim == 1 ? seg_r : seg_z;
+      int f = (int)(dim == 1 ? t : t - Fr.n);
+      int JK = F.pf[f];
+      Pair pr = pair_of(m, dim, JK);
+      int base = F.fI[JK];
+      int N = F.fI[JK + 1] - base;
+      int x = f - base;
+      int fp = base + (x == 0 ? N - 1 : x - 1);
+      size_t slot = phi_slot(nt, dim, F.LR[f], F.L[f], F.R[f]);
       double v = Phi[slot];
       for (int role = 0; role < 2; ++role) {
          EdgeRole er = edge_role(m, dim, pr.j, pr.k, role);
          if (!er.ok) continue;
-         int obase = F.fI[er.oJK];
-         int oN = F.fI[er.oJK + 1] - obase;
+         int oN = F.fI[er.oJK + 1] - F.fI[er.oJK];
+         const double *seg = segd + lbase[er.self_first ? JK : er.oJK];
          int h0, h1 = estep[(size_t)role * F.n + f], t0, t1;
          range_from_steps(h1, estep[(size_t)role * F.n + fp], x == 0, N + oN, &h0, &t0, &t1);
-         double rline = m.r_jph[er.lj];
-         // Corner positions of the face's L and R cells and of the facing
-         // face's: an r face (c, cR) is the line's first face, (cU, cUR) its
-         // second; a z face (c, cU) the first, (cR, cUR) the second.
-         int sL = er.self_first ? 0 : (dim == 1 ? 2 : 1);
-         int sR = er.self_first ? (dim == 1 ? 1 : 2) : 3;
-         int oL = er.self_first ? (dim == 1 ? 2 : 1) : 0;
-         int oR = er.self_first ? 3 : (dim == 1 ? 1 : 2);
-         EdgeCell own_L = edge_cell(piph, dphi, Eown, nt, fL, sL);
-         EdgeCell own_R = edge_cell(piph, dphi, Eown, nt, fR, sR);
          for (int part = 0; part < 2; ++part) {
             int lo = part == 0 ? h0 : t0, hi = part == 0 ? h1 : t1;
-            // While this face is current, the merge has consumed x of its
-            // list (all N in the tail), so the facing face at step e is
-            // element e - x (e - N) of the other list.
-            int consumed = part == 0 ? x : N;
             for (int e = lo; e <= hi; ++e) {
-               int y = e - consumed;
-               int g = obase + (y == oN ? 0 : y);
-               EdgeCell other_L = edge_cell(piph, dphi, Eown, nt, F.L[g], oL);
-               EdgeCell other_R = edge_cell(piph, dphi, Eown, nt, F.R[g], oR);
-               auto at = [&](int p) {
-                  EdgeCell r;
-                  r.up = p == sL ? own_L.up : p == sR ? own_R.up : p == oL ? other_L.up : other_R.up;
-                  r.lo = p == sL ? own_L.lo : p == sR ? own_R.lo : p == oL ? other_L.lo : other_R.lo;
-                  r.ea = p == sL ? own_L.ea : p == sR ? own_R.ea : p == oL ? other_L.ea : other_R.ea;
-                  r.eb = p == sL ? own_L.eb : p == sR ? own_R.eb : p == oL ? other_L.eb : other_R.eb;
-                  return r;
-               };
-               double E, dl;
-               edge_segment(at(0), at(1), at(2), at(3), rline, m.Pmax, &E, &dl);
-               double Edldt = E * dl * dt;
-               if (er.sgn > 0) v += Edldt;
-               else v -= Edldt;
+               if (er.sgn > 0) v += seg[e];
+               else v -= seg[e];
             }
          }
       }

@@ -60,6 +60,8 @@ struct Buffers {
    double *Epre = nullptr, *Eo = nullptr, *Eown = nullptr, *Ecarry = nullptr;
    signed char *flag[2] = {nullptr, nullptr};
    int *estep[2] = {nullptr, nullptr};  // merge steps of each face on its two edge lines
+   int *lbase[2] = {nullptr, nullptr};  // first segment of the edge line whose first list is pair JK
+   double *seg[2] = {nullptr, nullptr}; // E dl dt of every segment of every edge line
    int *bcstart = nullptr;          // where each ghost cell's overlap sum starts
    int bc_maxnp[4][8] = {{0}};      // largest Np of each side's ghost layers
    PlanetSet pl[2];
@@ -226,6 +228,20 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
          DH_TRY(alloc_n(x, &b.fB[d], sizeof(double) * n));
          DH_TRY(alloc_n(x, &b.flag[d], sizeof(signed char) * n));
          DH_TRY(alloc_n(x, &b.estep[d], sizeof(int) * 2 * n));
+         // edge lines: the line whose first list is pair P holds the merge
+         // of P's faces with the next pair's (r faces: k + 1, z faces: j + 1)
+         std::vector<int> lb(F.npairs + 1, 0);
+         long nseg = 0;
+         for (int P = 0; P < F.npairs; ++P) {
+            lb[P] = (int)nseg;
+            Pair pr = pair_of(m, dim, P);
+            EdgeRole er = edge_role(m, dim, pr.j, pr.k, 1);
+            if (er.ok) nseg += (fI[P + 1] - fI[P]) + (fI[er.oJK + 1] - fI[er.oJK]);
+         }
+         lb[F.npairs] = (int)nseg;
+         DH_TRY(alloc_n(x, &b.lbase[d], sizeof(int) * lb.size()));
+         DH_TRY(x.put(b.lbase[d], lb.data(), sizeof(int) * lb.size()));
+         DH_TRY(alloc_n(x, &b.seg[d], sizeof(double) * (size_t)(nseg > 0 ? nseg : 1)));
          DH_TRY(x.zero(b.emf_f[d], sizeof(double) * 3 * n));
       }
    }
@@ -402,7 +418,11 @@ struct Engine {
             if (dim_active(m, 1))
                DH_TRY(x.run("line_steps_r", KLineSteps<1>{m, b.F[0], b.estep[0]}, b.F[0].n));
             DH_TRY(x.run("line_steps_z", KLineSteps<2>{m, b.F[1], b.estep[1]}, b.F[1].n));
-            KEdgeAdjust ke{m, dt, b.F[0], b.F[1], b.estep[0], b.estep[1], b.piph, b.dphi, b.Eown, b.Phi};
+            KEdgeSegments ks{m, dt, b.F[0], b.F[1], b.estep[0], b.estep[1], b.lbase[0], b.lbase[1],
+                             b.piph, b.dphi, b.Eown, b.seg[0], b.seg[1]};
+            DH_TRY(x.run("edge_segments", ks, (long)b.F[0].n + b.F[1].n));
+            KEdgeAdjust ke{m, b.F[0], b.F[1], b.estep[0], b.estep[1], b.lbase[0], b.lbase[1],
+                           b.seg[0], b.seg[1], b.Phi};
             DH_TRY(x.run("edge_adjust", ke, (long)b.F[0].n + b.F[1].n));
          }
       }

@@ -35,13 +35,14 @@ _NAMES = [
     (r"KEdgesFinal", "edges_final"),
     (r"KUpdateB", "update_B"),
     (r"KLineSteps<1>", "line_steps_r"), (r"KLineSteps<2>", "line_steps_z"),
-    (r"KEdgeAdjust", "edge_adjust"),
+    (r"KEdgeSegments", "edge_segments"), (r"KEdgeAdjust", "edge_adjust"),
     (r"KUpdate<", "update"),
     (r"KFlipFlags<1>", "flip_flags_r"), (r"KFlipFlags<2>", "flip_flags_z"),
     (r"k_flips", "flips"),
     (r"KBfacesC2p<", "bfaces_c2p"),
     (r"KBcScan", "bc_scan"), (r"KBcFill<", "bc_fill"),
-    (r"k_mindt<", "mindt"),
+    (r"k_mindt<", "mindt"), (r"k_bc_scan", "bc_scan"), (r"k_diag<", "diagnostics"),
+    (r"KDiagAvg", "diag_avg"),
     (r"KWcell<", "wcell"), (r"KWcellAvg", "wcell"),
     (r"KInitCells<", "init"),
 ]
