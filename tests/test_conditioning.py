@@ -125,3 +125,26 @@ def test_gpu_parity_cases_are_well_conditioned(cname, over, steps):
     out = _perturbed(cfg, par, a, dts, "pow,sincos")
     drift = max(field_errors(out, ref).values())
     assert 0.0 < drift < 1e-11, drift
+
+
+@pytest.mark.parametrize("solver,steps", [(0, 3), (2, 3)])
+def test_grmhd_hll_and_hlld_are_well_conditioned(solver, steps):
+    """The GRMHD disk with HLL or the relativistic HLLD over three steps moves
+    by less than 1e-11 under the one-ulp change."""
+    cfg, par, a, dts, ref = _reference_run(
+        "gr_disk", {"Num_R": 32, "Num_Z": 8, "Riemann_Solver": solver}, steps)
+    drift = max(field_errors(_perturbed(cfg, par, a, dts, "pow,sincos"), ref).values())
+    assert drift < 1e-11, drift
+
+
+def test_grmhd_hllc_field_flux_is_discontinuous():
+    """GR HLLC (riemann.c:262-290 with grmhd2.c:182-251) leaves the field of
+    its star state at zero, so the field flux jumps when a face's contact
+    speed crosses the face speed -- in a disk the radial contact speed is
+    close to the r faces' zero speed.  A one-ulp change moves the field by
+    more than 1e-4 in two steps, while one step from a common state moves
+    every field by round-off only: GR HLLC is compared step by step."""
+    cfg, par, a, dts, ref = _reference_run(
+        "gr_disk", {"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 1}, 2)
+    e = field_errors(_perturbed(cfg, par, a, dts, "pow,sincos"), ref)
+    assert max(e[f"cons{q}"] for q in (5, 6, 7)) > 1e-4

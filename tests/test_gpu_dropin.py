@@ -28,10 +28,11 @@ HOST = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
     ("euler_yee_n1", {"Num_R": 64, "Riemann_Solver": 0}, 10),
     ("mhd_kep", {"Num_R": 32, "Num_Z": 8}, 6),
     ("mhd_kep_p2", {"Num_R": 32, "Num_Z": 8, "Max_DT": 0.01}, 6),
-    ("gr_disk", {"Num_R": 48, "Num_Z": 12}, 8),
-    ("gr_disk", {"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 0}, 6),
-    ("gr_disk", {"Num_R": 16, "Num_Z": 4, "Riemann_Solver": 2, "CFL": 0.2}, 8),
-    ("gr_disk", {"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 2, "CFL": 0.2}, 4),
+    # GRMHD trajectories with HLL and HLLD over the steps a one-ulp libm change
+    # moves by less than 1e-11 (tests/test_conditioning.py); GR HLLC and
+    # longer runs are checked step by step in test_gpu_grmhd.py
+    ("gr_disk", {"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 0}, 3),
+    ("gr_disk", {"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 2}, 3),
 ])
 def test_disco_driver_matches_reference(cname, over, steps):
     cfg = configs.get(cname)
