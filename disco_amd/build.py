@@ -62,7 +62,21 @@ HOST_CONFIGS = {
     "gr_disk": dict(hydro="grmhd2", initial="gr_disk", planets="none",
                     boundary="fixed_r_outflow_z", analysis="diag_euler",
                     num_c=8, num_n=0, ct_mode=2),
+    "gr_disk3d": dict(hydro="grmhd2", metric="schw_ks_rpz", initial="gr_disk", planets="none",
+                      boundary="fixed_r_outflow_z", analysis="diag_euler",
+                      num_c=8, num_n=0, ct_mode=2),
 }
+
+
+def host_defines(c: dict) -> list:
+    """The -D flags of one module selection (the reference's Makefile_opt.in
+    variables and paul.h sizes)."""
+    defs = [f"-DNUM_C={c['num_c']}", f"-DNUM_N={c['num_n']}", f"-DCT_MODE={c['ct_mode']}",
+            f'-DBOUNDARY="{c["boundary"]}"', f'-DINITIAL="{c["initial"]}"',
+            f'-DANALYSIS="{c["analysis"]}"', f'-DHYDRO="{c["hydro"]}"']
+    if "metric" in c:
+        defs.append(f'-DMETRIC="{c["metric"]}"')
+    return defs
 
 
 def build_host(configs=None) -> None:
@@ -73,9 +87,7 @@ def build_host(configs=None) -> None:
     for name, c in HOST_CONFIGS.items():
         if configs and name not in configs:
             continue
-        defs = [f"-DNUM_C={c['num_c']}", f"-DNUM_N={c['num_n']}", f"-DCT_MODE={c['ct_mode']}",
-                f'-DBOUNDARY="{c["boundary"]}"', f'-DINITIAL="{c["initial"]}"',
-                f'-DANALYSIS="{c["analysis"]}"', f'-DHYDRO="{c["hydro"]}"']
+        defs = host_defines(c)
         srcs = [os.path.join(host, "disco_host.c"),
                 os.path.join(plug, "Initial", c["initial"] + ".c"),
                 os.path.join(plug, "Planets", c["planets"] + ".c")]

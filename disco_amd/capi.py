@@ -34,13 +34,14 @@ _PARAM_INTS = ["num_c", "num_n", "ct_mode", "hydro", "riemann_solver", "mesh_mot
                "exact_mesh_omega", "energy_omega", "timestep_mode", "ct", "isothermal",
                "cs2_profile", "visc_flag", "alpha_flag", "rot_frame", "grav2d",
                "nobc_rmin", "nobc_rmax", "nobc_zmin", "nobc_zmax", "r_periodic",
-               "z_periodic", "bc_rmin", "bc_rmax", "bc_zmin", "bc_zmax", "nozzle_type"]
+               "z_periodic", "bc_rmin", "bc_rmax", "bc_zmin", "bc_zmax", "nozzle_type",
+               "metric"]
 _PARAM_DOUBLES = ["gamma_law", "plm", "cfl", "max_dt", "rho_floor", "p_floor",
                   "viscosity", "mach", "metric_par2", "rot_omega", "rot_d"]
 
 
 class DhParams(C.Structure):
-    _fields_ = ([(n, C.c_int) for n in _PARAM_INTS] + [("pad_", C.c_int * 3)]
+    _fields_ = ([(n, C.c_int) for n in _PARAM_INTS] + [("pad_", C.c_int * 2)]
                 + [(n, C.c_double) for n in _PARAM_DOUBLES]
                 + [("init_par", C.c_double * 9), ("phi_order", C.c_double)])
 
@@ -75,6 +76,7 @@ def make_params(cfg: "cfgmod.Config", par: Dict[str, object]) -> DhParams:
     p.num_c, p.num_n, p.ct_mode = cfg.num_c, cfg.num_n, cfg.ct_mode
     p.hydro = {"euler": cfgmod.HYDRO_EULER, "mhd": cfgmod.HYDRO_MHD,
                "grmhd2": cfgmod.HYDRO_GRMHD}[cfg.hydro]
+    p.metric = cfgmod.METRICS[cfg.metric_module]
     for k, f in _PAR_INT_KEYS.items():
         # readpar.c:31-36 parses integers through %lf and a cast.
         setattr(p, f, int(float(full[k])))

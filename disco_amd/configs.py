@@ -13,6 +13,8 @@ from typing import Dict, Optional, Tuple
 # DH_BC_* (include/disco_hip.h)
 BC_NONE, BC_HELD, BC_ZEROGRAD, BC_DIODE = 0, 1, 2, 3
 HYDRO_EULER, HYDRO_MHD, HYDRO_GRMHD = 0, 1, 2
+# DH_METRIC_* by Metric module name
+METRICS = {"flat_rpz": 0, "schw_ks_eq_rpz": 1, "schw_ks_rpz": 2}
 
 
 @dataclasses.dataclass(frozen=True)
@@ -27,6 +29,13 @@ class Config:
     ct_mode: int
     par: Tuple[Tuple[str, object], ...]
     analysis: str = "diag_euler"  # Diagnostics module (ANALYSIS)
+    metric: str = ""           # Metric module (METRIC); "" is the hydro module's default
+
+    @property
+    def metric_module(self) -> str:
+        if self.metric:
+            return self.metric
+        return "schw_ks_eq_rpz" if self.hydro == "grmhd2" else "flat_rpz"
 
     def par_dict(self, **overrides) -> Dict[str, object]:
         d = dict(self.par)
@@ -118,6 +127,12 @@ CONFIGS["gr_disk"] = _cfg(
     Riemann_Solver=1, Mesh_Motion=4, Exact_Mesh_Omega=2, Energy_Omega=0,
     Constrained_Transport=1, Isothermal=0, Metric_Par2=1.0, Adiabatic_Index=4.0 / 3.0,
     Init_Par1=100.0, Density_Floor=1e-8, Pressure_Floor=1e-10)
+
+# The GRMHD disk in the full (r, z) Schwarzschild metric (schw_ks_rpz): off
+# the equator gravity has a vertical component, so the slab is not in
+# equilibrium and falls towards z = 0.
+CONFIGS["gr_disk3d"] = dataclasses.replace(CONFIGS["gr_disk"], name="gr_disk3d",
+                                           metric="schw_ks_rpz")
 
 # The disk with a companion of mass ratio 1e-3 at r = 1.25 (Planets/central.c
 # reads Mass_Ratio) and the Euler diagnostics, which read planet 1.

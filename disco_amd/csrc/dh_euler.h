@@ -17,7 +17,7 @@ struct Euler {
 
    // prim2cons (euler.c:36-62)
    static DHD void prim2cons(const dh_params &P, const double *prim, double *cons,
-                             double r, double dV)
+                             double r, double /*z*/, double dV)
    {
       double rho = prim[RHO];
       double Pp = prim[PPP];
@@ -68,7 +68,7 @@ struct Euler {
    // cons2prim (euler.c:102-139), with density/pressure floors and the
    // isothermal override.
    static DHD void cons2prim(const dh_params &P, const double *cons, double *prim,
-                             double r, double dV)
+                             double r, double /*z*/, double dV)
    {
       double rho = cons[DDD] / dV;
       if (rho < P.rho_floor) rho = P.rho_floor;
@@ -258,27 +258,30 @@ struct Euler {
    }
 
    // get_omega (euler.c:32-34): the cell's own angular velocity.
-   static DHD double omega(const dh_params &, const double *prim, double) { return prim[UPP]; }
+   static DHD double omega(const dh_params &, const double *prim, double, double)
+   {
+      return prim[UPP];
+   }
 
    // solve_riemann (riemann.c:216-318), HLL/HLLC branch: the interface flux
    // F and the state U* it is associated with (the caller forms F - w U*).
    static DHD void riemann(const dh_params &P, const double *pL, const double *pR,
-                           double r, const double *n, double w, double *Flux,
+                           double r, double z, const double *n, double w, double *Flux,
                            double *Ustr)
    {
       double Sl, Sr, Ss;
       vel(P, pL, pR, &Sl, &Sr, &Ss, n, r);
       if (w < Sl) {
          flux(P, pL, Flux, r, n);
-         prim2cons(P, pL, Ustr, r, 1.0);
+         prim2cons(P, pL, Ustr, r, z, 1.0);
       } else if (w > Sr) {
          flux(P, pR, Flux, r, n);
-         prim2cons(P, pR, Ustr, r, 1.0);
+         prim2cons(P, pR, Ustr, r, z, 1.0);
       } else if (P.riemann_solver == RS_HLL) {
          double Fl[NQ], Fr[NQ], Ul[NQ], Ur[NQ];
          double aL = Sr, aR = -Sl;
-         prim2cons(P, pL, Ul, r, 1.0);
-         prim2cons(P, pR, Ur, r, 1.0);
+         prim2cons(P, pL, Ul, r, z, 1.0);
+         prim2cons(P, pR, Ur, r, z, 1.0);
          flux(P, pL, Fl, r, n);
          flux(P, pR, Fr, r, n);
 #pragma unroll
@@ -290,7 +293,7 @@ struct Euler {
          double Uk[NQ], Fk[NQ];
          const double *pk = (w < Ss) ? pL : pR;
          double Sk = (w < Ss) ? Sl : Sr;
-         prim2cons(P, pk, Uk, r, 1.0);
+         prim2cons(P, pk, Uk, r, z, 1.0);
          getUstar(P, pk, Ustr, r, Sk, Ss, n);
          flux(P, pk, Fk, r, n);
 #pragma unroll

@@ -1,15 +1,15 @@
 // dh_grmhd.h -- general-relativistic MHD on the moving cylindrical mesh
 // (device side of the reference Hydro/grmhd2.c) in the Schwarzschild metric
-// in Kerr-Schild coordinates evaluated on the equator
-// (Hydro/Metric/schw_ks_eq_rpz.c) with the Eulerian observer frame
+// in Kerr-Schild cylindrical coordinates (t, r, phi, z), either at (r, z)
+// (Hydro/Metric/schw_ks_rpz.c) or evaluated on the equator, z = 0
+// (Hydro/Metric/schw_ks_eq_rpz.c), with the Eulerian observer frame
 // (Hydro/Frame/normal.c).
 //
 // Primitive layout {rho, P, u_r, u_phi, u_z, B^r, B^phi, B^z, passives...}
 // with covariant four-velocity components u_i and the densitised field;
 // conserved {D, tau, S_r, S_phi, S_z, B^r dV / r, B^phi dV, B^z dV, ...}
-// (grmhd2.c:88-180).  Every metric quantity of this metric depends on the
-// cylindrical radius only, so a module function needs r and nothing else of
-// the position.
+// (grmhd2.c:88-180).  The metric is axisymmetric, so a module function needs
+// the position's r and z (the reference's x[0] and x[2]).
 //
 // Riemann solvers: HLL and HLLC (riemann.c:216-292 with grmhd2.c's vel,
 // flux, prim2cons and getUstar) and the relativistic HLLD of hlld_grmhd.c
@@ -22,18 +22,27 @@
 
 namespace dh {
 
-// Metric (schw_ks_eq_rpz.c:19-189) and frame (normal.c:9-42) at radius r,
-// with M = Metric_Par2.  gam / igam are the spatial metric and its inverse,
-// be the shift, al the lapse, U the frame four-velocity.
+// Metric (schw_ks_rpz.c:20-189, schw_ks_eq_rpz.c:19-189) and frame
+// (normal.c:9-42) at (r, z), with M = Metric_Par2.  gam / igam are the spatial
+// metric and its inverse, be the shift, al the lapse, U the frame
+// four-velocity.  The equatorial module is the same metric at z = 0: its
+// lapse takes R = r where this one takes sqrt(r^2 + z^2), which at z = 0 is r
+// exactly (the square root of a rounded square is the number itself).
 struct GRMetric {
    double al, be[3], gam[9], igam[9], jac, sqrtgam, U[4];
 };
 
-DHD double gr_lapse(double M, double r) { return 1.0 / sqrt(1.0 + 2 * M / r); }
+// The z at which the metric module evaluates a position.
+DHD double gr_z(const dh_params &P, double z) { return P.metric == DH_METRIC_SCHW_KS ? z : 0.0; }
 
-DHD void gr_shift(double M, double r, double *b)
+DHD double gr_lapse(double M, double r, double z)
 {
-   double z = 0.0;
+   double R = sqrt(r * r + z * z);
+   return 1.0 / sqrt(1.0 + 2 * M / R);
+}
+
+DHD void gr_shift(double M, double r, double z, double *b)
+{
    double R = sqrt(r * r + z * z);
    double a2 = 1.0 / (1.0 + 2 * M / R);
    b[0] = 2 * M * r / (R * R) * a2;
@@ -41,9 +50,8 @@ DHD void gr_shift(double M, double r, double *b)
    b[2] = 2 * M * z / (R * R) * a2;
 }
 
-DHD void gr_gam(double M, double r, double *g)
+DHD void gr_gam(double M, double r, double z, double *g)
 {
-   double z = 0.0;
    double R = sqrt(r * r + z * z);
    double H = 2 * M / R;
    double st = r / R, ct = z / R;
@@ -58,9 +66,8 @@ DHD void gr_gam(double M, double r, double *g)
    g[8] = 1.0 + H * ct * ct;
 }
 
-DHD void gr_igam(double M, double r, double *g)
+DHD void gr_igam(double M, double r, double z, double *g)
 {
-   double z = 0.0;
    double R = sqrt(r * r + z * z);
    double H = 2 * M / R;
    double st = r / R, ct = z / R;
@@ -77,11 +84,10 @@ DHD void gr_igam(double M, double r, double *g)
 }
 
 // d g_{mu nu} / d x^i for i = 1 (r) and 3 (z); zero otherwise.
-DHD void gr_der_g(double M, double r, int i, double *dg)
+DHD void gr_der_g(double M, double r, double z, int i, double *dg)
 {
    for (int mu = 0; mu < 16; ++mu) dg[mu] = 0.0;
    if (i != 1 && i != 3) return;
-   double z = 0.0;
    double R = sqrt(r * r + z * z);
    double st = r / R, ct = z / R;
    double MoR2 = M / (R * R), MoR3 = M / (R * R * R), MoR4 = M / (R * R * R * R);
@@ -109,9 +115,8 @@ DHD void gr_der_g(double M, double r, int i, double *dg)
 // Killing directions of the metric: t and phi (x^0, x^2).
 DHD bool gr_killing(int mu) { return !(mu == 1 || mu == 3); }
 
-DHD void gr_der_lapse(double M, double r, double *da)
+DHD void gr_der_lapse(double M, double r, double z, double *da)
 {
-   double z = 0.0;
    double R = sqrt(r * r + z * z);
    double st = r / R, ct = z / R;
    double MoR2 = M / (R * R);
@@ -122,9 +127,8 @@ DHD void gr_der_lapse(double M, double r, double *da)
    da[3] = a * a * a * MoR2 * ct;
 }
 
-DHD void gr_der_shift(double M, double r, double *db)
+DHD void gr_der_shift(double M, double r, double z, double *db)
 {
-   double z = 0.0;
    double R = sqrt(r * r + z * z);
    double st = r / R, ct = z / R;
    double denom = 1.0 / (R * R + 2 * M * R);
@@ -140,24 +144,24 @@ DHD void gr_der_shift(double M, double r, double *db)
 }
 
 // The Eulerian observer (normal.c:9-42).
-DHD void gr_frame_U(double M, double r, double *U)
+DHD void gr_frame_U(double M, double r, double z, double *U)
 {
    double b[3];
-   double ia = 1.0 / gr_lapse(M, r);
-   gr_shift(M, r, b);
+   double ia = 1.0 / gr_lapse(M, r, z);
+   gr_shift(M, r, z, b);
    U[0] = ia;
    U[1] = -ia * b[0];
    U[2] = -ia * b[1];
    U[3] = -ia * b[2];
 }
 
-DHD void gr_frame_der_U(double M, double r, double *dU)
+DHD void gr_frame_der_U(double M, double r, double z, double *dU)
 {
    double da[4], b[3], db[12];
-   double ia = 1.0 / gr_lapse(M, r);
-   gr_der_lapse(M, r, da);
-   gr_shift(M, r, b);
-   gr_der_shift(M, r, db);
+   double ia = 1.0 / gr_lapse(M, r, z);
+   gr_der_lapse(M, r, z, da);
+   gr_shift(M, r, z, b);
+   gr_der_shift(M, r, z, db);
    for (int i = 0; i < 16; ++i) dU[i] = 0.0;
    for (int i = 0; i < 4; ++i) {
       if (gr_killing(i)) continue;
@@ -166,16 +170,19 @@ DHD void gr_frame_der_U(double M, double r, double *dU)
    }
 }
 
-DHD GRMetric gr_metric(double M, double r)
+// The metric at a position (r, z); z is the reference's x[2], which the
+// equatorial module ignores.
+DHD GRMetric gr_metric(const dh_params &P, double r, double z_pos)
 {
+   double M = P.metric_par2, z = gr_z(P, z_pos);
    GRMetric g;
-   g.al = gr_lapse(M, r);
-   gr_shift(M, r, g.be);
-   gr_gam(M, r, g.gam);
-   gr_igam(M, r, g.igam);
+   g.al = gr_lapse(M, r, z);
+   gr_shift(M, r, z, g.be);
+   gr_gam(M, r, z, g.gam);
+   gr_igam(M, r, z, g.igam);
    g.jac = r / r;  // metric_jacobian (= r) over r
    g.sqrtgam = g.jac / g.al;
-   gr_frame_U(M, r, g.U);
+   gr_frame_U(M, r, z, g.U);
    return g;
 }
 
@@ -235,10 +242,10 @@ struct GRMHD {
 
    // prim2cons (grmhd2.c:88-180)
    static DHD void prim2cons(const dh_params &P, const double *prim, double *cons, double r,
-                             double dV)
+                             double z, double dV)
    {
       double gl = P.gamma_law;
-      GRMetric g = gr_metric(P.metric_par2, r);
+      GRMetric g = gr_metric(P, r, z);
       GRState s = gr_state(gl, g, r, prim);
       double rhoh = s.rho + gl / (gl - 1.0) * s.Pp + s.b2;
       double rhoe = s.Pp / (gl - 1.0);
@@ -436,10 +443,11 @@ struct GRMHD {
    // get_Ustar_HLLD (hlld_grmhd.c:90-368): the relativistic HLLD fan solved
    // in the face's orthonormal frame, moved back to coordinate components.
    static DHD void riemann_hlld(const dh_params &P, const double *pL, const double *pR,
-                                double r, const double *n, double w, double *F, double *U)
+                                double r, double z, const double *n, double w, double *F,
+                                double *U)
    {
       double gl = P.gamma_law;
-      GRMetric g = gr_metric(P.metric_par2, r);
+      GRMetric g = gr_metric(P, r, z);
       double al = g.al, jac = g.jac, sqrtgam = g.sqrtgam;
       const double *be = g.be, *gam = g.gam, *igam = g.igam, *Uf = g.U;
       double rhoL = pL[RHO], PL = pL[PPP];
@@ -545,28 +553,29 @@ struct GRMHD {
    }
 
    // solve_riemann (riemann.c:216-292): HLL, HLLC or HLLD flux and fan
-   // state at face speed w.  r is the face's radius (the phi scale factor).
+   // state at face speed w.  (r, z) is the face's position; r is also the
+   // phi scale factor.
    static DHD void riemann(const dh_params &P, const double *pL, const double *pR, double r,
-                           const double *n, double w, double *F, double *U)
+                           double z, const double *n, double w, double *F, double *U)
    {
       if (P.riemann_solver == RS_HLLD) {
-         riemann_hlld(P, pL, pR, r, n, w, F, U);
+         riemann_hlld(P, pL, pR, r, z, n, w, F, U);
          return;
       }
-      GRMetric g = gr_metric(P.metric_par2, r);
+      GRMetric g = gr_metric(P, r, z);
       double Sl, Sr, Ss;
       vel(P, g, r, pL, pR, n, &Sl, &Sr, &Ss);
       if (w < Sl) {
          flux(P, g, r, pL, n, F);
-         prim2cons(P, pL, U, r, 1.0);
+         prim2cons(P, pL, U, r, z, 1.0);
       } else if (w > Sr) {
          flux(P, g, r, pR, n, F);
-         prim2cons(P, pR, U, r, 1.0);
+         prim2cons(P, pR, U, r, z, 1.0);
       } else if (P.riemann_solver == RS_HLL) {
          double Fl[NQ], Fr[NQ], Ul[NQ], Ur[NQ];
          double aL = Sr, aR = -Sl;
-         prim2cons(P, pL, Ul, r, 1.0);
-         prim2cons(P, pR, Ur, r, 1.0);
+         prim2cons(P, pL, Ul, r, z, 1.0);
+         prim2cons(P, pR, Ur, r, z, 1.0);
          flux(P, g, r, pL, n, Fl);
          flux(P, g, r, pR, n, Fr);
 #pragma unroll
@@ -578,7 +587,7 @@ struct GRMHD {
          const double *pk = w < Ss ? pL : pR;
          double Sk = w < Ss ? Sl : Sr;
          double Uk[NQ], Fk[NQ];
-         prim2cons(P, pk, Uk, r, 1.0);
+         prim2cons(P, pk, Uk, r, z, 1.0);
          ustar(P, g, r, pk, Sk, Ss, n, U);
          flux(P, g, r, pk, n, Fk);
 #pragma unroll
@@ -588,9 +597,9 @@ struct GRMHD {
 
    // cons2prim (grmhd2.c:253-270) with the Noble et al. 2-D Newton solve
    // (grmhd2.c:1096-1314); prim holds the previous primitives on entry (the
-   // initial guess) and the result on exit.  r is the cell centroid radius.
+   // initial guess) and the result on exit.  (r, z) is the cell centroid.
    static DHD void cons2prim(const dh_params &P, const double *cons_in, double *prim, double r,
-                             double dV)
+                             double z, double dV)
    {
       double gl = P.gamma_law;
       double cons[NQ];
@@ -602,7 +611,7 @@ struct GRMHD {
       double D = cons[DDD];
       double S[3] = {cons[SRR], cons[LLL], cons[SZZ]};
       double tau = cons[TAU];
-      GRMetric g = gr_metric(P.metric_par2, r);
+      GRMetric g = gr_metric(P, r, z);
       double al = g.al, jac = g.jac, sqrtgam = g.sqrtgam;
       const double *be = g.be, *gam = g.gam, *igam = g.igam, *U = g.U;
       double B[3] = {r * cons[BRR] / sqrtgam, cons[BPP] / sqrtgam, cons[BZZ] / sqrtgam};
@@ -680,18 +689,18 @@ struct GRMHD {
    }
 
    // source (grmhd2.c:383-500): 1/2 T^{mu nu} d_i g_{mu nu} in the momenta
-   // and the frame term in the energy, at the cell's mid radius.
+   // and the frame term in the energy, at the cell's mid point.
    static DHD void source(const dh_params &P, const double *prim, double *cons, double rp,
                           double rm, double phip, double phim, double zp, double zm,
                           double dVdt, int polar, double Pmax)
    {
       double gl = P.gamma_law, M = P.metric_par2;
-      double r = 0.5 * (rm + rp);
-      GRMetric g = gr_metric(M, r);
+      double r = 0.5 * (rm + rp), z = gr_z(P, 0.5 * (zm + zp));
+      GRMetric g = gr_metric(P, r, z);
       double al = g.al, jac = g.jac, sqrtgam = g.sqrtgam;
       const double *be = g.be, *gam = g.gam, *igam = g.igam, *U = g.U;
       double dU[16];
-      gr_frame_der_U(M, r, dU);
+      gr_frame_der_U(M, r, z, dU);
       double rho = prim[RHO], Pp = prim[PPP];
       double l[4] = {0.0, prim[URR], prim[UPP], prim[UZZ]};
       double B[3] = {prim[BRR] / sqrtgam, prim[BPP] / (r * sqrtgam), prim[BZZ] / sqrtgam};
@@ -734,7 +743,7 @@ struct GRMHD {
          Sk[i] = 0.0;
          if (gr_killing(i + 1)) continue;
          double dg[16];
-         gr_der_g(M, r, i + 1, dg);
+         gr_der_g(M, r, z, i + 1, dg);
          for (int mu = 0; mu < 4; ++mu)
             for (int nu = 0; nu < 4; ++nu)
                Sk[i] += (rhoh * u[mu] * u[nu] * rcorr * pcorr + ig[4 * mu + nu] * Pt -
@@ -753,7 +762,7 @@ struct GRMHD {
       cons[LLL] += jac * Sk[1] * dVdt;
       cons[SZZ] += jac * Sk[2] * dVdt;
       cons[TAU] += jac * S0 * dVdt;
-      (void)phip; (void)phim; (void)zp; (void)zm; (void)polar; (void)Pmax;
+      (void)phip; (void)phim; (void)polar; (void)Pmax;
    }
 
    static DHD void visc_source(const dh_params &, const double *, const double *,
@@ -766,7 +775,7 @@ struct GRMHD {
    {
    }
 
-   // mindt (grmhd2.c:748-832) at the cell's mid radius; w is the mean
+   // mindt (grmhd2.c:748-832) at the cell's mid point; w is the mean
    // angular velocity of the cell's two phi faces.
    static DHD double mindt(const dh_params &P, const double *prim, double wc, double rp,
                            double rm, double phip, double phim, double zp, double zm,
@@ -774,7 +783,7 @@ struct GRMHD {
    {
       double gl = P.gamma_law;
       double r = 0.5 * (rm + rp);
-      GRMetric g = gr_metric(P.metric_par2, r);
+      GRMetric g = gr_metric(P, r, 0.5 * (zm + zp));
       double al = g.al;
       const double *be = g.be, *gam = g.gam, *igam = g.igam;
       double sqrtgam = r / (r * al);  // metric_jacobian / (r al)
@@ -822,10 +831,10 @@ struct GRMHD {
       return dt;
    }
 
-   // get_omega (grmhd2.c:64-86): the coordinate angular velocity at radius r.
-   static DHD double omega(const dh_params &P, const double *prim, double r)
+   // get_omega (grmhd2.c:64-86): the coordinate angular velocity at (r, z).
+   static DHD double omega(const dh_params &P, const double *prim, double r, double z)
    {
-      GRMetric g = gr_metric(P.metric_par2, r);
+      GRMetric g = gr_metric(P, r, z);
       double l[3] = {prim[URR], prim[UPP], prim[UZZ]};
       double u2 = 0.0;
       for (int i = 0; i < 3; ++i)

@@ -173,9 +173,10 @@ struct KWcell {
          double pc[NQ], pp[NQ];
          load_q<NQ>(prim, nt, c, pc);
          load_q<NQ>(prim, nt, cp, pp);
-         int j = a % m.Nr;
+         int j = a % m.Nr, k = a / m.Nr;
          double r = centroid_r(m.r_jph[j], m.r_jph[j - 1]);
-         w = .5 * (H::omega(P, pc, r) + H::omega(P, pp, r));
+         double z = centroid_z(m.z_kph[k], m.z_kph[k - 1]);
+         w = .5 * (H::omega(P, pc, r, z) + H::omega(P, pp, r, z));
       }
       wiph[c] = w;
    }
@@ -559,7 +560,7 @@ struct KFluxPhi {
       const double n[3] = {0.0, 1.0, 0.0};
       double w = r * wiph[c];
       double Flux[NQ], Ustr[NQ];
-      H::riemann(P, pL, pR, r, n, w, Flux, Ustr);
+      H::riemann(P, pL, pR, r, centroid_z(zp, zm), n, w, Flux, Ustr);
       if (P.visc_flag) {
          double gr[NQ], gp[NQ], gz[NQ];
 #pragma unroll
@@ -623,9 +624,12 @@ struct KFluxTrans {
       double n[3] = {0.0, 0.0, 0.0};
       n[dim == 1 ? 0 : 2] = 1.0;
       int j = pr.j;
+      // the face centroid (faces_alt.c:11-18)
+      int k = pr.k;
       double r = dim == 1 ? .5 * (m.r_jph[j] + m.r_jph[j]) : centroid_r(m.r_jph[j], m.r_jph[j - 1]);
+      double z = dim == 2 ? .5 * (m.z_kph[k] + m.z_kph[k]) : centroid_z(m.z_kph[k], m.z_kph[k - 1]);
       double Flux[NQ], Ustr[NQ];
-      H::riemann(P, pL, pR, r, n, 0.0, Flux, Ustr);
+      H::riemann(P, pL, pR, r, z, n, 0.0, Flux, Ustr);
       if (P.visc_flag) {
          double gr[NQ], gp[NQ], gz[NQ];
 #pragma unroll
@@ -843,7 +847,7 @@ struct KUpdate {
          double r = centroid_r(rp, rm);
          double dVn = cell_dV(rp, rm, pp, pp - dp_new, zp, zm, Pmax);
          double out[NQ];
-         H::cons2prim(P, cs, out, r, dVn);
+         H::cons2prim(P, cs, out, r, centroid_z(zp, zm), dVn);
 #pragma unroll
          for (int q = 0; q < NQ; ++q) prim[q * nt + c] = out[q];
       }
@@ -1459,7 +1463,7 @@ struct KBfacesC2p {
       load_q<NQ>(cons, nt, c, cs);
       double dVc = cell_dV(rp, rm, piph_new[c], piph_new[c] - dphi_new[c], zp, zm, m.Pmax);
       load_q<NQ>(prim, nt, c, out);  // the previous primitives: an iterative solve's guess
-      H::cons2prim(P, cs, out, x0, dVc);
+      H::cons2prim(P, cs, out, x0, centroid_z(zp, zm), dVc);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) prim[q * nt + c] = out[q];
    }
@@ -1658,8 +1662,9 @@ struct KInitCells {
       double r = centroid_r(rp, rm);
       double pr[NQ], cs[NQ];
       load_q<NQ>(prim, nt, c, pr);
-      H::prim2cons(P, pr, cs, r, dV);
-      if constexpr (!IsMHD<H>::value) H::cons2prim(P, cs, pr, r, dV);
+      double z = centroid_z(zp, zm);
+      H::prim2cons(P, pr, cs, r, z, dV);
+      if constexpr (!IsMHD<H>::value) H::cons2prim(P, cs, pr, r, z, dV);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
          cons[q * nt + c] = cs[q];

@@ -19,7 +19,7 @@ struct MHD {
 
    // prim2cons (mhd.c:35-69)
    static DHD void prim2cons(const dh_params &P, const double *prim, double *cons,
-                             double r, double dV)
+                             double r, double /*z*/, double dV)
    {
       double rho = prim[RHO];
       double Pp = prim[PPP];
@@ -46,7 +46,7 @@ struct MHD {
 
    // cons2prim (mhd.c:138-185)
    static DHD void cons2prim(const dh_params &P, const double *cons, double *prim,
-                             double r, double dV)
+                             double r, double /*z*/, double dV)
    {
       double rho = cons[DDD] / dV;
       if (rho < P.rho_floor) rho = P.rho_floor;
@@ -151,7 +151,10 @@ struct MHD {
       return dt;
    }
 
-   static DHD double omega(const dh_params &, const double *prim, double) { return prim[UPP]; }
+   static DHD double omega(const dh_params &, const double *prim, double, double)
+   {
+      return prim[UPP];
+   }
 
    // ---- HLLD (hlld.c) ----
 
@@ -327,7 +330,8 @@ struct MHD {
    // get_Ustar_HLLD (hlld.c:282-401): flux and fan state at speed w.  hp is
    // the phi scale factor (r) at the face.
    static DHD void riemann(const dh_params &P, const double *primL, const double *primR,
-                           double hp, const double *n, double w, double *F, double *U)
+                           double hp, double /*z*/, const double *n, double w, double *F,
+                           double *U)
    {
       double gam = P.gamma_law;
       double pL[NQ], pR[NQ];

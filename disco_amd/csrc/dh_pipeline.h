@@ -75,6 +75,8 @@ struct Buffers {
 inline int validate(const dh_grid *g, const dh_params &P, long *ntot_out)
 {
    int nq = P.num_c + P.num_n;
+   if (P.hydro != DH_HYDRO_GRMHD && P.metric != DH_METRIC_FLAT)
+      return fail(DH_EINVAL, "euler and mhd run with the flat_rpz metric");
    if (P.hydro == DH_HYDRO_EULER) {
       if (P.num_c != 5) return fail(DH_EINVAL, "euler needs NUM_C = 5 (got %d)", P.num_c);
       if (P.ct_mode != 0) return fail(DH_EINVAL, "CT_MODE %d needs the MHD module", P.ct_mode);
@@ -103,6 +105,8 @@ inline int validate(const dh_grid *g, const dh_params &P, long *ntot_out)
          return fail(DH_EINVAL, "grmhd2 needs ghost annuli on every side");
       if (P.mesh_motion == 3 || P.energy_omega)
          return fail(DH_EINVAL, "grmhd2 takes Mesh_Motion 0, 1 or 4 and no energy frame");
+      if (P.metric != DH_METRIC_SCHW_KS_EQ && P.metric != DH_METRIC_SCHW_KS)
+         return fail(DH_EINVAL, "grmhd2 runs with the schw_ks_eq_rpz or schw_ks_rpz metric");
    } else {
       return fail(DH_EINVAL, "hydro module %d not supported", P.hydro);
    }

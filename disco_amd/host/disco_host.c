@@ -34,6 +34,9 @@
 #ifndef HYDRO
 #define HYDRO (NUM_C > 5 ? "mhd" : "euler")
 #endif
+#ifndef METRIC
+#define METRIC (!strcmp(HYDRO, "grmhd2") ? "schw_ks_eq_rpz" : "flat_rpz")
+#endif
 
 /* Plugin symbols (reference plugin ABI). */
 void setICparams(struct domain *);
@@ -331,6 +334,10 @@ static void fill_params(struct domain *d, dh_params *P, const int *bc)
    P->hydro = !strcmp(HYDRO, "grmhd2") ? DH_HYDRO_GRMHD
               : !strcmp(HYDRO, "mhd")  ? DH_HYDRO_MHD
                                        : DH_HYDRO_EULER;
+   P->metric = !strcmp(METRIC, "flat_rpz")         ? DH_METRIC_FLAT
+               : !strcmp(METRIC, "schw_ks_eq_rpz") ? DH_METRIC_SCHW_KS_EQ
+               : !strcmp(METRIC, "schw_ks_rpz")    ? DH_METRIC_SCHW_KS
+                                                   : -1; /* refused at dh_create */
    P->riemann_solver = L->Riemann_Solver;
    P->mesh_motion = L->Mesh_Motion;
    P->exact_mesh_omega = L->Exact_Mesh_Omega;

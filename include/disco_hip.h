@@ -50,8 +50,16 @@ enum {
 enum {
    DH_HYDRO_EULER = 0,  /* Hydro/euler.c */
    DH_HYDRO_MHD = 1,    /* Hydro/mhd.c */
-   DH_HYDRO_GRMHD = 2   /* Hydro/grmhd2.c with Metric/schw_ks_eq_rpz.c (M = metric_par2)
-                           and Frame/normal.c */
+   DH_HYDRO_GRMHD = 2   /* Hydro/grmhd2.c with Frame/normal.c and a DH_METRIC_SCHW_* metric
+                           (M = metric_par2) */
+};
+
+/* Metric modules (reference Hydro/Metric/<name>.c). */
+enum {
+   DH_METRIC_FLAT = 0,     /* flat_rpz.c (euler, mhd) */
+   DH_METRIC_SCHW_KS_EQ = 1, /* schw_ks_eq_rpz.c: Schwarzschild in Kerr-Schild coordinates
+                              evaluated on the equator (z = 0) */
+   DH_METRIC_SCHW_KS = 2   /* schw_ks_rpz.c: the same metric at (r, z) */
 };
 
 /* Boundary treatment of one side of the mesh, applied to ghost annuli after
@@ -95,7 +103,8 @@ typedef struct dh_params {
    int r_periodic, z_periodic;
    int bc_rmin, bc_rmax, bc_zmin, bc_zmax;  /* DH_BC_* per side */
    int nozzle_type;             /* Nozzle_Type (only 0 is supported) */
-   int pad_[3];
+   int metric;                  /* DH_METRIC_* (the METRIC module) */
+   int pad_[2];
    double gamma_law;            /* Adiabatic_Index */
    double plm;                  /* PLM */
    double cfl;                  /* CFL */

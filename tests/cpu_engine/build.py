@@ -38,7 +38,7 @@ def build_driver(config: str) -> str:
     """The `disco` driver of one module selection (disco_amd/host + plugins)
     linked to the host build of the engine instead of libdisco_hip.so."""
     sys.path.insert(0, REPO)
-    from disco_amd.build import HOST_CONFIGS
+    from disco_amd.build import HOST_CONFIGS, host_defines
     lib = build()
     c = HOST_CONFIGS[config]
     out = os.path.join(HERE, "_build", f"disco_{config}")
@@ -50,10 +50,7 @@ def build_driver(config: str) -> str:
     deps = srcs + [lib, os.path.join(host, "disco_host.h"), os.path.join(REPO, "disco_amd", "build.py")]
     if os.path.exists(out) and all(os.path.getmtime(out) >= os.path.getmtime(s) for s in deps):
         return out
-    cmd = ["gcc", "-O2", "-Wall", "-Wno-unused-parameter", f"-DNUM_C={c['num_c']}",
-           f"-DNUM_N={c['num_n']}", f"-DCT_MODE={c['ct_mode']}", f'-DBOUNDARY="{c["boundary"]}"',
-           f'-DINITIAL="{c["initial"]}"', f'-DANALYSIS="{c["analysis"]}"',
-           f'-DHYDRO="{c["hydro"]}"',
+    cmd = ["gcc", "-O2", "-Wall", "-Wno-unused-parameter", *host_defines(c),
            f"-I{os.path.join(REPO, 'include', 'disco')}",
            f"-I{os.path.join(REPO, 'include')}", f"-I{host}", *srcs, "-o", out + ".tmp",
            f"-L{os.path.dirname(lib)}", "-ldh_cpu", "-lm",

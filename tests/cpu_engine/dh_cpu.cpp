@@ -185,8 +185,8 @@ int dhx_riemann(dh_ctx *c, const double *pL, const double *pR, double r, const d
 {
    if (!c) return fail(DH_EINVAL, "null context");
    double F[9], U[9];
-   if (c->b.mhd && c->b.nq == 8) MHD<8>::riemann(c->b.P, pL, pR, r, n, w, F, U);
-   else if (!c->b.mhd && c->b.nq == 5) Euler<5>::riemann(c->b.P, pL, pR, r, n, w, F, U);
+   if (c->b.mhd && c->b.nq == 8) MHD<8>::riemann(c->b.P, pL, pR, r, 0.0, n, w, F, U);
+   else if (!c->b.mhd && c->b.nq == 5) Euler<5>::riemann(c->b.P, pL, pR, r, 0.0, n, w, F, U);
    else return fail(DH_EINVAL, "dhx_riemann: NQ %d not wired", c->b.nq);
    for (int q = 0; q < c->b.nq; ++q) Fw[q] = F[q] - w * U[q];
    return DH_OK;

@@ -11,6 +11,7 @@ trajectory to a looser bound and each of its steps, from a common state, to
 the bar (B) and to round-off (density, pressure, velocity, Phi).  The MHD
 disks the other GPU parity tests run move by less than a tenth of the bar.
 """
+import copy
 import os
 import subprocess
 import sys
@@ -140,11 +141,66 @@ def test_grmhd_hll_and_hlld_are_well_conditioned(solver, steps):
 def test_grmhd_hllc_field_flux_is_discontinuous():
     """GR HLLC (riemann.c:262-290 with grmhd2.c:182-251) leaves the field of
     its star state at zero, so the field flux jumps when a face's contact
-    speed crosses the face speed -- in a disk the radial contact speed is
-    close to the r faces' zero speed.  A one-ulp change moves the field by
-    more than 1e-4 in two steps, while one step from a common state moves
-    every field by round-off only: GR HLLC is compared step by step."""
+    speed crosses the face speed.  In the 3-D disk the contact speeds sit at
+    the faces' speeds (v_z = 0 at z faces at rest, v_phi at faces moving
+    with the gas), so a one-ulp change moves the field by more than 1e-4 in
+    two steps."""
     cfg, par, a, dts, ref = _reference_run(
         "gr_disk", {"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 1}, 2)
     e = field_errors(_perturbed(cfg, par, a, dts, "pow,sincos"), ref)
     assert max(e[f"cons{q}"] for q in (5, 6, 7)) > 1e-4
+
+
+def _ulp_state(a, seed=1):
+    """a with half of its primitive and conserved values one ulp larger."""
+    b = copy.deepcopy(a)
+    rng = np.random.default_rng(seed)
+    for arr in (b.prim, b.cons):
+        m = rng.random(arr.shape) < 0.5
+        arr[m] = np.nextafter(arr[m], np.inf)
+    return b
+
+
+def _one_step_spread(cname, over):
+    cfg = configs.get(cname)
+    par = cfg.par_dict(**over)
+    exe = configs.ref_binary(cfg)
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    d = tempfile.mkdtemp(prefix="condulp_")
+    state_io.write_par(d, par)
+    subprocess.run([exe, d, "0"], cwd=d, check=True, capture_output=True)
+    a = state_io.read_state(os.path.join(d, "state_0000.bin"))
+    lib = capi.load_library(cpu_build.build())
+    outs = []
+    for s in (a, _ulp_state(a)):
+        g = capi.DiscoHip(cfg, par, s, library=lib)
+        if not outs:
+            dt = g.getmindt()
+        g.timestep(dt)
+        outs.append(g.download(a))
+        g.close()
+    return field_errors(*outs)
+
+
+@pytest.mark.parametrize("over,ill", [
+    ({"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 1}, True),
+    ({"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 1, "Mesh_Motion": 0, "Init_Par0": 1}, True),
+    ({"Num_R": 32, "Num_Z": 1, "Riemann_Solver": 1, "Mesh_Motion": 0, "Init_Par0": 1}, False),
+    ({"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 0}, False),
+    ({"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 2}, False),
+])
+def test_grmhd_one_step_from_a_one_ulp_state(over, ill):
+    """One step from a state one ulp away: GR HLLC jumps wherever a contact
+    sits at a face speed -- the 3-D disk's z faces (v_z = 0), and with the
+    mesh moving with the gas its phi faces -- so the step moves by more than
+    1e-5; the 2-D disk falling in on a static mesh keeps every contact away
+    from its face and moves by round-off, as HLL and HLLD do everywhere.  The
+    GPU, whose FMA contraction rounds differently, is compared with GR HLLC
+    on that 2-D disk."""
+    e = _one_step_spread("gr_disk", over)
+    worst = max(e.values())
+    if ill:
+        assert worst > 1e-5, worst
+    else:
+        assert worst < 1e-12, worst

@@ -32,11 +32,21 @@ import build as cpu_build  # noqa: E402
     ("gr_disk", {"Num_R": 24, "Num_Z": 6}, 8),
     ("gr_disk", {"Num_R": 16, "Num_Z": 4, "Riemann_Solver": 0}, 4),
     ("gr_disk", {"Num_R": 16, "Num_Z": 4, "Mesh_Motion": 1}, 4),
+    # a 2-D disk on a static mesh, falling in (HLLC with its contacts away
+    # from the faces' speeds)
+    ("gr_disk", {"Num_R": 32, "Num_Z": 1, "Mesh_Motion": 0, "Init_Par0": 1}, 6),
     # the relativistic HLLD: a smooth run, one where the reference falls back
     # to the HLL state at some faces, and a strong field with many fallbacks
     ("gr_disk", {"Num_R": 16, "Num_Z": 4, "Riemann_Solver": 2, "CFL": 0.2}, 8),
     ("gr_disk", {"Num_R": 24, "Num_Z": 6, "Riemann_Solver": 2}, 4),
     ("gr_disk", {"Num_R": 24, "Num_Z": 6, "Riemann_Solver": 2, "Init_Par1": 10.0, "CFL": 0.3}, 6),
+    # the full (r, z) Schwarzschild metric (schw_ks_rpz): every solver, and a
+    # slab far enough off the equator that the vertical metric terms matter
+    ("gr_disk3d", {"Num_R": 24, "Num_Z": 6}, 6),
+    ("gr_disk3d", {"Num_R": 16, "Num_Z": 4, "Riemann_Solver": 0}, 4),
+    ("gr_disk3d", {"Num_R": 16, "Num_Z": 8, "Riemann_Solver": 2, "CFL": 0.3}, 6),
+    ("gr_disk3d", {"Num_R": 16, "Num_Z": 8, "Riemann_Solver": 2, "Z_Min": 1.0, "Z_Max": 3.0,
+                   "Mesh_Motion": 1}, 5),
 ])
 def test_driver_bit_exact_vs_reference(cname, over, steps):
     cfg = configs.get(cname)

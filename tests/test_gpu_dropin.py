@@ -29,10 +29,14 @@ HOST = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
     ("mhd_kep", {"Num_R": 32, "Num_Z": 8}, 6),
     ("mhd_kep_p2", {"Num_R": 32, "Num_Z": 8, "Max_DT": 0.01}, 6),
     # GRMHD trajectories with HLL and HLLD over the steps a one-ulp libm change
-    # moves by less than 1e-11 (tests/test_conditioning.py); GR HLLC and
-    # longer runs are checked step by step in test_gpu_grmhd.py
+    # moves by less than 1e-11, and GR HLLC on the 2-D disk falling in on a
+    # static mesh, whose contacts stay away from the faces' speeds
+    # (tests/test_conditioning.py); longer runs are checked step by step in
+    # test_gpu_grmhd.py
     ("gr_disk", {"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 0}, 3),
     ("gr_disk", {"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 2}, 3),
+    ("gr_disk", {"Num_R": 96, "Num_Z": 1, "Mesh_Motion": 0, "Init_Par0": 1}, 6),
+    ("gr_disk3d", {"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 0}, 3),
 ])
 def test_disco_driver_matches_reference(cname, over, steps):
     cfg = configs.get(cname)
