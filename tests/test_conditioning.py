@@ -204,3 +204,36 @@ def test_grmhd_one_step_from_a_one_ulp_state(over, ill):
         assert worst > 1e-5, worst
     else:
         assert worst < 1e-12, worst
+
+
+def test_grmhd_offequator_slab_branches_at_its_sixth_step():
+    """The slab z in [1, 3] of the (r, z) metric falls onto itself: its first
+    five HLLD steps move by less than 1e-12 under a one-ulp change of the
+    state, its sixth by more than 1e-5 (the solve's branches flip)."""
+    cfg = configs.get("gr_disk3d")
+    par = cfg.par_dict(Num_R=32, Num_Z=8, Riemann_Solver=2, Z_Min=1.0, Z_Max=3.0)
+    exe = configs.ref_binary(cfg)
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    d = tempfile.mkdtemp(prefix="condslab_")
+    state_io.write_par(d, par)
+    subprocess.run([exe, d, "0"], cwd=d, check=True, capture_output=True)
+    a = state_io.read_state(os.path.join(d, "state_0000.bin"))
+    lib = capi.load_library(cpu_build.build())
+    g = capi.DiscoHip(cfg, par, a, library=lib)
+    spread = []
+    for _ in range(6):
+        s = g.download(a)
+        outs = []
+        for t in (s, _ulp_state(s)):
+            h = capi.DiscoHip(cfg, par, t, library=lib)
+            if not outs:
+                dt = h.getmindt()
+            h.timestep(dt)
+            outs.append(h.download(a))
+            h.close()
+        spread.append(max(field_errors(*outs).values()))
+        g.timestep(dt)
+    g.close()
+    assert max(spread[:5]) < 1e-12, spread
+    assert spread[5] > 1e-5, spread

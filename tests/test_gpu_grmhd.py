@@ -33,14 +33,16 @@ CASES = [
     ("gr_disk", {"Num_R": 48, "Num_Z": 12, "Riemann_Solver": 0}),
     ("gr_disk", {"Num_R": 96, "Num_Z": 1, "Riemann_Solver": 1, "Mesh_Motion": 0, "Init_Par0": 1}),
     ("gr_disk", {"Num_R": 48, "Num_Z": 12, "Riemann_Solver": 2}),
-    # the full (r, z) metric, on the equator and on a slab off it
+    # the full (r, z) metric, on the equator and on a slab off it; the slab
+    # falls onto itself and from its sixth step its HLLD solve branches on
+    # round-off (tests/test_conditioning.py), so it is compared over five
     ("gr_disk3d", {"Num_R": 48, "Num_Z": 12, "Riemann_Solver": 0}),
-    ("gr_disk3d", {"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 2, "Z_Min": 1.0, "Z_Max": 3.0}),
+    ("gr_disk3d", {"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 2, "Z_Min": 1.0, "Z_Max": 3.0}, 5),
 ]
 
 
-@pytest.mark.parametrize("cname,over", CASES)
-def test_gpu_grmhd_steps_match_host_build(cname, over):
+@pytest.mark.parametrize("cname,over,nsteps", [c if len(c) == 3 else c + (6,) for c in CASES])
+def test_gpu_grmhd_steps_match_host_build(cname, over, nsteps):
     cfg = configs.get(cname)
     par = cfg.par_dict(**over)
     exe = configs.ref_binary(cfg)
@@ -50,11 +52,12 @@ def test_gpu_grmhd_steps_match_host_build(cname, over):
     state_io.write_par(d, par)
     subprocess.run([exe, d, "0"], cwd=d, check=True, capture_output=True)
     a = state_io.read_state(os.path.join(d, "state_0000.bin"))
-    print(f"case {cname} {over}: {a.n_real()} cells, 6 steps from common states", flush=True)
+    print(f"case {cname} {over}: {a.n_real()} cells, {nsteps} steps from common states",
+          flush=True)
     host = capi.load_library(cpu_build.build())
     g = capi.DiscoHip(cfg, par, a)
     worst = 0.0
-    for n in range(6):
+    for n in range(nsteps):
         s = g.download(a)
         outs, dts = [], []
         for lib in (capi.lib(), host):
