@@ -51,6 +51,18 @@ WORKLOADS = {
         desc="2-D Euler isentropic vortex, Nr=128 polar mesh (51,809 cells), HLLC, PLM 1.5, "
              "RK2, CFL 0.5, rigid mesh rotation",
         cpu_over={}, cpu_steps=150),
+    "gr_disk": dict(
+        config="gr_disk",
+        over={"Num_R": 128, "Num_Z": 32, "R_Min": 8.0, "R_Max": 20.0, "Riemann_Solver": 2,
+              "CFL": 0.2},
+        steps=10, warmup=1,
+        desc="3-D GRMHD disk on the Schwarzschild equator (Kerr-Schild, M=1): r in [8,20], "
+             "z in [-0.5,0.5], Nr=128, Nz=32, circular-geodesic gas rho=(r/6)^-1.5, "
+             "P=0.01 rho, gamma 4/3, vertical field at beta=100; relativistic HLLD, PLM 1.5, "
+             "constrained transport, Keplerian mesh motion, RK2, CFL 0.2",
+        cpu_over={"Num_R": 32, "Num_Z": 8, "R_Min": 8.0, "R_Max": 20.0, "Riemann_Solver": 2,
+                  "CFL": 0.2},
+        cpu_steps=10),
 }
 DEFAULT_WORKLOAD = "mhd_kep"
 
@@ -115,6 +127,10 @@ def kernel_bytes(name: str, nq: int, mhd: bool, nt: int, nr: int, nfr: int, nfz:
         # and moved geometry out
         return (d * nq * nr * (2.5 + 1 + (0 if mhd else 1)) + d * nq * (nt + nfa)
                 + 2 * steps + i * nt + d * 5 * nt)
+    if name == "c2p_guess":            # the invariants' inputs in, (v^2, eta) out
+        return d * 8 * nr + d * 2 * nt + i * nt + 4 * 2 * nt
+    if name == "c2p":                  # cons and prim in, guess in, prim and record out
+        return d * nq * nr * 3 + d * 2 * nt + i * nt + 4 * 2 * nt + nt
     if name == "bfaces_c2p":
         return (2 * steps + i * nt + d * 10 * nt + nfa * (2 * i + 1 + d)
                 + d * nq * nr * 2)
@@ -174,6 +190,10 @@ def main(argv=None):
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timers", action="store_true")
+    ap.add_argument("--c2p-guess", choices=["previous", "learned"], default="previous",
+                    help="grmhd2 cons2prim's initial guess (learned: the committed model)")
+    ap.add_argument("--c2p-stop", choices=["reference", "residual"], default="reference",
+                    help="grmhd2 cons2prim's stopping rule")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(json.dumps({"error": "disco_amd runs one mesh on one GPU; sharding the annulus "
@@ -189,6 +209,12 @@ def main(argv=None):
     warmup = args.warmup if args.warmup is not None else w["warmup"]
     cfg = configs.get(w["config"])
     par = cfg.par_dict(**w["over"])
+    gr = cfg.hydro == "grmhd2"
+    if gr and args.c2p_guess == "learned":
+        from disco_amd import c2p_model
+        par.update(C2P_Guess=1, C2P_Model=c2p_model.MODEL_PATH)
+    if gr and args.c2p_stop == "residual":
+        par["C2P_Stop"] = 1
     tmp = tempfile.mkdtemp(prefix="bench_")
     par_path = state_io.write_par(tmp, par)
     t_setup = time.perf_counter()
@@ -212,7 +238,7 @@ def main(argv=None):
     ncell = run.real_cells
     value = ncell * steps / secs
     nq, nf = cfg.nq, cfg.nf
-    mhd = cfg.hydro == "mhd"
+    mhd = cfg.hydro in ("mhd", "grmhd2")
 
     nt = C.c_long()
     lib.dh_sizes(ctx, C.byref(nt), None, None)
@@ -292,6 +318,12 @@ def main(argv=None):
         "kernels": kernels,
         "cpu_baseline": None,
     }
+    if gr:
+        st = (C.c_longlong * 4)()
+        if lib.dh_c2p_stats(ctx, st) == 0 and st[0]:
+            out["c2p"] = {"guess": args.c2p_guess, "stop": args.c2p_stop,
+                          "iterations_per_cell": st[1] / st[0],
+                          "learned_fraction": st[2] / st[0], "fallback_fraction": st[3] / st[0]}
     run.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, w["cpu_over"], w["cpu_steps"])

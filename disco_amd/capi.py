@@ -35,13 +35,13 @@ _PARAM_INTS = ["num_c", "num_n", "ct_mode", "hydro", "riemann_solver", "mesh_mot
                "cs2_profile", "visc_flag", "alpha_flag", "rot_frame", "grav2d",
                "nobc_rmin", "nobc_rmax", "nobc_zmin", "nobc_zmax", "r_periodic",
                "z_periodic", "bc_rmin", "bc_rmax", "bc_zmin", "bc_zmax", "nozzle_type",
-               "metric"]
+               "metric", "c2p_guess", "c2p_stop"]
 _PARAM_DOUBLES = ["gamma_law", "plm", "cfl", "max_dt", "rho_floor", "p_floor",
                   "viscosity", "mach", "metric_par2", "rot_omega", "rot_d"]
 
 
 class DhParams(C.Structure):
-    _fields_ = ([(n, C.c_int) for n in _PARAM_INTS] + [("pad_", C.c_int * 2)]
+    _fields_ = ([(n, C.c_int) for n in _PARAM_INTS]
                 + [(n, C.c_double) for n in _PARAM_DOUBLES]
                 + [("init_par", C.c_double * 9), ("phi_order", C.c_double)])
 
@@ -86,6 +86,10 @@ def make_params(cfg: "cfgmod.Config", par: Dict[str, object]) -> DhParams:
         p.init_par[i] = float(full[f"Init_Par{i}"])
     p.bc_rmin, p.bc_rmax, p.bc_zmin, p.bc_zmax = cfg.dh_bc(full)
     p.phi_order = 2.0  # PHI_ORDER, planet.c:4
+    # grmhd2 cons2prim's guess and stopping rule (DH_C2P_*; not reference
+    # in.par keys, so absent from reference runs)
+    p.c2p_guess = int(full.get("C2P_Guess", 0))
+    p.c2p_stop = int(full.get("C2P_Stop", 0))
     return p
 
 
@@ -136,6 +140,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "dh_get_diagnostics": ([vp, _dp], C.c_int),
         "dh_zero_diagnostics": ([vp], C.c_int),
         "dh_avg_diagnostics": ([vp, C.c_double], C.c_int),
+        "dh_set_c2p_model": ([vp, C.POINTER(C.c_float), C.c_long], C.c_int),
+        "dh_c2p_harvest": ([vp, C.POINTER(C.c_float), C.POINTER(C.c_float)], C.c_int),
+        "dh_c2p_predict": ([vp, C.POINTER(C.c_float)], C.c_int),
+        "dh_c2p_stats": ([vp, C.POINTER(C.c_longlong)], C.c_int),
         "dh_last_error": ([], C.c_char_p),
     }
     for name, (args, res) in sig.items():
@@ -149,7 +157,8 @@ EXPORTED_SYMBOLS = ["dh_create", "dh_destroy", "dh_sizes", "dh_upload_state",
                     "dh_download_state", "dh_init_cells", "dh_set_planets", "dh_getmindt", "dh_timestep",
                     "dh_face_count", "dh_get_faces", "dh_sync", "dh_enable_timers",
                     "dh_read_timers", "dh_add_diagnostics", "dh_get_diagnostics",
-                    "dh_zero_diagnostics", "dh_avg_diagnostics", "dh_last_error"]
+                    "dh_zero_diagnostics", "dh_avg_diagnostics", "dh_set_c2p_model",
+                    "dh_c2p_harvest", "dh_c2p_predict", "dh_c2p_stats", "dh_last_error"]
 
 _LIB: Optional[C.CDLL] = None
 
@@ -237,6 +246,34 @@ class DiscoHip:
 
     def zero_diagnostics(self):
         self._check(self.lib.dh_zero_diagnostics(self.ctx))
+
+    def set_c2p_model(self, params: np.ndarray):
+        """Loads the learned cons2prim guess's parameters (float32, the layout
+        of disco_amd/csrc/dh_c2pnet.h)."""
+        p = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        self._check(self.lib.dh_set_c2p_model(self.ctx, p.ctypes.data_as(C.POINTER(C.c_float)),
+                                              p.size))
+
+    def c2p_harvest(self):
+        """(features [ntot, 8], targets [ntot, 2]) of the current state; NaN
+        rows outside the real cells."""
+        f = np.zeros((self.ntot, 8), np.float32)
+        t = np.zeros((self.ntot, 2), np.float32)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))  # noqa: E731
+        self._check(self.lib.dh_c2p_harvest(self.ctx, fp(f), fp(t)))
+        return f, t
+
+    def c2p_predict(self) -> np.ndarray:
+        """The model's guesses (v^2, eta) [ntot, 2] for the current state."""
+        g = np.zeros((self.ntot, 2), np.float32)
+        self._check(self.lib.dh_c2p_predict(self.ctx, g.ctypes.data_as(C.POINTER(C.c_float))))
+        return g
+
+    def c2p_stats(self) -> Dict[str, int]:
+        """Totals of the last cons2prim over the real cells."""
+        out = (C.c_longlong * 4)()
+        self._check(self.lib.dh_c2p_stats(self.ctx, out))
+        return dict(zip(("cells", "iterations", "learned", "fallbacks"), list(out)))
 
     def init_cells(self):
         self._check(self.lib.dh_init_cells(self.ctx))

@@ -241,6 +241,121 @@ __global__ __launch_bounds__(TPB) void k_mindt(Mesh m, dh_params P, const double
 }
 
 // ---------------------------------------------------------------------------
+// The learned cons2prim guess: the MLP of dh_c2pnet.h on 64 cells per
+// wavefront, its three layers as 16x16x16 bfloat16 MFMA products with float32
+// sums.  A workgroup (4 wavefronts) takes 256 consecutive cells per tile;
+// every lane forms its cell's features, and activations pass between the
+// layers through LDS, where the A operands (16 rows of cells x 16 inputs)
+// are read back in the register layout of the instruction: lane l holds row
+// l % 16, inputs 4 (l / 16) .. 4 (l / 16) + 3.  The products' results
+// arrive as lane l = column l % 16, rows 4 (l / 16) .. + 3.  Cells outside
+// the real mesh run the same instructions on zero features and write NaN.
+// ---------------------------------------------------------------------------
+
+typedef short v4s_t __attribute__((ext_vector_type(4)));
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+
+constexpr int C2P_XP = 20;  // LDS row pitch (bfloat16) of the features: 16 + 4 against bank conflicts
+constexpr int C2P_HP = 68;  // of a hidden layer: 64 + 4
+
+template <int NQ>
+__global__ __launch_bounds__(TPB) void k_c2p_guess(Mesh m, dh_params P, const double *cons,
+                                                   const double *piph, const double *dphi,
+                                                   const float *prm, const v4s_t *frag,
+                                                   float *guess, long ntiles)
+{
+   __shared__ __align__(16) unsigned short lx[TPB * C2P_XP];
+   __shared__ __align__(16) unsigned short lh[TPB * C2P_HP];
+   __shared__ float lo[TPB * C2PNET_OUT];
+   int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   int col = lane & 15, quad = lane >> 4;
+   unsigned short *X = lx + wv * 64 * C2P_XP;
+   unsigned short *H = lh + wv * 64 * C2P_HP;
+   float *O = lo + wv * 64 * C2PNET_OUT;
+   for (long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+      long c = t * TPB + threadIdx.x;
+      float f[C2PNET_IN];
+      bool have = false;
+      double e = 0.0;
+      if (c < m.ntot) {
+         int a = m.cjk[c];
+         int j = a % m.Nr, k = a / m.Nr;
+         if (is_real(m, j, k)) {
+            C2pInv v = cell_c2p_invariants<NQ>(m, P, cons, piph, dphi, c, j, k);
+            c2p_features(v, P.gamma_law, f);
+            e = v.e;
+            have = true;
+         }
+      }
+      for (int i = 0; i < C2PNET_IN; ++i)
+         X[lane * C2P_XP + i] =
+             have ? bf16_bits((f[i] - prm[C2PNET_IN_MU + i]) / prm[C2PNET_IN_SD + i]) : 0;
+      for (int i = C2PNET_IN; i < 16; ++i) X[lane * C2P_XP + i] = 0;
+      __syncthreads();
+      // layer 1: 64 x 16 features times 16 x 64 weights
+      for (int rb = 0; rb < 4; ++rb) {
+         v4s_t av = *(const v4s_t *)(X + (16 * rb + col) * C2P_XP + 4 * quad);
+         v4f_t acc[4];
+         for (int nb = 0; nb < 4; ++nb) {
+            acc[nb] = v4f_t{0.f, 0.f, 0.f, 0.f};
+            acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                av, frag[(C2PNET_FRAG_L1 + nb) * 64 + lane], acc[nb], 0, 0, 0);
+         }
+         for (int nb = 0; nb < 4; ++nb)
+            for (int i = 0; i < 4; ++i) {
+               float h = acc[nb][i] + prm[C2PNET_B1 + 16 * nb + col];
+               H[(16 * rb + 4 * quad + i) * C2P_HP + 16 * nb + col] = bf16_bits(h > 0.f ? h : 0.f);
+            }
+      }
+      __syncthreads();
+      // layer 2: 64 x 64 times 64 x 64, one row block at a time, in place
+      for (int rb = 0; rb < 4; ++rb) {
+         v4f_t acc[4];
+         for (int nb = 0; nb < 4; ++nb) acc[nb] = v4f_t{0.f, 0.f, 0.f, 0.f};
+         for (int kb = 0; kb < 4; ++kb) {
+            v4s_t av = *(const v4s_t *)(H + (16 * rb + col) * C2P_HP + 16 * kb + 4 * quad);
+            for (int nb = 0; nb < 4; ++nb)
+               acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                   av, frag[(C2PNET_FRAG_L2 + 4 * kb + nb) * 64 + lane], acc[nb], 0, 0, 0);
+         }
+         __syncthreads();  // the row block's inputs are read before it is overwritten
+         for (int nb = 0; nb < 4; ++nb)
+            for (int i = 0; i < 4; ++i) {
+               float h = acc[nb][i] + prm[C2PNET_B2 + 16 * nb + col];
+               H[(16 * rb + 4 * quad + i) * C2P_HP + 16 * nb + col] = bf16_bits(h > 0.f ? h : 0.f);
+            }
+      }
+      __syncthreads();
+      // layer 3: 64 x 64 times 64 x 16 (two live columns)
+      for (int rb = 0; rb < 4; ++rb) {
+         v4f_t acc = v4f_t{0.f, 0.f, 0.f, 0.f};
+         for (int kb = 0; kb < 4; ++kb) {
+            v4s_t av = *(const v4s_t *)(H + (16 * rb + col) * C2P_HP + 16 * kb + 4 * quad);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                av, frag[(C2PNET_FRAG_L3 + kb) * 64 + lane], acc, 0, 0, 0);
+         }
+         if (col < C2PNET_OUT)
+            for (int i = 0; i < 4; ++i)
+               O[(16 * rb + 4 * quad + i) * C2PNET_OUT + col] = acc[i] + prm[C2PNET_B3 + col];
+      }
+      __syncthreads();
+      if (c < m.ntot) {
+         if (have) {
+            float tt[C2PNET_OUT], g[2];
+            for (int o = 0; o < C2PNET_OUT; ++o)
+               tt[o] = O[lane * C2PNET_OUT + o] * prm[C2PNET_OUT_SD + o] + prm[C2PNET_OUT_MU + o];
+            c2p_targets_to_guess(tt, e, g);
+            guess[2 * c] = g[0];
+            guess[2 * c + 1] = g[1];
+         } else {
+            guess[2 * c] = guess[2 * c + 1] = __builtin_nanf("");
+         }
+      }
+      __syncthreads();  // LDS is reused by the next tile
+   }
+}
+
+// ---------------------------------------------------------------------------
 // HIP executor
 // ---------------------------------------------------------------------------
 
@@ -382,6 +497,22 @@ struct HipExec {
       return DH_OK;
    }
 
+   template <int NQ>
+   int c2p_guess(const Mesh &m, const dh_params &P, const double *cons, const double *piph,
+                 const double *dphi, const float *prm, const unsigned short *frag, float *guess)
+   {
+      long ntiles = (m.ntot + TPB - 1) / TPB;
+      unsigned nb = (unsigned)(ntiles < 4096 ? ntiles : 4096);
+      if (nb == 0) return DH_OK;
+      {
+         Bracket b(*this, "c2p_guess");
+         k_c2p_guess<NQ><<<nb, TPB, 0, stream>>>(m, P, cons, piph, dphi, prm,
+                                                  (const v4s_t *)frag, guess, ntiles);
+      }
+      LAUNCH_CHECK();
+      return DH_OK;
+   }
+
    int alloc(void **p, size_t bytes)
    {
       if (bytes == 0) bytes = 8;
@@ -390,6 +521,19 @@ struct HipExec {
          return fail(DH_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
       allocs.push_back(*p);
       return DH_OK;
+   }
+   // Frees one allocation of this executor (after the work queued on it).
+   int release(void *p)
+   {
+      if (!p) return DH_OK;
+      HIPCHK(hipStreamSynchronize(stream));
+      for (size_t i = 0; i < allocs.size(); ++i)
+         if (allocs[i] == p) {
+            allocs.erase(allocs.begin() + (long)i);
+            HIPCHK(hipFree(p));
+            return DH_OK;
+         }
+      return fail(DH_EINVAL, "release of an unknown pointer");
    }
    int put(void *dst, const void *src, size_t bytes)
    {
@@ -594,6 +738,30 @@ int dh_avg_diagnostics(dh_ctx *c, double t_avg)
 {
    if (!c) return fail(DH_EINVAL, "null context");
    return avg_diagnostics(c->x, c->b, t_avg);
+}
+
+int dh_set_c2p_model(dh_ctx *c, const float *params, long n)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return set_c2p_model(c->x, c->b, params, n);
+}
+
+int dh_c2p_harvest(dh_ctx *c, float *feat, float *targ)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return c2p_harvest(c->x, c->b, feat, targ);
+}
+
+int dh_c2p_predict(dh_ctx *c, float *guess)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return c2p_predict(c->x, c->b, guess);
+}
+
+int dh_c2p_stats(dh_ctx *c, long long *out)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return c2p_stats(c->x, c->b, out);
 }
 
 int dh_enable_timers(dh_ctx *c, int on)

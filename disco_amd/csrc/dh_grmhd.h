@@ -233,6 +233,116 @@ DHD GRState gr_state(double gamma_law, const GRMetric &g, double r, const double
    return s;
 }
 
+// The quantities of a conserved state (per unit volume) that fix the
+// noble2d solution (grmhd2.c:1140-1168): s2 = S.S / D^2, e, Q = sqrt(gam)
+// B^2 / D and psi = sqrt(gam) (B.S)^2 / D^3, with the metric they were
+// formed in.  The adiabatic solution (v^2, eta = W h) depends on these four
+// and the adiabatic index only.
+struct C2pInv {
+   GRMetric g;
+   double D, S[3], tau, B[3], s2, Us, BS, B2, Q, psi, e;
+};
+
+DHD void c2p_invariants(const GRMetric &g, double r, const double *cons, C2pInv &v)
+{
+   v.g = g;
+   const double *be = g.be, *gam = g.gam, *igam = g.igam, *U = g.U;
+   double sqrtgam = g.sqrtgam;
+   v.D = cons[DDD];
+   v.S[0] = cons[SRR];
+   v.S[1] = cons[LLL];
+   v.S[2] = cons[SZZ];
+   v.tau = cons[TAU];
+   v.B[0] = r * cons[BRR] / sqrtgam;
+   v.B[1] = cons[BPP] / sqrtgam;
+   v.B[2] = cons[BZZ] / sqrtgam;
+   const double *S = v.S, *B = v.B;
+   double D = v.D;
+   double s2 = 0.0, Us = 0.0, BS = 0.0, B2 = 0.0;
+   for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) s2 += igam[3 * i + j] * S[i] * S[j];
+   s2 /= D * D;
+   for (int i = 0; i < 3; ++i) Us += S[i] * (be[i] * U[0] + U[i + 1]);
+   Us /= D;
+   for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) B2 += gam[3 * i + j] * B[i] * B[j];
+   double Q = sqrtgam * B2 / D;
+   for (int i = 0; i < 3; ++i) BS += B[i] * S[i];
+   double psi = sqrtgam * BS * BS / (D * D * D);
+   v.s2 = s2;
+   v.Us = Us;
+   v.BS = BS;
+   v.B2 = B2;
+   v.Q = Q;
+   v.psi = psi;
+   v.e = (v.tau / D + Us + 1.0) / (g.al * U[0]);
+}
+
+// The record cons2prim leaves per cell: Newton iterations (saturating at
+// C2P_INFO_ITERS) and whether the learned guess was used or fell back.
+enum { C2P_INFO_ITERS = 63, C2P_INFO_LEARNED = 64, C2P_INFO_FALLBACK = 128 };
+// Relative residual of the residual stop rule (DH_C2P_STOP_RESIDUAL).
+constexpr double C2P_RES_TOL = 1.0e-10;
+
+DHD bool c2p_guess_ok(const float *g)
+{
+   return g[0] >= 0.0f && g[0] < 1.0f && g[1] >= 1.0f && g[1] < 3.0e38f;
+}
+
+// The Newton iteration of noble2d (grmhd2.c:1170-1240) from (v2, eta), at
+// most 30 iterations.  stop = DH_C2P_STOP_REFERENCE: the reference's rule,
+// |d eta / eta| < 1e-12 and then Nextra = 10 more iterations;
+// DH_C2P_STOP_RESIDUAL: both residuals below C2P_RES_TOL relative to
+// (eta + Q)^2 and eta, no extra iterations.  Leaves the last iterate in
+// (v2, eta) and the iteration count in it; returns whether the rule was met.
+DHD bool noble2d(const C2pInv &v, double n, int stop, double &v2_io, double &eta_io, int &it)
+{
+   const double prec = 1.0e-12;
+   const double max_iter = 30;
+   const double Nextra = 10;
+   double Q = v.Q, psi = v.psi, s2 = v.s2, e = v.e;
+   double v21 = v2_io, eta1 = eta_io;
+   double v2, eta, w;
+   int clean = -1;
+   bool met = false;
+   while (1) {
+      v2 = v21;
+      eta = eta1;
+      w = 1.0 / sqrt(1.0 - v2);
+      double fa = (eta + Q) * (eta + Q) * v2 - psi * (2 * eta + Q) / (eta * eta) - s2;
+      double fb = eta - n * (1 - v2) * (eta - w) + 0.5 * Q * (1 + v2) - 0.5 * psi / (eta * eta) - e;
+      double dfadv2 = (eta + Q) * (eta + Q);
+      double dfadet = 2 * (eta + Q) * (v2 + psi / (eta * eta * eta));
+      double dfbdv2 = n * (eta - 0.5 * w) + 0.5 * Q;
+      double dfbdet = 1 - n * (1 - v2) + psi / (eta * eta * eta);
+      double detj = dfadv2 * dfbdet - dfbdv2 * dfadet;
+      v21 = v2 - (dfbdet * fa - dfadet * fb) / detj;
+      eta1 = eta - (-dfbdv2 * fa + dfadv2 * fb) / detj;
+      it++;
+      if (v21 > 1.0) v21 = 0.5 * (v2 + 1.0);
+      if (v21 < 0.0) v21 = 0.5 * v2;
+      if (eta1 < 1.0) eta1 = 0.5 * (eta + 1.0);
+      if (stop == DH_C2P_STOP_RESIDUAL) {
+         if (fabs(fa) <= C2P_RES_TOL * dfadv2 && fabs(fb) <= C2P_RES_TOL * eta) {
+            met = true;
+            break;
+         }
+      } else {
+         double err = (eta1 - eta) / eta;
+         if (fabs(err) < prec && clean < 0) clean = (int)Nextra + 1;
+         if (clean >= 0) clean--;
+         if (clean == 0) {
+            met = true;
+            break;
+         }
+      }
+      if (it == max_iter) break;
+   }
+   v2_io = v21;
+   eta_io = eta1;
+   return met;
+}
+
 template <int NQ>
 struct GRMHD {
    static constexpr int NC = 8;
@@ -601,81 +711,70 @@ struct GRMHD {
    static DHD void cons2prim(const dh_params &P, const double *cons_in, double *prim, double r,
                              double z, double dV)
    {
+      cons2prim_guess(P, cons_in, prim, r, z, dV, nullptr, nullptr);
+   }
+
+   // cons2prim with a learned initial guess (v^2, eta) when guess is not
+   // null and P.c2p_guess asks for it: a guess outside v^2 in [0, 1),
+   // eta >= 1 (or not finite), or a solve from it that does not converge,
+   // falls back to the previous primitives.  info, when not null, receives
+   // the solve's record (C2P_INFO_*).
+   static DHD void cons2prim_guess(const dh_params &P, const double *cons_in, double *prim,
+                                   double r, double z, double dV, const float *guess,
+                                   unsigned char *info)
+   {
       double gl = P.gamma_law;
       double cons[NQ];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) cons[q] = cons_in[q] / dV;
-      const double prec = 1.0e-12;
-      const double max_iter = 30;
-      const double Nextra = 10;
-      double D = cons[DDD];
-      double S[3] = {cons[SRR], cons[LLL], cons[SZZ]};
-      double tau = cons[TAU];
-      GRMetric g = gr_metric(P, r, z);
-      double al = g.al, jac = g.jac, sqrtgam = g.sqrtgam;
-      const double *be = g.be, *gam = g.gam, *igam = g.igam, *U = g.U;
-      double B[3] = {r * cons[BRR] / sqrtgam, cons[BPP] / sqrtgam, cons[BZZ] / sqrtgam};
-      double s2 = 0.0, Us = 0.0, BS = 0.0, B2 = 0.0;
-      for (int i = 0; i < 3; ++i)
-         for (int j = 0; j < 3; ++j) s2 += igam[3 * i + j] * S[i] * S[j];
-      s2 /= D * D;
-      for (int i = 0; i < 3; ++i) Us += S[i] * (be[i] * U[0] + U[i + 1]);
-      Us /= D;
-      for (int i = 0; i < 3; ++i)
-         for (int j = 0; j < 3; ++j) B2 += gam[3 * i + j] * B[i] * B[j];
-      double Q = sqrtgam * B2 / D;
-      for (int i = 0; i < 3; ++i) BS += B[i] * S[i];
-      double psi = sqrtgam * BS * BS / (D * D * D);
-      double e = (tau / D + Us + 1.0) / (al * U[0]);
+      C2pInv v;
+      c2p_invariants(gr_metric(P, r, z), r, cons, v);
+      const GRMetric &g = v.g;
       double n = (gl - 1.0) / gl;
 
-      // initial guess: the previous v^2 and eta
+      // the plain guess: the previous v^2 and eta
       double u2 = 0.0;
       double l[3] = {prim[URR], prim[UPP], prim[UZZ]};
       for (int i = 0; i < 3; ++i)
-         for (int j = 0; j < 3; ++j) u2 += igam[3 * i + j] * l[i] * l[j];
+         for (int j = 0; j < 3; ++j) u2 += g.igam[3 * i + j] * l[i] * l[j];
       double w = sqrt(1.0 + u2);
-      double v21 = u2 / (1.0 + u2);
-      double eta1 = w * (1.0 + gl / (gl - 1.0) * prim[PPP] / prim[RHO]);
-      double v2, eta;
-      int it = 0, clean = -1;
-      while (1) {
-         v2 = v21;
-         eta = eta1;
-         w = 1.0 / sqrt(1.0 - v2);
-         double fa = (eta + Q) * (eta + Q) * v2 - psi * (2 * eta + Q) / (eta * eta) - s2;
-         double fb = eta - n * (1 - v2) * (eta - w) + 0.5 * Q * (1 + v2) - 0.5 * psi / (eta * eta) - e;
-         double dfadv2 = (eta + Q) * (eta + Q);
-         double dfadet = 2 * (eta + Q) * (v2 + psi / (eta * eta * eta));
-         double dfbdv2 = n * (eta - 0.5 * w) + 0.5 * Q;
-         double dfbdet = 1 - n * (1 - v2) + psi / (eta * eta * eta);
-         double detj = dfadv2 * dfbdet - dfbdv2 * dfadet;
-         v21 = v2 - (dfbdet * fa - dfadet * fb) / detj;
-         eta1 = eta - (-dfbdv2 * fa + dfadv2 * fb) / detj;
-         it++;
-         if (v21 > 1.0) v21 = 0.5 * (v2 + 1.0);
-         if (v21 < 0.0) v21 = 0.5 * v2;
-         if (eta1 < 1.0) eta1 = 0.5 * (eta + 1.0);
-         double err = (eta1 - eta) / eta;
-         if (fabs(err) < prec && clean < 0) clean = (int)Nextra + 1;
-         if (clean >= 0) clean--;
-         if (clean == 0 || it == max_iter) break;
+      double v2p = u2 / (1.0 + u2);
+      double etap = w * (1.0 + gl / (gl - 1.0) * prim[PPP] / prim[RHO]);
+
+      bool learned = guess && P.c2p_guess == DH_C2P_GUESS_LEARNED && c2p_guess_ok(guess);
+      double v2 = learned ? (double)guess[0] : v2p;
+      double eta = learned ? (double)guess[1] : etap;
+      int it = 0;
+      bool ok = noble2d(v, n, P.c2p_stop, v2, eta, it);
+      bool fell_back = false;
+      if (learned && !ok) {
+         int it2 = 0;
+         v2 = v2p;
+         eta = etap;
+         noble2d(v, n, P.c2p_stop, v2, eta, it2);
+         it += it2;
+         fell_back = true;
       }
-      v2 = v21;
-      eta = eta1;
+      if (info) {
+         int k = it < C2P_INFO_ITERS ? it : C2P_INFO_ITERS;
+         *info = (unsigned char)(k | (learned ? C2P_INFO_LEARNED : 0) |
+                                 (fell_back || (guess && !learned) ? C2P_INFO_FALLBACK : 0));
+      }
       w = 1.0 / sqrt(1 - v2);
-      double u0 = w / al;
+      double u0 = w / g.al;
       double h = eta / w;
-      double rho = D / (jac * u0);
+      double rho = v.D / (g.jac * u0);
       if (rho < P.rho_floor) rho = P.rho_floor;
       double Pp = n * rho * (h - 1.0);
       if (Pp < P.p_floor * rho) Pp = P.p_floor * rho;
       h = 1.0 + gl / (gl - 1.0) * Pp / rho;
+      const double *gam = g.gam, *B = v.B;
       double Bd[3];
       for (int i = 0; i < 3; ++i)
          Bd[i] = gam[3 * i] * B[0] + gam[3 * i + 1] * B[1] + gam[3 * i + 2] * B[2];
+      double sqrtgam = g.sqrtgam, D = v.D;
       for (int i = 0; i < 3; ++i)
-         l[i] = (S[i] + sqrtgam * BS * Bd[i] / (D * h * w)) / (D * h + sqrtgam * B2 / w);
+         l[i] = (v.S[i] + sqrtgam * v.BS * Bd[i] / (D * h * w)) / (D * h + sqrtgam * v.B2 / w);
       prim[RHO] = rho;
       prim[URR] = l[0];
       prim[UPP] = l[1];

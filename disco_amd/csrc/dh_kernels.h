@@ -13,6 +13,7 @@
 // which sums its contributions in the reference's order, so a step is
 // deterministic and needs no atomics.
 #pragma once
+#include "dh_c2pnet.h"
 #include "dh_ct.h"
 #include "dh_euler.h"
 #include "dh_geom.h"
@@ -37,6 +38,15 @@ struct DiagPlanet {
 template <class H>
 struct IsMHD {
    static constexpr bool value = H::NC == 8;
+};
+
+template <class H>
+struct IsGR {
+   static constexpr bool value = false;
+};
+template <int NQ>
+struct IsGR<GRMHD<NQ>> {
+   static constexpr bool value = true;
 };
 
 // One direction's face list: faces of annulus pair P are fI[P] .. fI[P+1]-1.
@@ -1407,6 +1417,7 @@ struct KBfacesC2p {
    double *cons, *prim;
    int type;    // 0: primitive B (set_B_fields), 1: conserved B
    int do_c2p;  // cons2prim of real cells afterwards
+   unsigned char *info;  // GRMHD: each cons2prim's record (C2P_INFO_*), or null
    DHD void operator()(long c) const
    {
       size_t nt = (size_t)m.ntot;
@@ -1463,9 +1474,97 @@ struct KBfacesC2p {
       load_q<NQ>(cons, nt, c, cs);
       double dVc = cell_dV(rp, rm, piph_new[c], piph_new[c] - dphi_new[c], zp, zm, m.Pmax);
       load_q<NQ>(prim, nt, c, out);  // the previous primitives: an iterative solve's guess
-      H::cons2prim(P, cs, out, x0, centroid_z(zp, zm), dVc);
+      if constexpr (IsGR<H>::value)
+         H::cons2prim_guess(P, cs, out, x0, centroid_z(zp, zm), dVc, nullptr,
+                            info ? info + c : nullptr);
+      else
+         H::cons2prim(P, cs, out, x0, centroid_z(zp, zm), dVc);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) prim[q * nt + c] = out[q];
+   }
+};
+
+// cons2prim of the real cells alone (the last part of KBfacesC2p), for
+// grmhd2 with a learned initial guess: guess[2c], guess[2c+1] = (v^2, eta)
+// of cell c from k_c2p_guess, which runs between the two.
+template <int NQ>
+struct KC2p {
+   Mesh m;
+   dh_params P;
+   const double *piph_new, *dphi_new, *cons;
+   double *prim;
+   const float *guess;
+   unsigned char *info;
+   DHD void operator()(long c) const
+   {
+      size_t nt = (size_t)m.ntot;
+      int a = m.cjk[c];
+      int j = a % m.Nr, k = a / m.Nr;
+      if (!is_real(m, j, k)) return;
+      double rp = m.r_jph[j], rm = m.r_jph[j - 1];
+      double zp = m.z_kph[k], zm = m.z_kph[k - 1];
+      double cs[NQ], out[NQ];
+      load_q<NQ>(cons, nt, c, cs);
+      load_q<NQ>(prim, nt, c, out);
+      double dVc = cell_dV(rp, rm, piph_new[c], piph_new[c] - dphi_new[c], zp, zm, m.Pmax);
+      GRMHD<NQ>::cons2prim_guess(P, cs, out, centroid_r(rp, rm), centroid_z(zp, zm), dVc,
+                                 guess + 2 * c, info ? info + c : nullptr);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) prim[q * nt + c] = out[q];
+   }
+};
+
+// The invariants of cell c's conserved state at its centroid, and its
+// volume, as cons2prim forms them.
+template <int NQ>
+DHD C2pInv cell_c2p_invariants(const Mesh &m, const dh_params &P, const double *cons,
+                               const double *piph, const double *dphi, long c, int j, int k)
+{
+   size_t nt = (size_t)m.ntot;
+   double rp = m.r_jph[j], rm = m.r_jph[j - 1];
+   double zp = m.z_kph[k], zm = m.z_kph[k - 1];
+   double dV = cell_dV(rp, rm, piph[c], piph[c] - dphi[c], zp, zm, m.Pmax);
+   double cs[NQ];
+   load_q<NQ>(cons, nt, c, cs);
+#pragma unroll
+   for (int q = 0; q < NQ; ++q) cs[q] /= dV;
+   double r = centroid_r(rp, rm);
+   C2pInv v;
+   c2p_invariants(gr_metric(P, r, centroid_z(zp, zm)), r, cs, v);
+   return v;
+}
+
+// Training data of the learned guess: for every real cell the model's
+// features (feat[8c..8c+7]) and targets (targ[2c], targ[2c+1]: v^2 and
+// log(eta / e)) of its current state, whose primitives cons2prim produced
+// from its conserved state; other cells get NaN.
+template <int NQ>
+struct KC2pHarvest {
+   Mesh m;
+   dh_params P;
+   const double *cons, *prim, *piph, *dphi;
+   float *feat, *targ;
+   DHD void operator()(long c) const
+   {
+      size_t nt = (size_t)m.ntot;
+      int a = m.cjk[c];
+      int j = a % m.Nr, k = a / m.Nr;
+      if (!is_real(m, j, k)) {
+         for (int i = 0; i < C2PNET_IN; ++i) feat[C2PNET_IN * c + i] = NAN;
+         targ[2 * c] = targ[2 * c + 1] = NAN;
+         return;
+      }
+      C2pInv v = cell_c2p_invariants<NQ>(m, P, cons, piph, dphi, c, j, k);
+      c2p_features(v, P.gamma_law, feat + C2PNET_IN * c);
+      double gl = P.gamma_law;
+      double l[3] = {prim[URR * nt + c], prim[UPP * nt + c], prim[UZZ * nt + c]};
+      double u2 = 0.0;
+      for (int i = 0; i < 3; ++i)
+         for (int jj = 0; jj < 3; ++jj) u2 += v.g.igam[3 * i + jj] * l[i] * l[jj];
+      double w = sqrt(1.0 + u2);
+      double eta = w * (1.0 + gl / (gl - 1.0) * prim[PPP * nt + c] / prim[RHO * nt + c]);
+      targ[2 * c] = (float)(u2 / (1.0 + u2));
+      targ[2 * c + 1] = (float)log(eta / v.e);
    }
 };
 

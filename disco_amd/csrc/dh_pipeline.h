@@ -10,10 +10,12 @@
 //   flips(m, dim, F, flag, Phi, rkPhi)   hand-overs of flagged faces per track
 //   min_dt<NQ,H>(m, P, prim, piph, dphi, wiph, &dt)
 //   diag(m, P, kind, pl1, prim, piph, dphi, dt, Qrz)   per-annulus diagnostics
+//   c2p_guess<NQ>(m, P, cons, piph, dphi, prm, frag, guess)   learned cons2prim guesses
 //   alloc/put/get/copy/zero/sync  memory
 #pragma once
 #include <cstdarg>
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -69,6 +71,13 @@ struct Buffers {
    // first use; the kind that accumulated it
    double *diagQ = nullptr;
    int diag_kind = 0;
+   // grmhd2 cons2prim: every real cell's solve record (C2P_INFO_*); with the
+   // learned guess its model (float parameters and bfloat16 MFMA fragments)
+   // and the per-cell guesses (v^2, eta)
+   unsigned char *c2p_info = nullptr;
+   float *c2p_prm = nullptr, *c2p_guess = nullptr;
+   unsigned short *c2p_frag = nullptr;
+   bool c2p_model = false;
 };
 
 // Checks a grid and parameter set against what the engine implements.
@@ -110,6 +119,13 @@ inline int validate(const dh_grid *g, const dh_params &P, long *ntot_out)
    } else {
       return fail(DH_EINVAL, "hydro module %d not supported", P.hydro);
    }
+   if (P.c2p_guess != DH_C2P_GUESS_PREVIOUS && P.c2p_guess != DH_C2P_GUESS_LEARNED)
+      return fail(DH_EINVAL, "bad c2p_guess %d", P.c2p_guess);
+   if (P.c2p_stop != DH_C2P_STOP_REFERENCE && P.c2p_stop != DH_C2P_STOP_RESIDUAL)
+      return fail(DH_EINVAL, "bad c2p_stop %d", P.c2p_stop);
+   if (P.hydro != DH_HYDRO_GRMHD &&
+       (P.c2p_guess != DH_C2P_GUESS_PREVIOUS || P.c2p_stop != DH_C2P_STOP_REFERENCE))
+      return fail(DH_EINVAL, "c2p_guess and c2p_stop apply to grmhd2 only");
    if (P.rot_frame) return fail(DH_EINVAL, "RotFrame is not supported");
    if (P.nozzle_type) return fail(DH_EINVAL, "Nozzle_Type is not supported");
    if (P.r_periodic || P.z_periodic) return fail(DH_EINVAL, "periodic boundaries are not supported");
@@ -301,6 +317,15 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
       DH_TRY(x.zero(b.emf_phi, 4 * cb));
       DH_TRY(x.zero(b.Ecarry, 16 * cb));
    }
+   if (P.hydro == DH_HYDRO_GRMHD) {
+      DH_TRY(alloc_n(x, &b.c2p_info, (size_t)ntot));
+      DH_TRY(x.zero(b.c2p_info, (size_t)ntot));
+      if (P.c2p_guess == DH_C2P_GUESS_LEARNED) {
+         DH_TRY(alloc_n(x, &b.c2p_prm, sizeof(float) * C2PNET_NPARAM));
+         DH_TRY(alloc_n(x, &b.c2p_frag, sizeof(unsigned short) * C2PNET_NFRAG));
+         DH_TRY(alloc_n(x, &b.c2p_guess, sizeof(float) * 2 * (size_t)ntot));
+      }
+   }
    b.pl[0].n = b.pl[1].n = 0;
    return DH_OK;
 }
@@ -447,9 +472,20 @@ struct Engine {
                DH_TRY(x.flips(m, dim, b.F[d], b.flag[d], b.Phi, b.rkPhi));
             }
          }
+         // with the learned guess cons2prim waits for the model's guesses,
+         // which need the cells' conserved field first
+         bool learned = IsGR<H>::value && P.c2p_guess == DH_C2P_GUESS_LEARNED;
          KBfacesC2p<NQ, H> kbc{m, P, b.F[0], b.F[1], b.rng[0], b.rng[1], b.piph2, b.dphi2,
-                               b.Phi, b.cons, b.prim, 1, 1};
+                               b.Phi, b.cons, b.prim, 1, learned ? 0 : 1, b.c2p_info};
          DH_TRY(x.run("bfaces_c2p", kbc, m.ntot));
+         if constexpr (IsGR<H>::value) {
+            if (learned) {
+               DH_TRY(x.template c2p_guess<NQ>(m, P, b.cons, b.piph2, b.dphi2, b.c2p_prm,
+                                               b.c2p_frag, b.c2p_guess));
+               KC2p<NQ> kc{m, P, b.piph2, b.dphi2, b.cons, b.prim, b.c2p_guess, b.c2p_info};
+               DH_TRY(x.run("c2p", kc, m.ntot));
+            }
+         }
       }
       std::swap(b.piph, b.piph2);
       std::swap(b.dphi, b.dphi2);
@@ -460,6 +496,8 @@ struct Engine {
    template <int NQ, class H>
    int timestep(double dt)
    {
+      if (b.P.c2p_guess == DH_C2P_GUESS_LEARNED && !b.c2p_model)
+         return fail(DH_EINVAL, "the learned cons2prim guess has no model (dh_set_c2p_model)");
       DH_TRY(x.copy(b.rkcons, b.cons, sizeof(double) * NQ * b.m.ntot));
       if (b.mhd) DH_TRY(x.copy(b.rkPhi, b.Phi, sizeof(double) * 5 * b.m.ntot));
       DH_TRY((substep<NQ, H>(0.0, dt, 1, 0)));
@@ -482,7 +520,7 @@ struct Engine {
          // and ghost cells their boundary values, so no trace of it remains.
          DH_TRY(topology<NQ>(false));
          KBfacesC2p<NQ, H> k1{m, b.P, b.F[0], b.F[1], b.rng[0], b.rng[1], b.piph, b.dphi,
-                              b.Phi, b.cons, b.prim, 1, 1};
+                              b.Phi, b.cons, b.prim, 1, 1, nullptr};
          DH_TRY(x.run("init", k1, m.ntot));
       }
       DH_TRY(x.run("init", KWcell<NQ, H>{m, b.P, b.prim, b.wiph}, m.ntot));
@@ -698,6 +736,91 @@ int export_faces(X &x, Buffers &b, int dim, int *L, int *R, int *LRtype, double 
          if (dA) dA[f] = hdA[f];
       }
    }
+   return DH_OK;
+}
+
+// The learned cons2prim guess's parameters (dh_set_c2p_model): validated,
+// kept in float32 for the biases and normalisation and as bfloat16 MFMA
+// fragments for the weights.
+template <class X>
+int set_c2p_model(X &x, Buffers &b, const float *prm, long n)
+{
+   if (b.P.hydro != DH_HYDRO_GRMHD || b.P.c2p_guess != DH_C2P_GUESS_LEARNED)
+      return fail(DH_EINVAL, "the context was not created with c2p_guess = DH_C2P_GUESS_LEARNED");
+   if (!prm || n != C2PNET_NPARAM)
+      return fail(DH_EINVAL, "a c2p model has %d parameters (got %ld)", (int)C2PNET_NPARAM, n);
+   for (long i = 0; i < n; ++i)
+      if (!std::isfinite(prm[i])) return fail(DH_EINVAL, "c2p model parameter %ld is not finite", i);
+   for (int i = 0; i < C2PNET_IN; ++i)
+      if (!(prm[C2PNET_IN_SD + i] != 0.0f)) return fail(DH_EINVAL, "c2p model in_sd[%d] is zero", i);
+   std::vector<unsigned short> frag(C2PNET_NFRAG);
+   c2pnet_fragments(prm, frag.data());
+   DH_TRY(x.put(b.c2p_prm, prm, sizeof(float) * C2PNET_NPARAM));
+   DH_TRY(x.put(b.c2p_frag, frag.data(), sizeof(unsigned short) * C2PNET_NFRAG));
+   b.c2p_model = true;
+   return DH_OK;
+}
+
+// Training data of the learned guess from the current state (KC2pHarvest):
+// feat[8 * ntot], targ[2 * ntot] host arrays, NaN outside the real cells.
+template <class X>
+int c2p_harvest(X &x, Buffers &b, float *feat, float *targ)
+{
+   if (b.P.hydro != DH_HYDRO_GRMHD) return fail(DH_EINVAL, "c2p harvest needs grmhd2");
+   if (!feat || !targ) return fail(DH_EINVAL, "null argument");
+   size_t nt = (size_t)b.m.ntot;
+   float *df = nullptr, *dt = nullptr;
+   DH_TRY(alloc_n(x, &df, sizeof(float) * C2PNET_IN * nt));
+   DH_TRY(alloc_n(x, &dt, sizeof(float) * 2 * nt));
+   int rc = DH_OK;
+   if (b.nq == 8)
+      rc = x.run("c2p_harvest", KC2pHarvest<8>{b.m, b.P, b.cons, b.prim, b.piph, b.dphi, df, dt}, b.m.ntot);
+   else
+      rc = x.run("c2p_harvest", KC2pHarvest<9>{b.m, b.P, b.cons, b.prim, b.piph, b.dphi, df, dt}, b.m.ntot);
+   if (!rc) rc = x.get(feat, df, sizeof(float) * C2PNET_IN * nt);
+   if (!rc) rc = x.get(targ, dt, sizeof(float) * 2 * nt);
+   int rf = x.release(df), rt = x.release(dt);
+   return rc ? rc : (rf ? rf : rt);
+}
+
+// The model's guesses (v^2, eta) for the current state, guess[2 * ntot]
+// (NaN outside the real cells): the values cons2prim would start from.
+template <class X>
+int c2p_predict(X &x, Buffers &b, float *guess)
+{
+   if (!b.c2p_model) return fail(DH_EINVAL, "no learned cons2prim model (dh_set_c2p_model)");
+   if (!guess) return fail(DH_EINVAL, "null argument");
+   if (b.nq == 8)
+      DH_TRY(x.template c2p_guess<8>(b.m, b.P, b.cons, b.piph, b.dphi, b.c2p_prm, b.c2p_frag,
+                                     b.c2p_guess));
+   else
+      DH_TRY(x.template c2p_guess<9>(b.m, b.P, b.cons, b.piph, b.dphi, b.c2p_prm, b.c2p_frag,
+                                     b.c2p_guess));
+   return x.get(guess, b.c2p_guess, sizeof(float) * 2 * (size_t)b.m.ntot);
+}
+
+// Totals over the real cells of the last cons2prim's records: {cells,
+// Newton iterations, cells solved from the learned guess, cells that fell
+// back to the previous primitives}.
+template <class X>
+int c2p_stats(X &x, Buffers &b, long long *out)
+{
+   if (b.P.hydro != DH_HYDRO_GRMHD) return fail(DH_EINVAL, "c2p statistics need grmhd2");
+   if (!out) return fail(DH_EINVAL, "null argument");
+   std::vector<unsigned char> info((size_t)b.m.ntot);
+   DH_TRY(x.get(info.data(), b.c2p_info, info.size()));
+   long long s[4] = {0, 0, 0, 0};
+   for (int a = 0; a < b.m.nann; ++a) {
+      if (!is_real(b.m, a % b.m.Nr, a / b.m.Nr)) continue;
+      for (int c = b.h_off[a]; c < b.h_off[a + 1]; ++c) {
+         unsigned char v = info[c];
+         s[0]++;
+         s[1] += v & C2P_INFO_ITERS;
+         s[2] += (v & C2P_INFO_LEARNED) ? 1 : 0;
+         s[3] += (v & C2P_INFO_FALLBACK) ? 1 : 0;
+      }
+   }
+   for (int i = 0; i < 4; ++i) out[i] = s[i];
    return DH_OK;
 }
 

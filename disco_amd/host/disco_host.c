@@ -53,7 +53,16 @@ struct disco_accel {
    int bc[4];     /* DH_BC_* per side */
    int host_dirty; /* mirror older than the device state */
    int diag_kind;  /* DH_DIAG_* of the Analysis module */
+   /* grmhd2 cons2prim options (in.par C2P_Guess, C2P_Stop, C2P_Model) */
+   int c2p_guess, c2p_stop;
+   char c2p_model[512];
 };
+
+static struct disco_accel *accel_of(struct domain *d)
+{
+   if (!d->accel) d->accel = calloc(1, sizeof(struct disco_accel));
+   return (struct disco_accel *)d->accel;
+}
 
 static int diag_kind_of_module(const char *name)
 {
@@ -112,6 +121,30 @@ static int par_lookup(const char *file, const char *key, int type, void *out)
    if (type == V_INT) *(int *)out = (int)d;
    else *(double *)out = d;
    return 0;
+}
+
+/* A string value: the first word after the key and its separators. */
+static int par_lookup_str(const char *file, const char *key, char *out, size_t cap)
+{
+   FILE *f = fopen(file, "r");
+   if (!f) return 1;
+   char line[512], name[512];
+   int found = 0;
+   while (!found && fgets(line, sizeof line, f)) {
+      name[0] = 0;
+      if (sscanf(line, "%511s", name) == 1 && strcmp(name, key) == 0) {
+         const char *v = line + strspn(line, " \t") + strlen(key);
+         v += strspn(v, "\t :=>");
+         size_t n = strcspn(v, " \t\r\n");
+         if (n > 0 && n < cap) {
+            memcpy(out, v, n);
+            out[n] = 0;
+            found = 1;
+         }
+      }
+   }
+   fclose(f);
+   return found ? 0 : 1;
 }
 
 int disco_read_par(struct domain *d, const char *file)
@@ -209,6 +242,12 @@ int disco_read_par(struct domain *d, const char *file)
    RD("Nozzle_Par3", V_DBL, &L->nozzlePar3);
    RD("Nozzle_Par4", V_DBL, &L->nozzlePar4);
 #undef RD
+   /* Options of this path with no reference key; absent, the reference's
+    * cons2prim (previous primitives as guess, its stopping rule). */
+   struct disco_accel *A = accel_of(d);
+   par_lookup(file, "C2P_Guess", V_INT, &A->c2p_guess);
+   par_lookup(file, "C2P_Stop", V_INT, &A->c2p_stop);
+   par_lookup_str(file, "C2P_Model", A->c2p_model, sizeof A->c2p_model);
    if (t2pi) { L->t_min *= 2. * M_PI; L->t_max *= 2. * M_PI; }
    if (p2pi) L->phimax *= 2. * M_PI;
    if (zpi) { L->zmin *= M_PI; L->zmax *= M_PI; }
@@ -338,6 +377,9 @@ static void fill_params(struct domain *d, dh_params *P, const int *bc)
                : !strcmp(METRIC, "schw_ks_eq_rpz") ? DH_METRIC_SCHW_KS_EQ
                : !strcmp(METRIC, "schw_ks_rpz")    ? DH_METRIC_SCHW_KS
                                                    : -1; /* refused at dh_create */
+   const struct disco_accel *A = (const struct disco_accel *)d->accel;
+   P->c2p_guess = A ? A->c2p_guess : DH_C2P_GUESS_PREVIOUS;
+   P->c2p_stop = A ? A->c2p_stop : DH_C2P_STOP_REFERENCE;
    P->riemann_solver = L->Riemann_Solver;
    P->mesh_motion = L->Mesh_Motion;
    P->exact_mesh_omega = L->Exact_Mesh_Omega;
@@ -440,6 +482,24 @@ static int set_B_fields(struct domain *d)
    return rc ? host_fail("dh_upload_state: %s", dh_last_error()) : 0;
 }
 
+/* The learned cons2prim guess's parameters: a raw little-endian float32
+ * file (in.par C2P_Model) of exactly the model's parameter count. */
+static int load_c2p_model(struct disco_accel *A)
+{
+   enum { NPARAM = 4886 };
+   if (!A->c2p_model[0]) return host_fail("C2P_Guess 1 needs C2P_Model (a model file)");
+   FILE *f = fopen(A->c2p_model, "rb");
+   if (!f) return host_fail("cannot open C2P_Model %s", A->c2p_model);
+   float *w = (float *)malloc(sizeof(float) * (NPARAM + 1));
+   size_t n = fread(w, sizeof(float), NPARAM + 1, f);
+   fclose(f);
+   int rc = 0;
+   if (n != NPARAM) rc = host_fail("C2P_Model %s: %zu floats, expected %d", A->c2p_model, n, NPARAM);
+   else if (dh_set_c2p_model(A->ctx, w, NPARAM)) rc = host_fail("dh_set_c2p_model: %s", dh_last_error());
+   free(w);
+   return rc;
+}
+
 int disco_setup(struct domain *d)
 {
    struct param_list *L = &d->theParList;
@@ -450,8 +510,7 @@ int disco_setup(struct domain *d)
    disco_setup_grid(d);
    int Nr = d->Nr, Nz = d->Nz;
    int *Np = d->Np;
-   struct disco_accel *A = (struct disco_accel *)calloc(1, sizeof *A);
-   d->accel = A;
+   struct disco_accel *A = accel_of(d);
    A->off = (long *)calloc(Nr * Nz + 1, sizeof(long));
    for (int a = 0; a < Nr * Nz; ++a) A->off[a + 1] = A->off[a] + Np[a];
    A->ntot = A->off[Nr * Nz];
@@ -567,6 +626,7 @@ int disco_setup(struct domain *d)
    dh_params P;
    fill_params(d, &P, A->bc);
    if (dh_create(&g, &P, &A->ctx)) return host_fail("dh_create: %s", dh_last_error());
+   if (P.c2p_guess == DH_C2P_GUESS_LEARNED && load_c2p_model(A)) return 1;
    long nt = A->ntot;
    double *prim = (double *)malloc(sizeof(double) * nt * NUM_Q);
    double *piph = (double *)malloc(sizeof(double) * nt);

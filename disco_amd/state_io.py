@@ -186,11 +186,16 @@ PRESETS: Dict[str, Dict[str, object]] = {
 }
 
 
+# Keys of this path with no reference counterpart (grmhd2 cons2prim's guess,
+# stopping rule and model file); written only when given.
+EXTRA_KEYS = ("C2P_Guess", "C2P_Stop", "C2P_Model")
+
+
 def format_par(overrides: Optional[Dict[str, object]] = None) -> str:
     vals = dict(_DEFAULTS)
     if overrides:
         for k, v in overrides.items():
-            if k not in vals:
+            if k not in vals and k not in EXTRA_KEYS:
                 raise KeyError(f"unknown in.par key {k!r}")
             vals[k] = v
     lines = []

@@ -62,6 +62,17 @@ enum {
    DH_METRIC_SCHW_KS = 2   /* schw_ks_rpz.c: the same metric at (r, z) */
 };
 
+/* The initial guess and stopping rule of grmhd2's cons2prim (noble2d,
+ * grmhd2.c:1096-1314).  DH_C2P_GUESS_PREVIOUS and DH_C2P_STOP_REFERENCE are
+ * the reference's: the previous primitives, and |d eta/eta| < 1e-12 followed
+ * by 10 more iterations.  DH_C2P_GUESS_LEARNED takes (v^2, eta) from the
+ * model given to dh_set_c2p_model (no counterpart in the reference), falling
+ * back to the previous primitives where its output is unphysical or the solve
+ * from it does not converge; DH_C2P_STOP_RESIDUAL stops when both residuals
+ * are below 1e-10 relative, without the extra iterations. */
+enum { DH_C2P_GUESS_PREVIOUS = 0, DH_C2P_GUESS_LEARNED = 1 };
+enum { DH_C2P_STOP_REFERENCE = 0, DH_C2P_STOP_RESIDUAL = 1 };
+
 /* Boundary treatment of one side of the mesh, applied to ghost annuli after
  * every substep (reference onestep.c:194-210 -> Boundary/<name>.c).
  *   DH_BC_HELD:    ghost prims are left as uploaded.  This is exactly the
@@ -104,7 +115,8 @@ typedef struct dh_params {
    int bc_rmin, bc_rmax, bc_zmin, bc_zmax;  /* DH_BC_* per side */
    int nozzle_type;             /* Nozzle_Type (only 0 is supported) */
    int metric;                  /* DH_METRIC_* (the METRIC module) */
-   int pad_[2];
+   int c2p_guess;               /* DH_C2P_GUESS_*: grmhd2 cons2prim's initial guess */
+   int c2p_stop;                /* DH_C2P_STOP_*: grmhd2 cons2prim's stopping rule */
    double gamma_law;            /* Adiabatic_Index */
    double plm;                  /* PLM */
    double cfl;                  /* CFL */
@@ -196,6 +208,24 @@ int dh_get_diagnostics(dh_ctx *ctx, double *Qrz);
 int dh_zero_diagnostics(dh_ctx *ctx);
 /* Divides the accumulated Qrz by t_avg (avg_diagnostics, analysis.c:28-49). */
 int dh_avg_diagnostics(dh_ctx *ctx, double t_avg);
+
+/* The learned initial guess of grmhd2's cons2prim (no counterpart in the
+ * reference: an MLP 8 -> 64 -> 64 -> 2 run on MFMA).
+ * dh_set_c2p_model loads its n = 4886 float parameters (layout in
+ * disco_amd/csrc/dh_c2pnet.h) into a context created with
+ * c2p_guess = DH_C2P_GUESS_LEARNED; dh_timestep refuses to run such a
+ * context before.  dh_c2p_harvest writes the model's features
+ * feat[8 * ntot] and targets targ[2 * ntot] of the current state (NaN
+ * outside the real cells), the training data.  dh_c2p_stats sums the last
+ * cons2prim's records over the real cells: out[4] = {cells, Newton
+ * iterations, cells solved from the learned guess, cells that fell back to
+ * the previous primitives}.  dh_c2p_predict writes the model's guesses
+ * guess[2 * ntot] = (v^2, eta) per cell for the current state (NaN outside
+ * the real cells). */
+int dh_set_c2p_model(dh_ctx *ctx, const float *params, long n);
+int dh_c2p_harvest(dh_ctx *ctx, float *feat, float *targ);
+int dh_c2p_predict(dh_ctx *ctx, float *guess);
+int dh_c2p_stats(dh_ctx *ctx, long long *out);
 
 /* Average device time (ms) of the kernels of the last `dh_timestep` calls,
  * accumulated per kernel family since dh_reset_timers; used by the benchmark

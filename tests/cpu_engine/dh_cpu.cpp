@@ -89,6 +89,37 @@ struct HostExec {
       *out = v;
       return DH_OK;
    }
+   // The learned cons2prim guesses, cell by cell, with the device's bfloat16
+   // rounding (c2pnet_eval_host) in place of its MFMA products.
+   template <int NQ>
+   int c2p_guess(const Mesh &m, const dh_params &P, const double *cons, const double *piph,
+                 const double *dphi, const float *prm, const unsigned short *, float *guess)
+   {
+      for (long c = 0; c < m.ntot; ++c) {
+         int a = m.cjk[c];
+         int j = a % m.Nr, k = a / m.Nr;
+         if (!is_real(m, j, k)) {
+            guess[2 * c] = guess[2 * c + 1] = NAN;
+            continue;
+         }
+         C2pInv v = cell_c2p_invariants<NQ>(m, P, cons, piph, dphi, c, j, k);
+         float f[C2PNET_IN], t[C2PNET_OUT];
+         c2p_features(v, P.gamma_law, f);
+         c2pnet_eval_host(prm, f, t);
+         c2p_targets_to_guess(t, v.e, guess + 2 * c);
+      }
+      return DH_OK;
+   }
+   int release(void *p)
+   {
+      for (size_t i = 0; i < allocs.size(); ++i)
+         if (allocs[i] == p) {
+            allocs.erase(allocs.begin() + (long)i);
+            free(p);
+            return DH_OK;
+         }
+      return p ? fail(DH_EINVAL, "release of an unknown pointer") : DH_OK;
+   }
    int alloc(void **p, size_t bytes)
    {
       *p = calloc(bytes ? bytes : 8, 1);
@@ -314,6 +345,30 @@ int dh_avg_diagnostics(dh_ctx *c, double t_avg)
 {
    if (!c) return fail(DH_EINVAL, "null context");
    return avg_diagnostics(c->x, c->b, t_avg);
+}
+
+int dh_set_c2p_model(dh_ctx *c, const float *params, long n)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return set_c2p_model(c->x, c->b, params, n);
+}
+
+int dh_c2p_harvest(dh_ctx *c, float *feat, float *targ)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return c2p_harvest(c->x, c->b, feat, targ);
+}
+
+int dh_c2p_predict(dh_ctx *c, float *guess)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return c2p_predict(c->x, c->b, guess);
+}
+
+int dh_c2p_stats(dh_ctx *c, long long *out)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return c2p_stats(c->x, c->b, out);
 }
 
 int dh_enable_timers(dh_ctx *c, int) { return c ? DH_OK : fail(DH_EINVAL, "null context"); }
