@@ -7,9 +7,10 @@ noble2d (grmhd2.c:1096-1314) solves for (v^2, eta) given four invariants of
 the conserved state; the model (disco_amd/csrc/dh_c2pnet.h: 8 -> 64 -> 64 ->
 2, ReLU, bfloat16 weights on MFMA) maps features of those invariants to the
 solution.  Training data come from GR disk runs through the host layer
-(`harvest`, dh_c2p_harvest after every step); `fit` trains the MLP in float32
-with torch and returns the parameter block; `save`/`load` keep it as raw
-little-endian float32, the file the host layer's in.par key C2P_Model names.
+(`harvest`, dh_c2p_harvest after every step); `fit` trains the MLP with the
+engine's own float32 forward, backward and Adam kernels (dh_c2p_train) and
+returns the parameter block; `save`/`load` keep it as raw little-endian
+float32, the file the host layer's in.par key C2P_Model names.
 """
 from __future__ import annotations
 
@@ -105,48 +106,40 @@ def harvest(runs: Iterable[Tuple[str, dict]], steps: int, size: Tuple[int, int],
     return np.concatenate(xs), np.concatenate(ys)
 
 
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
 def fit(X: np.ndarray, Y: np.ndarray, epochs: int = 40, batch: int = 2048, lr: float = 3e-3,
-        seed: int = 0) -> np.ndarray:
-    """Fits the MLP to (features, targets) in float32 with Adam and returns
-    the parameter block (dh_c2pnet.h layout)."""
-    import torch
-    torch.manual_seed(seed)
-    X = np.asarray(X, np.float32)
-    Y = np.asarray(Y, np.float32)
-    in_mu, in_sd = X.mean(0), X.std(0)
-    in_sd[in_sd < 1e-6] = 1.0  # a constant feature (the adiabatic index of a run set)
-    out_mu, out_sd = Y.mean(0), Y.std(0)
-    out_sd[out_sd < 1e-12] = 1.0
-    xn = torch.from_numpy((X - in_mu) / in_sd)
-    yn = torch.from_numpy((Y - out_mu) / out_sd)
-    net = torch.nn.Sequential(torch.nn.Linear(N_IN, N_H), torch.nn.ReLU(),
-                              torch.nn.Linear(N_H, N_H), torch.nn.ReLU(),
-                              torch.nn.Linear(N_H, N_OUT))
-    opt = torch.optim.Adam(net.parameters(), lr=lr)
-    sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, epochs)
-    n = len(xn)
-    g = torch.Generator().manual_seed(seed)
-    for ep in range(epochs):
-        perm = torch.randperm(n, generator=g)
-        tot = 0.0
-        for i in range(0, n, batch):
-            idx = perm[i:i + batch]
-            loss = torch.mean((net(xn[idx]) - yn[idx]) ** 2)
-            opt.zero_grad()
-            loss.backward()
-            opt.step()
-            tot += float(loss.detach()) * len(idx)
-        sched.step()
-        if ep % 10 == 0 or ep == epochs - 1:
-            print(f"epoch {ep}: mse {tot / n:.3e}", flush=True)
-    w = [m for m in net if isinstance(m, torch.nn.Linear)]
-    parts = [in_mu, in_sd]
-    for lin in w:
-        parts += [lin.weight.detach().numpy().T.ravel(), lin.bias.detach().numpy()]
-    parts += [out_mu, out_sd]
-    p = np.concatenate([np.asarray(a, np.float32).ravel() for a in parts])
-    assert p.size == N_PARAM
-    return p
+        seed: int = 0, library=None) -> Tuple[np.ndarray, float]:
+    """Trains the model on (features, targets) with the engine's own
+    training kernels (dh_c2p_train: on the GPU, or on the CPU with the host
+    build as `library`); returns the parameter block and the last epoch's
+    mean squared error (normalised targets)."""
+    from . import capi
+    lib = library if library is not None else capi.lib()
+    X = np.ascontiguousarray(X, np.float32)
+    Y = np.ascontiguousarray(Y, np.float32)
+    p = np.zeros(N_PARAM, np.float32)
+    loss = C.c_float()
+    if lib.dh_c2p_train(_fp(X), _fp(Y), len(X), epochs, batch, lr, seed, _fp(p), C.byref(loss)):
+        raise RuntimeError(f"dh_c2p_train: {lib.dh_last_error().decode()}")
+    return p, loss.value
+
+
+def grad(params: np.ndarray, X: np.ndarray, Y: np.ndarray, library=None):
+    """(gradient of the mean squared error over all rows with respect to the
+    parameter block, the error) from the engine's kernels (dh_c2p_grad)."""
+    from . import capi
+    lib = library if library is not None else capi.lib()
+    params = np.ascontiguousarray(params, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    Y = np.ascontiguousarray(Y, np.float32)
+    g = np.zeros(N_PARAM, np.float32)
+    loss = C.c_float()
+    if lib.dh_c2p_grad(_fp(params), _fp(X), _fp(Y), len(X), _fp(g), C.byref(loss)):
+        raise RuntimeError(f"dh_c2p_grad: {lib.dh_last_error().decode()}")
+    return g, loss.value
 
 
 def predict(params: np.ndarray, X: np.ndarray) -> np.ndarray:
@@ -194,7 +187,9 @@ def main(argv: Optional[List[str]] = None) -> int:
     print(f"harvested {len(X)} samples in {time.time() - t0:.1f} s", flush=True)
     rng = np.random.default_rng(1)
     test = rng.random(len(X)) < 0.1
-    p = fit(X[~test], Y[~test], epochs=a.epochs)
+    t0 = time.time()
+    p, loss = fit(X[~test], Y[~test], epochs=a.epochs)
+    print(f"trained on the GPU in {time.time() - t0:.1f} s: last-epoch mse {loss:.3e}", flush=True)
     print("held-out errors:", report(p, X[test], Y[test]), flush=True)
     save(p, a.out)
     print(f"wrote {a.out}", flush=True)

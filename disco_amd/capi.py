@@ -144,6 +144,11 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "dh_c2p_harvest": ([vp, C.POINTER(C.c_float), C.POINTER(C.c_float)], C.c_int),
         "dh_c2p_predict": ([vp, C.POINTER(C.c_float)], C.c_int),
         "dh_c2p_stats": ([vp, C.POINTER(C.c_longlong)], C.c_int),
+        "dh_c2p_train": ([C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_long, C.c_int,
+                          C.c_int, C.c_float, C.c_ulonglong, C.POINTER(C.c_float),
+                          C.POINTER(C.c_float)], C.c_int),
+        "dh_c2p_grad": ([C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                         C.c_long, C.POINTER(C.c_float), C.POINTER(C.c_float)], C.c_int),
         "dh_last_error": ([], C.c_char_p),
     }
     for name, (args, res) in sig.items():
@@ -158,7 +163,8 @@ EXPORTED_SYMBOLS = ["dh_create", "dh_destroy", "dh_sizes", "dh_upload_state",
                     "dh_face_count", "dh_get_faces", "dh_sync", "dh_enable_timers",
                     "dh_read_timers", "dh_add_diagnostics", "dh_get_diagnostics",
                     "dh_zero_diagnostics", "dh_avg_diagnostics", "dh_set_c2p_model",
-                    "dh_c2p_harvest", "dh_c2p_predict", "dh_c2p_stats", "dh_last_error"]
+                    "dh_c2p_harvest", "dh_c2p_predict", "dh_c2p_stats", "dh_c2p_train",
+                    "dh_c2p_grad", "dh_last_error"]
 
 _LIB: Optional[C.CDLL] = None
 

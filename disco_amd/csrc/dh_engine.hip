@@ -612,6 +612,23 @@ static int do_faces(dh_ctx *c)
 // C ABI (include/disco_hip.h)
 // ---------------------------------------------------------------------------
 
+// The training entry points own a temporary executor (and, on the GPU, a
+// stream) for the call.
+static int with_exec(int (*fn)(dh_ctx *, void *), void *arg)
+{
+   int ndev = 0;
+   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(DH_ENODEV, "no GPU visible");
+   dh_ctx *c = new dh_ctx();
+   if (hipStreamCreate(&c->x.stream) != hipSuccess) {
+      delete c;
+      return fail(DH_EHIP, "hipStreamCreate failed");
+   }
+   int rc = fn(c, arg);
+   if (!rc) rc = c->x.sync();
+   release(c);
+   return rc;
+}
+
 extern "C" {
 
 const char *dh_last_error(void) { return g_err; }
@@ -756,6 +773,43 @@ int dh_c2p_predict(dh_ctx *c, float *guess)
 {
    if (!c) return fail(DH_EINVAL, "null context");
    return c2p_predict(c->x, c->b, guess);
+}
+
+
+struct TrainArgs {
+   const float *X, *Y;
+   long n;
+   int epochs, batch;
+   float lr;
+   unsigned long long seed;
+   float *params, *loss;
+};
+
+struct GradArgs {
+   const float *params, *X, *Y;
+   long n;
+   float *grad, *loss;
+};
+
+int dh_c2p_train(const float *X, const float *Y, long n, int epochs, int batch, float lr,
+                 unsigned long long seed, float *params, float *loss)
+{
+   TrainArgs a{X, Y, n, epochs, batch, lr, seed, params, loss};
+   return with_exec([](dh_ctx *c, void *p) {
+      TrainArgs *t = (TrainArgs *)p;
+      return c2p_train(c->x, t->X, t->Y, t->n, t->epochs, t->batch, t->lr, t->seed, t->params,
+                       t->loss);
+   }, &a);
+}
+
+int dh_c2p_grad(const float *params, const float *X, const float *Y, long n, float *grad,
+                float *loss)
+{
+   GradArgs a{params, X, Y, n, grad, loss};
+   return with_exec([](dh_ctx *c, void *p) {
+      GradArgs *t = (GradArgs *)p;
+      return c2p_grad(c->x, t->params, t->X, t->Y, t->n, t->grad, t->loss);
+   }, &a);
 }
 
 int dh_c2p_stats(dh_ctx *c, long long *out)

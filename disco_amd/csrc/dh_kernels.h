@@ -49,6 +49,19 @@ struct IsGR<GRMHD<NQ>> {
    static constexpr bool value = true;
 };
 
+// A kernel run in another order: thread t does element order[t].  Every
+// element's work is independent of when it runs (owner-computes, no
+// atomics), so only memory locality changes; the order is a permutation of
+// the kernel's elements, and its workgroups go to the XCDs in contiguous
+// runs so that neighbouring elements of the order share an L2.
+template <class K>
+struct Ordered {
+   K k;
+   const int *order;
+   static constexpr bool xcd_contiguous = true;
+   DHD void operator()(long t) const { k(order[t]); }
+};
+
 // One direction's face list: faces of annulus pair P are fI[P] .. fI[P+1]-1.
 struct Faces {
    int n, npairs;

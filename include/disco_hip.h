@@ -227,6 +227,20 @@ int dh_c2p_harvest(dh_ctx *ctx, float *feat, float *targ);
 int dh_c2p_predict(dh_ctx *ctx, float *guess);
 int dh_c2p_stats(dh_ctx *ctx, long long *out);
 
+/* Training of the learned guess on the device (no context): float32
+ * forward and backward passes, deterministic per-parameter gradient sums
+ * and Adam (disco_amd/csrc/dh_c2ptrain.h).  dh_c2p_train fits the model to
+ * n rows of features X[n][8] and targets Y[n][2] (as dh_c2p_harvest writes
+ * them) for `epochs` passes in batches of `batch` rows at learning rate lr
+ * (cosine-decayed), from `seed`; it writes the 4886-float parameter block
+ * and the last epoch's mean squared error.  dh_c2p_grad writes the gradient
+ * of the mean squared error over all n rows with respect to the parameter
+ * block (zero at the normalisation constants) and the error itself. */
+int dh_c2p_train(const float *X, const float *Y, long n, int epochs, int batch, float lr,
+                 unsigned long long seed, float *params, float *loss);
+int dh_c2p_grad(const float *params, const float *X, const float *Y, long n, float *grad,
+                float *loss);
+
 /* Average device time (ms) of the kernels of the last `dh_timestep` calls,
  * accumulated per kernel family since dh_reset_timers; used by the benchmark
  * to report the dominant kernel.  names/ms/calls receive up to `cap` rows. */

@@ -234,6 +234,15 @@ extern "C" int dhx_last_true(int N, int g, const unsigned char *truth)
    return last_true(N, g, [&](int m) { return truth[m] != 0; });
 }
 
+// The training entry points own a temporary executor for the call.
+static int with_exec(int (*fn)(dh_ctx *, void *), void *arg)
+{
+   dh_ctx *c = new dh_ctx();
+   int rc = fn(c, arg);
+   release(c);
+   return rc;
+}
+
 extern "C" {
 
 const char *dh_last_error(void) { return g_err; }
@@ -363,6 +372,43 @@ int dh_c2p_predict(dh_ctx *c, float *guess)
 {
    if (!c) return fail(DH_EINVAL, "null context");
    return c2p_predict(c->x, c->b, guess);
+}
+
+
+struct TrainArgs {
+   const float *X, *Y;
+   long n;
+   int epochs, batch;
+   float lr;
+   unsigned long long seed;
+   float *params, *loss;
+};
+
+struct GradArgs {
+   const float *params, *X, *Y;
+   long n;
+   float *grad, *loss;
+};
+
+int dh_c2p_train(const float *X, const float *Y, long n, int epochs, int batch, float lr,
+                 unsigned long long seed, float *params, float *loss)
+{
+   TrainArgs a{X, Y, n, epochs, batch, lr, seed, params, loss};
+   return with_exec([](dh_ctx *c, void *p) {
+      TrainArgs *t = (TrainArgs *)p;
+      return c2p_train(c->x, t->X, t->Y, t->n, t->epochs, t->batch, t->lr, t->seed, t->params,
+                       t->loss);
+   }, &a);
+}
+
+int dh_c2p_grad(const float *params, const float *X, const float *Y, long n, float *grad,
+                float *loss)
+{
+   GradArgs a{params, X, Y, n, grad, loss};
+   return with_exec([](dh_ctx *c, void *p) {
+      GradArgs *t = (GradArgs *)p;
+      return c2p_grad(c->x, t->params, t->X, t->Y, t->n, t->grad, t->loss);
+   }, &a);
 }
 
 int dh_c2p_stats(dh_ctx *c, long long *out)

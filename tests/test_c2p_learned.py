@@ -58,7 +58,7 @@ def trained(host):
         ys.append(t[k])
     g.close()
     X, Y = np.concatenate(xs), np.concatenate(ys)
-    return c2p_model.fit(X, Y, epochs=4, batch=256), X, Y
+    return c2p_model.fit(X, Y, epochs=4, batch=256, library=host)[0], X, Y
 
 
 def test_harvested_targets_solve_noble2d(trained):
@@ -127,20 +127,31 @@ def test_unphysical_model_falls_back_bit_for_bit(host, trained):
 
 @pytest.mark.parametrize("guess,stop", [(1, 0), (0, 1), (1, 1)])
 def test_learned_guess_and_residual_rule_match_reference_rule(host, trained, guess, stop):
-    """Runs from the learned guess and/or with the residual rule agree with
-    the reference's cons2prim to 1e-13; the reference rule costs its 10
-    extra iterations per cell, the residual rule does not."""
-    ref, ref_stats = _run(host, {})
-    out, stats = _run(host, {"C2P_Guess": guess, "C2P_Stop": stop},
-                      trained[0] if guess else None)
-    worst = max(field_errors(out, ref).values())
-    assert worst < 1e-13, worst
-    for s, r in zip(stats, ref_stats):
+    """Each step from the reference rule's run's state, with the learned
+    guess and/or the residual rule, agrees with the reference rule's step to
+    1e-13; the reference rule costs its 10 extra iterations per cell, the
+    residual rule does not."""
+    cfg, par, a = _initial()
+    g = capi.DiscoHip(cfg, par, a, library=host)
+    for _ in range(4):
+        s = g.download(a)
+        h = capi.DiscoHip(cfg, {**par, "C2P_Guess": guess, "C2P_Stop": stop}, s, library=host)
         if guess:
-            assert s["learned"] + s["fallbacks"] == s["cells"] and s["learned"] > 0.9 * s["cells"], s
-        it = s["iterations"] / s["cells"]
+            h.set_c2p_model(trained[0])
+        h.timestep(0.05)
+        out, st = h.download(a), h.c2p_stats()
+        h.close()
+        g.timestep(0.05)
+        ref, rst = g.download(a), g.c2p_stats()
+        worst = max(field_errors(out, ref).values())
+        assert worst < 1e-13, worst
+        if guess:
+            assert st["learned"] + st["fallbacks"] == st["cells"]
+            assert st["learned"] > 0.9 * st["cells"], st
+        it = st["iterations"] / st["cells"]
         assert (it >= 11.0) if stop == 0 else (it < 6.0), it
-    assert min(r["iterations"] / r["cells"] for r in ref_stats) >= 11.0
+        assert rst["iterations"] / rst["cells"] >= 11.0
+    g.close()
 
 
 def test_model_is_validated(host, trained):
@@ -172,3 +183,38 @@ def test_model_file_round_trip(tmp_path, trained):
     c2p_model.save(trained[0], path)
     assert os.path.getsize(path) == 4 * c2p_model.N_PARAM
     assert np.array_equal(c2p_model.load(path), trained[0].astype(np.float32))
+
+
+def test_driver_reads_c2p_keys(tmp_path, trained):
+    """The drop-in driver with in.par C2P_Guess 1, C2P_Stop 1 and C2P_Model
+    runs from the model file and agrees with the plain driver to 1e-13; a
+    missing model file is refused at set-up."""
+    cfg = configs.get("gr_disk")
+    exe = cpu_build.build_driver("gr_disk")
+    model = str(tmp_path / "m.f32")
+    c2p_model.save(trained[0], model)
+    outs = []
+    for extra in ({}, {"C2P_Guess": 1, "C2P_Stop": 1, "C2P_Model": model}):
+        d = tmp_path / ("learned" if extra else "plain")
+        state_io.write_par(str(d), cfg.par_dict(**PAR, **extra))
+        subprocess.run([exe, "4"], cwd=d, check=True, capture_output=True, timeout=300)
+        outs.append(state_io.read_state(str(d / "output.dstate")))
+    assert max(field_errors(outs[1], outs[0]).values()) < 1e-13
+    d = tmp_path / "missing"
+    state_io.write_par(str(d), cfg.par_dict(**PAR, C2P_Guess=1, C2P_Model=str(tmp_path / "no")))
+    r = subprocess.run([exe, "1"], cwd=d, capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0 and "C2P_Model" in (r.stdout + r.stderr)
+
+
+def test_shipped_model_guesses_the_disk(host):
+    """The committed model (python -m disco_amd.c2p_model train) is used at
+    every cell of the disk and, with the residual rule, needs at most one
+    iteration more than the previous primitives."""
+    if not os.path.exists(c2p_model.MODEL_PATH):
+        pytest.skip("no shipped model")
+    p = c2p_model.load()
+    _, plain = _run(host, {"C2P_Stop": 1}, steps=2)
+    out, stats = _run(host, {"C2P_Guess": 1, "C2P_Stop": 1}, p, steps=2)
+    for s, r in zip(stats, plain):
+        assert s["learned"] >= 0.999 * s["cells"], s
+        assert s["iterations"] / s["cells"] <= r["iterations"] / r["cells"] + 1.05, (s, r)

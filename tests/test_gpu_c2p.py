@@ -35,7 +35,8 @@ def _initial():
 
 @pytest.fixture(scope="module")
 def model():
-    """A model fitted on the host build to three steps of the disk."""
+    """A model fitted on the GPU (dh_c2p_train) to three steps of the disk
+    run on the host build."""
     host = capi.load_library(cpu_build.build())
     cfg, par, a = _initial()
     g = capi.DiscoHip(cfg, par, a, library=host)
@@ -47,7 +48,7 @@ def model():
         xs.append(f[k])
         ys.append(t[k])
     g.close()
-    return c2p_model.fit(np.concatenate(xs), np.concatenate(ys), epochs=4, batch=256)
+    return c2p_model.fit(np.concatenate(xs), np.concatenate(ys), epochs=4, batch=256)[0]
 
 
 def test_gpu_model_matches_host_build(model):
@@ -74,32 +75,62 @@ def test_gpu_model_matches_host_build(model):
     assert np.median(deta) < 1e-5 and deta.max() < 1e-2
 
 
-def _run(cfg, par, a, over, model=None, steps=4):
-    g = capi.DiscoHip(cfg, {**par, **over}, a)
-    if model is not None:
-        g.set_c2p_model(model)
-    stats = []
-    for _ in range(steps):
-        g.timestep(0.05)
-        stats.append(g.c2p_stats())
-    out = g.download(a)
-    g.close()
-    return out, stats
-
-
 @pytest.mark.parametrize("guess,stop", [(1, 0), (0, 1), (1, 1)])
 def test_gpu_learned_guess_and_residual_rule(model, guess, stop):
+    """Each of four steps, taken from the reference rule's run's state with
+    the learned guess and/or the residual rule, agrees with the reference
+    rule's step to 1e-11.  The two rules' solutions differ by ~1e-15 per
+    cell; the second substep's relativistic HLLD amplifies that to up to
+    1e-12 within a step on this mesh (and trajectories to ~1e-10 within four
+    steps, tests/test_conditioning.py)."""
     cfg, par, a = _initial()
-    print(f"case guess {guess} stop {stop}: {a.n_real()} cells, 4 steps", flush=True)
-    ref, ref_stats = _run(cfg, par, a, {})
-    out, stats = _run(cfg, par, a, {"C2P_Guess": guess, "C2P_Stop": stop},
-                      model if guess else None)
-    worst = max(field_errors(out, ref).values())
-    it = [s["iterations"] / s["cells"] for s in stats]
-    print(f"  worst rel L1 vs the reference rule {worst:.2e}; iterations/cell {it}; "
-          f"reference {[r['iterations'] / r['cells'] for r in ref_stats]}", flush=True)
-    assert worst < 1e-13
-    for s in stats:
+    print(f"case guess {guess} stop {stop}: {a.n_real()} cells, 4 steps from common states",
+          flush=True)
+    g = capi.DiscoHip(cfg, par, a)
+    for n in range(4):
+        s = g.download(a)
+        h = capi.DiscoHip(cfg, {**par, "C2P_Guess": guess, "C2P_Stop": stop}, s)
         if guess:
-            assert s["learned"] > 0.9 * s["cells"], s
-    assert all((x >= 11.0) if stop == 0 else (x < 6.0) for x in it)
+            h.set_c2p_model(model)
+        h.timestep(0.05)
+        out, st = h.download(a), h.c2p_stats()
+        h.close()
+        g.timestep(0.05)
+        ref, rst = g.download(a), g.c2p_stats()
+        worst = max(field_errors(out, ref).values())
+        it, rit = st["iterations"] / st["cells"], rst["iterations"] / rst["cells"]
+        print(f"  step {n}: worst rel L1 {worst:.2e}; iterations/cell {it:.2f} "
+              f"(reference rule, previous primitives {rit:.2f}); learned {st['learned']}, "
+              f"fallbacks {st['fallbacks']}", flush=True)
+        assert worst < 1e-11
+        if guess:
+            assert st["learned"] > 0.9 * st["cells"], st
+        assert (it >= 11.0) if stop == 0 else (it < 6.0)
+        assert rit >= 11.0
+    g.close()
+
+
+def test_gpu_training_gradient_matches_oracle():
+    """dh_c2p_grad on the GPU against the float64 oracle (oracle/c2pnet.py,
+    pinned against finite differences in tests/test_c2p_train.py)."""
+    from oracle import c2pnet
+    from test_c2p_train import _problem
+    p, X, Y = _problem(n=4096)
+    g, loss = c2p_model.grad(p, X, Y)
+    go, lo = c2pnet.grad(p, X, Y)
+    scale = np.abs(go).max()
+    print(f"  loss {loss:.6e} (oracle {lo:.6e}); max |dg| / max |g| "
+          f"{np.abs(g - go).max() / scale:.2e}", flush=True)
+    assert abs(loss - lo) <= 1e-5 * lo
+    assert np.abs(g - go).max() <= 2e-5 * scale
+
+
+def test_gpu_training_reduces_the_error():
+    from oracle import c2pnet
+    from test_c2p_train import _problem
+    p, X, Y = _problem(n=20000)
+    _, l1 = c2p_model.fit(X, Y, epochs=1, batch=256, lr=3e-3, seed=5)
+    p10, l10 = c2p_model.fit(X, Y, epochs=10, batch=256, lr=3e-3, seed=5)
+    print(f"  mse after 1 epoch {l1:.3e}, after 10 {l10:.3e}", flush=True)
+    assert l10 < 0.5 * l1
+    assert np.all(np.isfinite(p10)) and c2pnet.loss(p10, X, Y) < 1.5 * l10
