@@ -51,6 +51,19 @@ WORKLOADS = {
         desc="2-D Euler isentropic vortex, Nr=128 polar mesh (51,809 cells), HLLC, PLM 1.5, "
              "RK2, CFL 0.5, rigid mesh rotation",
         cpu_over={}, cpu_steps=150),
+    "mhd_c5": dict(
+        config="mhd_kep",
+        over={"Num_R": 512, "Num_Z": 160, "R_Min": 0.5, "R_Max": 3.0, "Z_Min": -0.3,
+              "Z_Max": 0.3, "Init_Par0": 1, "Init_Par1": 100.0},
+        steps=5, warmup=1,
+        desc="3-D Newtonian MHD Keplerian disk at the one-GPU memory-filling size: r in [0.5,3], "
+             "z in [-0.3,0.3], Nr=512, Nz=160, Np=int(2 pi r/dr) per annulus, rho=1, "
+             "cs=v_K/20, zero-net-flux toroidal field B_phi ~ sin(2 pi (r-0.5)/2.5) at beta=100, "
+             "velocity noise 5e-5 v_phi; HLLD, PLM 1.5, constrained transport, Keplerian mesh "
+             "motion, RK2, CFL 0.5",
+        cpu_over={"Num_R": 64, "Num_Z": 20, "R_Min": 0.5, "R_Max": 3.0, "Z_Min": -0.3,
+                  "Z_Max": 0.3, "Init_Par0": 1, "Init_Par1": 100.0},
+        cpu_steps=3),
     "gr_disk": dict(
         config="gr_disk",
         over={"Num_R": 128, "Num_Z": 32, "R_Min": 8.0, "R_Max": 20.0, "Riemann_Solver": 2,
@@ -220,6 +233,8 @@ def main(argv=None):
     t_setup = time.perf_counter()
     run = HostRun(w["config"], par_path)
     t_setup = time.perf_counter() - t_setup
+    print(f"bench {args.workload}: set-up of {run.real_cells} cells in {t_setup:.1f} s",
+          file=sys.stderr, flush=True)
     lib = capi.lib()
     ctx = C.c_void_p(run.ctx)
 
@@ -235,6 +250,7 @@ def main(argv=None):
         run.timestep(run.getmindt())
     sync()
     secs = time.perf_counter() - t0
+    print(f"bench {args.workload}: {steps} timed steps in {secs:.2f} s", file=sys.stderr, flush=True)
     ncell = run.real_cells
     value = ncell * steps / secs
     nq, nf = cfg.nq, cfg.nf

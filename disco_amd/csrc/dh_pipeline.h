@@ -305,8 +305,8 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
    DH_TRY(alloc_n(x, &b.cons, qb));
    DH_TRY(alloc_n(x, &b.rkcons, qb));
    DH_TRY(alloc_n(x, &b.gradp, qb));
-   DH_TRY(alloc_n(x, &b.gradr, qb));
-   DH_TRY(alloc_n(x, &b.gradz, qb));
+   DH_TRY(alloc_n(x, &b.gradr, 2 * qb));  // r and z slopes in one block
+   b.gradz = b.gradr + (size_t)nq * ntot;
    DH_TRY(alloc_n(x, &b.dFp, qb));
    DH_TRY(alloc_n(x, &b.piph, cb));
    DH_TRY(alloc_n(x, &b.piph2, cb));
@@ -343,9 +343,15 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
       DH_TRY(alloc_n(x, &b.Phi, 5 * cb));
       DH_TRY(alloc_n(x, &b.rkPhi, 5 * cb));
       DH_TRY(alloc_n(x, &b.emf_phi, 4 * cb));
-      DH_TRY(alloc_n(x, &b.Epre, 16 * cb));
-      DH_TRY(alloc_n(x, &b.Eo, 4 * cb));
-      DH_TRY(alloc_n(x, &b.Eown, 4 * cb));
+      // The edge EMFs of a substep live from edges_collect to edge_segments,
+      // after the last reader of the slopes (the fluxes; the update reads
+      // them only for viscosity, which the MHD modules do not have) and
+      // before plm_phi and the gradients of the next substep rewrite them:
+      // Epre (16 per cell) lives in the r and z slopes, Eo and Eown (4 + 4)
+      // in the phi slopes.
+      b.Epre = b.gradr;
+      b.Eo = b.gradp;
+      b.Eown = b.gradp + 4 * (size_t)ntot;
       DH_TRY(alloc_n(x, &b.Ecarry, 16 * cb));
       DH_TRY(x.zero(b.Phi, 5 * cb));
       DH_TRY(x.zero(b.emf_phi, 4 * cb));

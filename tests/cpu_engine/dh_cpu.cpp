@@ -120,9 +120,13 @@ struct HostExec {
          }
       return p ? fail(DH_EINVAL, "release of an unknown pointer") : DH_OK;
    }
+   // With DH_CPU_POISON set, new memory is filled with 0xff bytes (NaN
+   // doubles, -1 ints) instead of zeros, as device memory holds whatever was
+   // there: a kernel that reads an element nothing wrote then shows it.
    int alloc(void **p, size_t bytes)
    {
       *p = calloc(bytes ? bytes : 8, 1);
+      if (*p && getenv("DH_CPU_POISON")) memset(*p, 0xff, bytes ? bytes : 8);
       if (!*p) return fail(DH_ENOMEM, "out of host memory (%zu bytes)", bytes);
       allocs.push_back(*p);
       return DH_OK;

@@ -158,3 +158,43 @@ def test_diagnostics_accumulate_average_zero(cpu_lib):
     g.zero_diagnostics()
     assert not g.diagnostics().any()
     g.close()
+
+
+@pytest.mark.parametrize("cname,over", [
+    ("mhd_kep", {"Num_R": 16, "Num_Z": 6}),
+    ("gr_disk", {"Num_R": 24, "Num_Z": 6, "R_Min": 8.0, "R_Max": 20.0, "Riemann_Solver": 2,
+                 "CFL": 0.2}),
+    ("euler_yee", {"Num_R": 24}),
+])
+def test_no_kernel_reads_memory_nothing_wrote(cname, over):
+    """Device memory starts with whatever was there: the host build filling
+    every allocation with 0xff bytes (NaN doubles, -1 ints) instead of zeros
+    gives the same steps bit for bit, so no kernel reads an element nothing
+    wrote (the edge-EMF arrays share memory with the slopes, so this also
+    checks that they are dead when reused)."""
+    import subprocess
+    import tempfile
+    cfg = configs.get(cname)
+    par = cfg.par_dict(**over)
+    exe = configs.ref_binary(cfg)
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built")
+    d = tempfile.mkdtemp(prefix="poison_")
+    state_io.write_par(d, par)
+    subprocess.run([exe, d, "0"], cwd=d, check=True, capture_output=True)
+    a = state_io.read_state(os.path.join(d, "state_0000.bin"))
+    lib = capi.load_library(cpu_build.build())
+    outs = []
+    for poison in (False, True):
+        if poison:
+            os.environ["DH_CPU_POISON"] = "1"
+        try:
+            g = capi.DiscoHip(cfg, par, a, library=lib)
+            for _ in range(3):
+                g.timestep(g.getmindt())
+            outs.append(g.download(a))
+            g.close()
+        finally:
+            os.environ.pop("DH_CPU_POISON", None)
+    for key in ("prim", "cons", "piph", "phi_flux"):
+        assert np.array_equal(getattr(outs[0], key), getattr(outs[1], key), equal_nan=True), key
