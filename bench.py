@@ -64,6 +64,14 @@ WORKLOADS = {
         cpu_over={"Num_R": 64, "Num_Z": 20, "R_Min": 0.5, "R_Max": 3.0, "Z_Min": -0.3,
                   "Z_Max": 0.3, "Init_Par0": 1, "Init_Par1": 100.0},
         cpu_steps=3),
+    "disk_ensemble": dict(
+        config="disk_planet", over={}, steps=200, warmup=2, members=64, seed=314159,
+        desc="64 independent 2-D isothermal disk-planet Euler runs, each its own mesh, dt and "
+             "stream: r in [0.4,2.5], Nr drawn from {64,128,256,512} by seed, "
+             "Np=int(2 pi r/dr), Sigma=r^-0.5, h=0.05, alpha viscosity 1e-3, a planet of mass "
+             "ratio q drawn log-uniform in [1e-5,1e-3] on a circular orbit at r=1 softened by "
+             "0.6h; HLLC, PLM 1.5, RK2, CFL 0.5, Keplerian mesh motion",
+        cpu_over={"Num_R": 128}, cpu_steps=60),
     "gr_disk": dict(
         config="gr_disk",
         over={"Num_R": 128, "Num_Z": 32, "R_Min": 8.0, "R_Max": 20.0, "Riemann_Solver": 2,
@@ -195,6 +203,88 @@ def measured_traffic(kernel: str, workload: str):
     return row.get("hbm_bytes_per_launch") if row else None
 
 
+def ensemble_members(w):
+    """(Num_R, Mass_Ratio) of every member: Nr uniform in {64, 128, 256, 512}
+    and q log-uniform in [1e-5, 1e-3], from the workload's seed."""
+    import numpy as np
+    rng = np.random.default_rng(w["seed"])
+    return [(int(rng.choice([64, 128, 256, 512])), float(10.0 ** rng.uniform(-5.0, -3.0)))
+            for _ in range(w["members"])]
+
+
+def run_ensemble(args, w, cfg):
+    """Independent runs stepped together: each step takes every member's
+    CFL time step (a device round trip per member), then launches every
+    member's RK2 step on its own stream, so that the members' kernels
+    overlap on the GPU.  value = all members' zone-updates per second."""
+    import ctypes as C
+    from disco_amd import capi
+    from disco_amd.host_lib import HostRun
+    steps = args.steps if args.steps is not None else w["steps"]
+    warmup = args.warmup if args.warmup is not None else w["warmup"]
+    members = ensemble_members(w)
+    tmp = tempfile.mkdtemp(prefix="bench_ens_")
+    t0 = time.perf_counter()
+    runs = []
+    for i, (nr, q) in enumerate(members):
+        par = cfg.par_dict(Num_R=nr, Mass_Ratio=q)
+        runs.append(HostRun(w["config"], state_io.write_par(os.path.join(tmp, str(i)), par)))
+    t_setup = time.perf_counter() - t0
+    ncell = sum(r.real_cells for r in runs)
+    print(f"bench {args.workload}: set-up of {len(runs)} members, {ncell} cells in "
+          f"{t_setup:.1f} s", file=sys.stderr, flush=True)
+    lib = capi.lib()
+
+    def step_all():
+        dts = [r.getmindt() for r in runs]
+        for r, dt in zip(runs, dts):
+            r.timestep(dt)
+
+    def sync_all():
+        for r in runs:
+            if lib.dh_sync(C.c_void_p(r.ctx)):
+                raise RuntimeError(lib.dh_last_error().decode())
+
+    for _ in range(warmup):
+        step_all()
+    sync_all()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step_all()
+    sync_all()
+    secs = time.perf_counter() - t0
+    print(f"bench {args.workload}: {steps} timed steps in {secs:.2f} s", file=sys.stderr, flush=True)
+    for r in runs:
+        r.close()
+    value = ncell * steps / secs
+    counts = {n: sum(1 for m in members if m[0] == n) for n in (64, 128, 256, 512)}
+    out = {
+        "metric": "zone-updates/s on 1 MI355X (%HBM roofline)",
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": secs / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (initial conditions from the workload's Initial plugin)",
+        "config": {"workload": w["desc"], "name": args.workload, "members": len(members),
+                   "members_by_Nr": counts, "real_cells": ncell, "setup_s": t_setup},
+        "roofline": None,
+        "step_roofline": {"b_alg_bytes_per_zone_update": b_alg(cfg.nq, cfg.nf),
+                          "achieved": b_alg(cfg.nq, cfg.nf) * value / 1e9,
+                          "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                          "frac": b_alg(cfg.nq, cfg.nf) * value / 1e9 / PEAK_HBM_GBS},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, w["cpu_over"], w["cpu_steps"])
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +308,9 @@ def main(argv=None):
     from disco_amd.host_lib import HostRun
 
     w = WORKLOADS[args.workload]
+    if "members" in w:
+        print(json.dumps(run_ensemble(args, w, configs.get(w["config"]))), flush=True)
+        return 0
     steps = args.steps if args.steps is not None else w["steps"]
     warmup = args.warmup if args.warmup is not None else w["warmup"]
     cfg = configs.get(w["config"])

@@ -62,6 +62,8 @@ HOST_CONFIGS = {
     "gr_disk": dict(hydro="grmhd2", initial="gr_disk", planets="none",
                     boundary="fixed_r_outflow_z", analysis="diag_euler",
                     num_c=8, num_n=0, ct_mode=2),
+    "disk_planet": dict(hydro="euler", initial="planet_disk", planets="planet_circ",
+                        boundary="fixed", analysis="diag_euler", num_c=5, num_n=0, ct_mode=0),
     "gr_disk3d": dict(hydro="grmhd2", metric="schw_ks_rpz", initial="gr_disk", planets="none",
                       boundary="fixed_r_outflow_z", analysis="diag_euler",
                       num_c=8, num_n=0, ct_mode=2),

@@ -134,6 +134,17 @@ CONFIGS["gr_disk"] = _cfg(
 CONFIGS["gr_disk3d"] = dataclasses.replace(CONFIGS["gr_disk"], name="gr_disk3d",
                                            metric="schw_ks_rpz")
 
+# One member of the disk-planet ensemble: a 2-D isothermal disk (Sigma =
+# r^-0.5, h = 0.05) with alpha viscosity 1e-3 and a planet of mass ratio
+# Mass_Ratio on a circular orbit at r = 1 softened by 0.6 h; HLLC, PLM 1.5,
+# Keplerian mesh motion, held radial boundaries.
+CONFIGS["disk_planet"] = _cfg(
+    "disk_planet", "euler", "planet_disk", "fixed", "planet_circ", 5, 0, 0,
+    Num_R=128, Num_Z=1, R_Min=0.4, R_Max=2.5, NoBC_Rmin=0, NoBC_Rmax=0, CFL=0.5, PLM=1.5,
+    Riemann_Solver=1, Mesh_Motion=4, Exact_Mesh_Omega=2, Energy_Omega=0, Isothermal=1,
+    Cs2_Profile=4, Mach_Number=20.0, Metric_Par2=1.0, Grav2D=1, Adiabatic_Index=5.0 / 3.0,
+    Use_Viscosity=1, Use_As_Alpha=1, Viscosity=1e-3, Mass_Ratio=1e-3)
+
 # The disk with a companion of mass ratio 1e-3 at r = 1.25 (Planets/central.c
 # reads Mass_Ratio) and the Euler diagnostics, which read planet 1.
 CONFIGS["mhd_kep_p2"] = dataclasses.replace(

@@ -152,7 +152,9 @@ inline int validate(const dh_grid *g, const dh_params &P, long *ntot_out)
       if (g->Np[a] < 1) return fail(DH_EINVAL, "annulus %d has %d cells", a, g->Np[a]);
       ntot += g->Np[a];
    }
-   if (ntot >= (1L << 31) / 16) return fail(DH_EINVAL, "mesh too large (%ld cells)", ntot);
+   // cells, faces (about two per cell and direction) and edge-line
+   // segments (about two per face) are indexed in int32
+   if (ntot >= (1L << 29)) return fail(DH_EINVAL, "mesh too large (%ld cells)", ntot);
    *ntot_out = ntot;
    return DH_OK;
 }
@@ -252,6 +254,7 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
             fI[j + Nrmax * k] = (int)n;
             n += g->Np[jk] + g->Np[jkp];
          }
+      if (n >= (1L << 31)) return fail(DH_EINVAL, "too many faces (%ld)", n);
       fI[Nrmax * Nzmax] = (int)n;
       F.npairs = Nrmax * Nzmax;
       F.n = (int)n;
@@ -291,6 +294,7 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
             EdgeRole er = edge_role(m, dim, pr.j, pr.k, 1);
             if (er.ok) nseg += (fI[P + 1] - fI[P]) + (fI[er.oJK + 1] - fI[er.oJK]);
          }
+         if (nseg >= (1L << 31)) return fail(DH_EINVAL, "too many edge segments (%ld)", nseg);
          lb[F.npairs] = (int)nseg;
          DH_TRY(alloc_n(x, &b.lbase[d], sizeof(int) * lb.size()));
          DH_TRY(x.put(b.lbase[d], lb.data(), sizeof(int) * lb.size()));

@@ -27,6 +27,10 @@ import build as cpu_build  # noqa: E402
     ("mhd_kep", {"Num_R": 8, "Num_Z": 4}, 5),
     ("mhd_kep", {"Num_R": 12, "Num_Z": 6, "Max_DT": 0.02}, 6),
     ("mhd_kep_p2", {"Num_R": 12, "Num_Z": 4, "Max_DT": 0.02}, 4),
+    # a disk-planet ensemble member: isothermal Euler with alpha viscosity
+    # (viscous fluxes and sources) and a planet on a circular orbit
+    ("disk_planet", {"Num_R": 32}, 6),
+    ("disk_planet", {"Num_R": 64, "Mass_Ratio": 1e-5, "Riemann_Solver": 0}, 4),
     # GRMHD on the equator of a Schwarzschild black hole: HLLC, HLL, and mesh
     # motion from the fluid's angular velocity (get_omega)
     ("gr_disk", {"Num_R": 24, "Num_Z": 6}, 8),

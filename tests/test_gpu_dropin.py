@@ -28,6 +28,8 @@ HOST = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
     ("euler_yee_n1", {"Num_R": 64, "Riemann_Solver": 0}, 10),
     ("mhd_kep", {"Num_R": 32, "Num_Z": 8}, 6),
     ("mhd_kep_p2", {"Num_R": 32, "Num_Z": 8, "Max_DT": 0.01}, 6),
+    # a disk-planet ensemble member (alpha viscosity, a planet on its orbit)
+    ("disk_planet", {"Num_R": 128}, 20),
     # GRMHD trajectories with HLL and HLLD over the steps a one-ulp libm change
     # moves by less than 1e-11, and GR HLLC on the 2-D disk falling in on a
     # static mesh, whose contacts stay away from the faces' speeds
