@@ -150,6 +150,45 @@ def cmd_copy_probe(args):
     print("copy probe: 1 GiB device-to-device")
 
 
+FP64_COUNTERS = ("SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_FMA_F64",
+                 "SQ_INSTS_VALU_TRANS_F64")
+PEAK_FP64_TFLOPS = 78.6  # MI355X vector FP64
+
+
+def cmd_fp64(args):
+    """FP64 rate of every kernel: wavefront FP64 instructions per dispatch
+    (one --pmc run with FP64_COUNTERS) times 64 lanes, an FMA counting two
+    operations, over the kernel's mean duration in a --kernel-trace run of
+    the same command."""
+    stats = kernel_stats(args.trace)
+    per = {c: counter_means(args.pmc, c) for c in FP64_COUNTERS}
+    rows = []
+    for name, (calls, tot, mean_ns) in stats.items():
+        if name not in per[FP64_COUNTERS[0]]:
+            continue
+        add, mul, fma, trans = (per[c].get(name, (0, 0.0))[1] for c in FP64_COUNTERS)
+        flops = 64.0 * (add + mul + trans + 2.0 * fma)
+        tf = flops / mean_ns / 1e3
+        rows.append((name, mean_ns / 1e6, flops, tf, tf / PEAK_FP64_TFLOPS))
+    rows.sort(key=lambda r: -r[1])
+    lines = [f"# FP64 rate: {args.title}", "",
+             f"FP64 operations per dispatch = 64 lanes x (ADD + MUL + TRANS + 2 FMA) wavefront "
+             f"instructions (rocprofv3 counters {', '.join(FP64_COUNTERS)}); rate over the mean "
+             f"duration of the same kernel in a kernel-trace run; peak {PEAK_FP64_TFLOPS} TFLOP/s "
+             f"(MI355X vector FP64).", ""]
+    if args.cmd:
+        lines += ["Commands:", "", "```", args.cmd, "```", ""]
+    lines += ["| kernel | timer name | mean (ms) | FP64 ops per dispatch | TFLOP/s | of peak |",
+              "|---|---|---|---|---|---|"]
+    for name, ms, flops, tf, frac in rows:
+        short = name if len(name) < 90 else name[:87] + "..."
+        lines.append(f"| `{short}` | {timer_name(name)} | {ms:.4f} | {flops:.3e} | {tf:.2f} | "
+                     f"{100 * frac:.1f}% |")
+    with open(args.out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print(f"wrote {args.out}")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="what", required=True)
@@ -163,8 +202,14 @@ def main(argv=None):
         t.add_argument(a, required=True)
     t.add_argument("--cmd", default="")
     sub.add_parser("copy-probe")
+    f = sub.add_parser("fp64")
+    for a in ("--pmc", "--trace", "--out"):
+        f.add_argument(a, required=True)
+    f.add_argument("--title", default="")
+    f.add_argument("--cmd", default="")
     args = ap.parse_args(argv)
-    {"kernels": cmd_kernels, "traffic": cmd_traffic, "copy-probe": cmd_copy_probe}[args.what](args)
+    {"kernels": cmd_kernels, "traffic": cmd_traffic, "copy-probe": cmd_copy_probe,
+     "fp64": cmd_fp64}[args.what](args)
     return 0
 
 
