@@ -167,6 +167,8 @@ def kernel_bytes(name: str, nq: int, mhd: bool, nt: int, nr: int, nfr: int, nfz:
         return 2 * zlayer * (4 * d + i)
     if name == "bc_fill":               # one z ghost layer from the layer inside it
         return zlayer * (d * (2 * nq + 4) + i)
+    if name == "copy":                  # cons -> rkcons (and Phi -> rkPhi): mean launch
+        return 2 * d * nt * (nq + 5) / 2 if mhd else 2 * d * nt * nq
     return 0.0
 
 
@@ -234,6 +236,9 @@ def run_ensemble(args, w, cfg):
     print(f"bench {args.workload}: set-up of {len(runs)} members, {ncell} cells in "
           f"{t_setup:.1f} s", file=sys.stderr, flush=True)
     lib = capi.lib()
+    for r in runs:
+        if lib.dh_enable_graphs(C.c_void_p(r.ctx), 1 if args.graphs == "on" else 0):
+            raise RuntimeError(lib.dh_last_error().decode())
 
     def step_all():
         dts = [r.getmindt() for r in runs]
@@ -271,7 +276,7 @@ def run_ensemble(args, w, cfg):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (initial conditions from the workload's Initial plugin)",
-        "config": {"workload": w["desc"], "name": args.workload, "members": len(members),
+        "config": {"workload": w["desc"], "name": args.workload, "graphs": args.graphs == "on", "members": len(members),
                    "members_by_Nr": counts, "real_cells": ncell, "setup_s": t_setup},
         "roofline": None,
         "step_roofline": {"b_alg_bytes_per_zone_update": b_alg(cfg.nq, cfg.nf),
@@ -297,6 +302,8 @@ def main(argv=None):
                     help="grmhd2 cons2prim's initial guess (learned: the committed model)")
     ap.add_argument("--c2p-stop", choices=["reference", "residual"], default="reference",
                     help="grmhd2 cons2prim's stopping rule")
+    ap.add_argument("--graphs", choices=["on", "off"], default="on",
+                    help="record the RK2 timestep once as a HIP graph and replay it")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(json.dumps({"error": "disco_amd runs one mesh on one GPU; sharding the annulus "
@@ -330,6 +337,8 @@ def main(argv=None):
           file=sys.stderr, flush=True)
     lib = capi.lib()
     ctx = C.c_void_p(run.ctx)
+    if lib.dh_enable_graphs(ctx, 1 if args.graphs == "on" else 0):
+        raise RuntimeError(lib.dh_last_error().decode())
 
     def sync():
         if lib.dh_sync(ctx):
@@ -417,7 +426,7 @@ def main(argv=None):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (initial conditions from the workload's Initial plugin, seeded noise)",
-        "config": {"workload": w["desc"], "name": args.workload, "real_cells": ncell,
+        "config": {"workload": w["desc"], "name": args.workload, "graphs": args.graphs == "on", "real_cells": ncell,
                    "cells_with_ghosts": ntot, "r_faces": nfr, "z_faces": nfz,
                    "setup_s": t_setup},
         "roofline": roofline,

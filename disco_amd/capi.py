@@ -133,6 +133,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "dh_timestep": ([vp, C.c_double], C.c_int),
         "dh_sync": ([vp], C.c_int),
         "dh_enable_timers": ([vp, C.c_int], C.c_int),
+        "dh_enable_graphs": ([vp, C.c_int], C.c_int),
         "dh_read_timers": ([vp, C.c_int, C.c_void_p, _dp, C.POINTER(C.c_long), _ip], C.c_int),
         "dh_face_count": ([vp, C.c_int, C.POINTER(C.c_long)], C.c_int),
         "dh_get_faces": ([vp, C.c_int, _ip, _ip, _ip, _dp, _dp], C.c_int),
@@ -160,7 +161,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
 
 EXPORTED_SYMBOLS = ["dh_create", "dh_destroy", "dh_sizes", "dh_upload_state",
                     "dh_download_state", "dh_init_cells", "dh_set_planets", "dh_getmindt", "dh_timestep",
-                    "dh_face_count", "dh_get_faces", "dh_sync", "dh_enable_timers",
+                    "dh_face_count", "dh_get_faces", "dh_sync", "dh_enable_timers", "dh_enable_graphs",
                     "dh_read_timers", "dh_add_diagnostics", "dh_get_diagnostics",
                     "dh_zero_diagnostics", "dh_avg_diagnostics", "dh_set_c2p_model",
                     "dh_c2p_harvest", "dh_c2p_predict", "dh_c2p_stats", "dh_c2p_train",
@@ -297,6 +298,10 @@ class DiscoHip:
 
     def enable_timers(self, on: bool = True):
         self._check(self.lib.dh_enable_timers(self.ctx, 1 if on else 0))
+
+    def enable_graphs(self, on: bool = True):
+        """Record an RK2 timestep once as a HIP graph and replay it."""
+        self._check(self.lib.dh_enable_graphs(self.ctx, 1 if on else 0))
 
     def read_timers(self):
         cap = 64

@@ -54,6 +54,23 @@ struct XcdContiguous<K, decltype((void)K::xcd_contiguous)> {
    static constexpr bool value = K::xcd_contiguous;
 };
 
+// The timestep's {dt, planets}, passed by value and stored by one thread,
+// so that a recorded timestep that follows reads this step's values.
+__global__ void k_set_step(StepArgs *dst, StepArgs sa) { *dst = sa; }
+
+// Whole-buffer copy and clear in 8-byte words (kernels rather than the
+// runtime's copy and memset, so that a recorded timestep holds kernel nodes
+// only).
+struct KCopyWords {
+   unsigned long long *dst;
+   const unsigned long long *src;
+   DHD void operator()(long i) const { dst[i] = src[i]; }
+};
+struct KZeroWords {
+   unsigned long long *p;
+   DHD void operator()(long i) const { p[i] = 0ull; }
+};
+
 // Start cell of every annulus (faces_alt.c:59-78): one wavefront per
 // annulus scans 64 cells at a time with ballots for the first cell in the
 // first quadrant that follows a cell in the fourth.
@@ -383,6 +400,11 @@ struct TimerRow {
 struct HipExec {
    hipStream_t stream = nullptr;
    bool timers = false;
+   // Recorded timestep (dh_enable_graphs): replayed while the key it was
+   // recorded under (the current face positions' buffer) is unchanged.
+   bool graphs = false;
+   hipGraphExec_t gexec = nullptr;
+   const void *gkey = nullptr;
    std::vector<TimerRow> rows;
    std::vector<void *> allocs;
    unsigned long long *d_min = nullptr;
@@ -547,14 +569,67 @@ struct HipExec {
       HIPCHK(hipStreamSynchronize(stream));
       return DH_OK;
    }
+   static bool words(const void *p, size_t bytes)
+   {
+      return bytes % 8 == 0 && (uintptr_t)p % 8 == 0;
+   }
    int copy(void *dst, const void *src, size_t bytes)
    {
+      if (words(dst, bytes) && words(src, bytes))
+         return run("copy", KCopyWords{(unsigned long long *)dst, (const unsigned long long *)src},
+                    (long)(bytes / 8));
       HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream));
       return DH_OK;
    }
    int zero(void *p, size_t bytes)
    {
+      if (words(p, bytes)) return run("zero", KZeroWords{(unsigned long long *)p}, (long)(bytes / 8));
       HIPCHK(hipMemsetAsync(p, 0, bytes, stream));
+      return DH_OK;
+   }
+   int set_step(StepArgs *dst, const StepArgs &sa)
+   {
+      {
+         Bracket b(*this, "set_step");
+         k_set_step<<<1, 1, 0, stream>>>(dst, sa);
+      }
+      LAUNCH_CHECK();
+      return DH_OK;
+   }
+   void drop_graph()
+   {
+      if (gexec) (void)hipGraphExecDestroy(gexec);
+      gexec = nullptr;
+      gkey = nullptr;
+   }
+   // Runs body, the launches of one timestep, eagerly; or, with graphs on
+   // and a non-null key, records it once on the stream and replays the
+   // recording from then on.  body must launch the same kernels with the
+   // same arguments whenever key is the same.
+   template <class F>
+   int replayable(const void *key, F &&body)
+   {
+      if (!graphs || timers || !key) return body();
+      if (!gexec || gkey != key) {
+         drop_graph();
+         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+         int rc = body();
+         hipGraph_t g = nullptr;
+         hipError_t e = hipStreamEndCapture(stream, &g);
+         if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (rc) return rc;
+            return fail(DH_EHIP, "timestep capture failed: %s", hipGetErrorString(e));
+         }
+         e = hipGraphInstantiateWithFlags(&gexec, g, 0);
+         (void)hipGraphDestroy(g);
+         if (e != hipSuccess) {
+            gexec = nullptr;
+            return fail(DH_EHIP, "timestep graph instantiation failed: %s", hipGetErrorString(e));
+         }
+         gkey = key;
+      }
+      HIPCHK(hipGraphLaunch(gexec, stream));
       return DH_OK;
    }
    int sync()
@@ -573,6 +648,7 @@ static void release(dh_ctx *c)
 {
    if (!c) return;
    if (c->x.stream) (void)hipStreamSynchronize(c->x.stream);
+   c->x.drop_graph();
    for (auto &row : c->x.rows)
       for (auto &pe : row.pending) {
          (void)hipEventDestroy(pe.first);
@@ -822,6 +898,17 @@ int dh_enable_timers(dh_ctx *c, int on)
 {
    if (!c) return fail(DH_EINVAL, "null context");
    c->x.timers = on != 0;
+   return DH_OK;
+}
+
+int dh_enable_graphs(dh_ctx *c, int on)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   c->x.graphs = on != 0;
+   if (!c->x.graphs) {
+      HIPCHK(hipStreamSynchronize(c->x.stream));
+      c->x.drop_graph();
+   }
    return DH_OK;
 }
 

@@ -28,6 +28,14 @@ struct PlanetSet {
    int n;
 };
 
+// What changes from one timestep to the next, kept in device memory so that
+// a captured timestep replays with the new values: each substep's dt and
+// planets.  The kernels take pointers into it.
+struct StepArgs {
+   double dt[2];
+   PlanetSet pl[2];
+};
+
 // Planet 1 as the Diagnostics modules read it (theDomain->thePlanets + 1,
 // used when Npl > 1).
 struct DiagPlanet {
@@ -552,7 +560,7 @@ template <int NQ, class H>
 struct KFluxPhi {
    Mesh m;
    dh_params P;
-   double dt;
+   const double *dtp;
    const double *prim, *gradp, *gradr, *gradz, *piph, *dphi, *wiph;
    double *dFp, *emf_phi;
    DHD void operator()(long c) const
@@ -567,7 +575,7 @@ struct KFluxPhi {
       double rp = m.r_jph[j], rm = m.r_jph[j - 1];
       double zp = m.z_kph[k], zm = m.z_kph[k - 1];
       double r = centroid_r(rp, rm);
-      double dAdt = (rp - rm) * (zp - zm) * dt;
+      double dAdt = (rp - rm) * (zp - zm) * *dtp;
       double dL = dphi[c], dR = dphi[cR];
       double pL[NQ], pR[NQ];
 #pragma unroll
@@ -614,7 +622,7 @@ struct KFluxTrans {
    Mesh m;
    dh_params P;
    static constexpr int dim = DIM;
-   double dt;
+   const double *dtp;
    Faces F;
    const double *prim, *gradp, *gradr, *gradz, *piph, *dphi;
    double *dF, *emf_face;
@@ -670,7 +678,7 @@ struct KFluxTrans {
          }
          add_visc<NQ, H>(P, pL, pR, gr, gp, gz, r, n, Flux);
       }
-      double dAdt = F.dA[f] * dt;
+      double dAdt = F.dA[f] * *dtp;
 #pragma unroll
       for (int q = 0; q < NQ; ++q) dF[q * nfs + f] = (Flux[q] - 0.0 * Ustr[q]) * dAdt;
       if constexpr (mhd) {
@@ -784,9 +792,10 @@ template <int NQ, class H>
 struct KUpdate {
    Mesh m;
    dh_params P;
-   double RK, dt;
+   double RK;
+   const double *dtp;
    int first_step;
-   PlanetSet pls;
+   const PlanetSet *pls;
    const double *prim_in;
    double *prim, *cons;
    const double *rkcons, *gradr, *gradp, *gradz, *piph, *dphi, *wiph;
@@ -804,6 +813,7 @@ struct KUpdate {
       int j = a % m.Nr, k = a / m.Nr;
       long cm = i == 0 ? c0 + np - 1 : c - 1;
       double Pmax = m.Pmax;
+      const double dt = *dtp;
 
       // Mesh motion of this cell and of its phi neighbour: move_cells on the
       // first substep only, clean_pi and calc_dp on every substep.
@@ -853,8 +863,8 @@ struct KUpdate {
       double pr[NQ];
       load_q<NQ>(prim_in, nt, c, pr);
       H::source(P, pr, cs, rp, rm, phip, phim, zp, zm, dVdt, polar, Pmax);
-      for (int p = 0; p < pls.n; ++p)
-         planet_src(P, pls.v + p * DH_PLANET_STRIDE, pr, cs, rp, rm, phip, phim, zp, zm, dVdt,
+      for (int p = 0; p < pls->n; ++p)
+         planet_src(P, pls->v + p * DH_PLANET_STRIDE, pr, cs, rp, rm, phip, phim, zp, zm, dVdt,
                     P.nobc_rmin == 1, Pmax);
       if (P.visc_flag) {
          double gr[NQ], gp[NQ], gz[NQ];
@@ -1119,7 +1129,8 @@ DHD double owner_edge(const double *Eo, size_t nt, int dim, int LR, int L, int R
 // faces, in the reference's order.
 struct KUpdateB {
    Mesh m;
-   double RK, dt;
+   double RK;
+   const double *dtp;
    Faces Fr, Fz;
    const int *rngr, *rngz;
    const double *Eo;
@@ -1128,6 +1139,7 @@ struct KUpdateB {
    DHD void operator()(long c) const
    {
       size_t nt = (size_t)m.ntot;
+      const double dt = *dtp;
       int a = m.cjk[c];
       int j = a % m.Nr, k = a / m.Nr;
       double ph[5];
@@ -1254,7 +1266,7 @@ struct KLineSteps {
 // steps at seg[lbase[JK] + step], JK the first list's pair.
 struct KEdgeSegments {
    Mesh m;
-   double dt;
+   const double *dtp;
    Faces Fr, Fz;
    const int *estep_r, *estep_z, *lbase_r, *lbase_z;
    const double *piph, *dphi, *Eown;
@@ -1262,6 +1274,7 @@ struct KEdgeSegments {
    DHD void operator()(long t) const
    {
       size_t nt = (size_t)m.ntot;
+      const double dt = *dtp;
       int dim = t < Fr.n ? 1 : 2;
       const Faces &F = dim == 1 ? Fr : Fz;
       const int *estep = dim == 1 ? estep_r : estep_z;
@@ -1360,12 +1373,12 @@ struct KFlipFlags {
    static constexpr int dim = DIM;
    Faces F;
    const double *piph, *wiph;
-   double dt;
+   const double *dtp;
    signed char *flag;
    DHD double moved_up(int f) const
    {
       int o = F.LR[f] == 0 ? F.L[f] : F.R[f];
-      return piph[o] + wiph[o] * dt;
+      return piph[o] + wiph[o] * *dtp;
    }
    DHD void operator()(long f) const
    {

@@ -74,6 +74,7 @@ struct Buffers {
    int *zord_c = nullptr;
    int bc_maxnp[4][8] = {{0}};      // largest Np of each side's ghost layers
    PlanetSet pl[2];
+   StepArgs *step = nullptr;  // device copy of {dt, planets} of the current timestep
    // diagnostics (analysis.c:52-106): Qrz[nann * DH_DIAG_MAXQ], allocated on
    // first use; the kind that accumulated it
    double *diagQ = nullptr;
@@ -371,6 +372,7 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
       }
    }
    b.pl[0].n = b.pl[1].n = 0;
+   DH_TRY(alloc_n(x, &b.step, sizeof(StepArgs)));
    return DH_OK;
 }
 
@@ -441,10 +443,12 @@ struct Engine {
       return DH_OK;
    }
 
-   // One substep (onestep.c:84-228).
+   // One substep (onestep.c:84-228); its dt and planets are b.step's
+   // entries sub.
    template <int NQ, class H>
-   int substep(double RK, double dt, int first_step, int sub)
+   int substep(double RK, int first_step, int sub)
    {
+      const double *dt = &b.step->dt[sub];
       constexpr bool mhd = IsMHD<H>::value;
       const Mesh &m = b.m;
       const dh_params &P = b.P;
@@ -500,7 +504,7 @@ struct Engine {
             DH_TRY(x.run("edge_adjust", ke, (long)b.F[0].n + b.F[1].n));
          }
       }
-      KUpdate<NQ, H> ku{m, P, RK, dt, first_step, b.pl[sub], b.prim, b.prim, b.cons,
+      KUpdate<NQ, H> ku{m, P, RK, dt, first_step, &b.step->pl[sub], b.prim, b.prim, b.cons,
                         b.rkcons, b.gradr, b.gradp, b.gradz, b.piph, b.dphi, b.wiph,
                         b.piph2, b.dphi2, b.dFp, b.dF[0], b.dF[1], b.F[0].fI, b.F[1].fI,
                         b.rng[0], b.rng[1], b.F[0].n, b.F[1].n};
@@ -538,16 +542,31 @@ struct Engine {
    }
 
    // timestep (timestep.c:7-45): RK2 unless Timestep = 1 (forward Euler).
+   // The step's dt and planets go to the device first; the rest launches
+   // the same kernels with the same arguments every step, so the executor
+   // may replay it as a recorded graph.  An RK2 step swaps the face
+   // positions twice and so leaves every buffer pointer where it was; a
+   // forward-Euler step does not (then the step is never recorded).
    template <int NQ, class H>
    int timestep(double dt)
    {
       if (b.P.c2p_guess == DH_C2P_GUESS_LEARNED && !b.c2p_model)
          return fail(DH_EINVAL, "the learned cons2prim guess has no model (dh_set_c2p_model)");
-      DH_TRY(x.copy(b.rkcons, b.cons, sizeof(double) * NQ * b.m.ntot));
-      if (b.mhd) DH_TRY(x.copy(b.rkPhi, b.Phi, sizeof(double) * 5 * b.m.ntot));
-      DH_TRY((substep<NQ, H>(0.0, dt, 1, 0)));
-      if (b.P.timestep_mode != 1) DH_TRY((substep<NQ, H>(0.5, 0.5 * dt, 0, 1)));
-      return DH_OK;
+      StepArgs sa;
+      memset(&sa, 0, sizeof sa);
+      sa.dt[0] = dt;
+      sa.dt[1] = 0.5 * dt;
+      sa.pl[0] = b.pl[0];
+      sa.pl[1] = b.pl[1];
+      DH_TRY(x.set_step(b.step, sa));
+      bool rk2 = b.P.timestep_mode != 1;
+      return x.replayable(rk2 ? (const void *)b.piph : nullptr, [&]() -> int {
+         DH_TRY(x.copy(b.rkcons, b.cons, sizeof(double) * NQ * b.m.ntot));
+         if (b.mhd) DH_TRY(x.copy(b.rkPhi, b.Phi, sizeof(double) * 5 * b.m.ntot));
+         DH_TRY((substep<NQ, H>(0.0, 1, 0)));
+         if (rk2) DH_TRY((substep<NQ, H>(0.5, 0, 1)));
+         return DH_OK;
+      });
    }
 
    // setupCells after the initial prims are in place (domain.c:139-242):

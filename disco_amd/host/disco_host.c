@@ -56,11 +56,18 @@ struct disco_accel {
    /* grmhd2 cons2prim options (in.par C2P_Guess, C2P_Stop, C2P_Model) */
    int c2p_guess, c2p_stop;
    char c2p_model[512];
+   /* in.par Graphs: 0 launches every kernel of a timestep eagerly; otherwise
+    * (the default) the RK2 step is recorded once and replayed */
+   int graphs;
 };
 
 static struct disco_accel *accel_of(struct domain *d)
 {
-   if (!d->accel) d->accel = calloc(1, sizeof(struct disco_accel));
+   if (!d->accel) {
+      struct disco_accel *A = calloc(1, sizeof(struct disco_accel));
+      if (A) A->graphs = 1;
+      d->accel = A;
+   }
    return (struct disco_accel *)d->accel;
 }
 
@@ -248,6 +255,7 @@ int disco_read_par(struct domain *d, const char *file)
    par_lookup(file, "C2P_Guess", V_INT, &A->c2p_guess);
    par_lookup(file, "C2P_Stop", V_INT, &A->c2p_stop);
    par_lookup_str(file, "C2P_Model", A->c2p_model, sizeof A->c2p_model);
+   par_lookup(file, "Graphs", V_INT, &A->graphs);
    if (t2pi) { L->t_min *= 2. * M_PI; L->t_max *= 2. * M_PI; }
    if (p2pi) L->phimax *= 2. * M_PI;
    if (zpi) { L->zmin *= M_PI; L->zmax *= M_PI; }
@@ -627,6 +635,8 @@ int disco_setup(struct domain *d)
    fill_params(d, &P, A->bc);
    if (dh_create(&g, &P, &A->ctx)) return host_fail("dh_create: %s", dh_last_error());
    if (P.c2p_guess == DH_C2P_GUESS_LEARNED && load_c2p_model(A)) return 1;
+   if (dh_enable_graphs(A->ctx, A->graphs != 0))
+      return host_fail("dh_enable_graphs: %s", dh_last_error());
    long nt = A->ntot;
    double *prim = (double *)malloc(sizeof(double) * nt * NUM_Q);
    double *piph = (double *)malloc(sizeof(double) * nt);

@@ -188,7 +188,7 @@ PRESETS: Dict[str, Dict[str, object]] = {
 
 # Keys of this path with no reference counterpart (grmhd2 cons2prim's guess,
 # stopping rule and model file); written only when given.
-EXTRA_KEYS = ("C2P_Guess", "C2P_Stop", "C2P_Model")
+EXTRA_KEYS = ("C2P_Guess", "C2P_Stop", "C2P_Model", "Graphs")
 
 
 def format_par(overrides: Optional[Dict[str, object]] = None) -> str:

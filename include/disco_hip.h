@@ -248,6 +248,15 @@ int dh_enable_timers(dh_ctx *ctx, int on);
 int dh_read_timers(dh_ctx *ctx, int cap, char (*names)[48], double *ms,
                    long *calls, int *nrows);
 
+/* With on != 0, an RK2 `dh_timestep` is recorded once as a HIP graph and
+ * replayed from then on (the step's dt and planets travel in device memory,
+ * so replays take new values); results are bit-identical to the eager
+ * launches.  Off by default; ignored while the timers are on and for
+ * forward-Euler steps.  No counterpart in the reference (timestep.c:7-45
+ * calls onestep twice); it removes the per-kernel launch cost of small
+ * meshes.  The host build accepts and ignores it. */
+int dh_enable_graphs(dh_ctx *ctx, int on);
+
 const char *dh_last_error(void);
 
 #ifdef __cplusplus

@@ -152,6 +152,16 @@ struct HostExec {
       return DH_OK;
    }
    int sync() { return DH_OK; }
+   int set_step(StepArgs *dst, const StepArgs &sa)
+   {
+      *dst = sa;
+      return DH_OK;
+   }
+   template <class F>
+   int replayable(const void *, F &&body)
+   {
+      return body();
+   }
 };
 
 struct dh_ctx {
@@ -422,6 +432,7 @@ int dh_c2p_stats(dh_ctx *c, long long *out)
 }
 
 int dh_enable_timers(dh_ctx *c, int) { return c ? DH_OK : fail(DH_EINVAL, "null context"); }
+int dh_enable_graphs(dh_ctx *c, int) { return c ? DH_OK : fail(DH_EINVAL, "null context"); }
 
 int dh_read_timers(dh_ctx *c, int, char (*)[48], double *, long *, int *nrows)
 {
