@@ -141,6 +141,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "dh_get_diagnostics": ([vp, _dp], C.c_int),
         "dh_zero_diagnostics": ([vp], C.c_int),
         "dh_avg_diagnostics": ([vp, C.c_double], C.c_int),
+        "dh_report": ([vp, _dp, C.c_int, _dp], C.c_int),
         "dh_set_c2p_model": ([vp, C.POINTER(C.c_float), C.c_long], C.c_int),
         "dh_c2p_harvest": ([vp, C.POINTER(C.c_float), C.POINTER(C.c_float)], C.c_int),
         "dh_c2p_predict": ([vp, C.POINTER(C.c_float)], C.c_int),
@@ -163,7 +164,7 @@ EXPORTED_SYMBOLS = ["dh_create", "dh_destroy", "dh_sizes", "dh_upload_state",
                     "dh_download_state", "dh_init_cells", "dh_set_planets", "dh_getmindt", "dh_timestep",
                     "dh_face_count", "dh_get_faces", "dh_sync", "dh_enable_timers", "dh_enable_graphs",
                     "dh_read_timers", "dh_add_diagnostics", "dh_get_diagnostics",
-                    "dh_zero_diagnostics", "dh_avg_diagnostics", "dh_set_c2p_model",
+                    "dh_zero_diagnostics", "dh_avg_diagnostics", "dh_report", "dh_set_c2p_model",
                     "dh_c2p_harvest", "dh_c2p_predict", "dh_c2p_stats", "dh_c2p_train",
                     "dh_c2p_grad", "dh_last_error"]
 
@@ -253,6 +254,18 @@ class DiscoHip:
 
     def zero_diagnostics(self):
         self._check(self.lib.dh_zero_diagnostics(self.ctx))
+
+    REPORT_COLUMNS = ("Torque", "Power", "Fr", "rho_min", "rhoavg_min", "PsiR", "PsiI", "Mass",
+                      "Mdot", "S_R", "L1_rho", "L1_isen", "L1_B", "Br2", "aM", "bM")
+
+    def report(self, planets: Optional[np.ndarray] = None) -> np.ndarray:
+        """report.c's 16 columns after t for the current state."""
+        out = np.zeros(16)
+        pl = None if planets is None else np.ascontiguousarray(planets, np.float64)
+        npl = 0 if pl is None else len(pl)
+        self._check(self.lib.dh_report(self.ctx, _ptr(pl), npl,
+                                       _ptr(out)))
+        return out
 
     def set_c2p_model(self, params: np.ndarray):
         """Loads the learned cons2prim guess's parameters (float32, the layout

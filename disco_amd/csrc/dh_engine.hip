@@ -233,6 +233,38 @@ __global__ __launch_bounds__(TPB) void k_diag(Mesh m, dh_params P, DiagPlanet p1
       for (int q = 0; q < NQD; ++q) Qrz[(size_t)a * NQD + q] += sum[q] * dt / vol;
 }
 
+// report (report.c:7-202): one wavefront per real annulus; each lane sums
+// the terms of every 64th cell, then the wave combines the lanes (the host
+// build sums the cells in order; the two differ by rounding only).
+__global__ __launch_bounds__(TPB) void k_report(Mesh m, dh_params P, DiagPlanet p0, DiagPlanet p1,
+                                                int mhd, const double *prim, const double *piph,
+                                                const double *dphi, double *R)
+{
+   int a = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+   int lane = threadIdx.x & 63;
+   if (a >= m.nann) return;
+   int j = a % m.Nr, k = a / m.Nr;
+   if (!is_real(m, j, k)) return;
+   double sum[RP_N], o[RP_N];
+#pragma unroll
+   for (int q = 0; q < RP_RHOMIN; ++q) sum[q] = 0.0;
+   sum[RP_RHOMIN] = HUGE_VAL;
+   for (int i = lane; i < m.Np[a]; i += 64) {
+      report_cell(m, P, p0, p1, mhd, prim, piph, dphi, m.off[a] + i, j, k, o);
+#pragma unroll
+      for (int q = 0; q < RP_RHOMIN; ++q) sum[q] += o[q];
+      sum[RP_RHOMIN] = fmin(sum[RP_RHOMIN], o[RP_RHOMIN]);
+   }
+   for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+      for (int q = 0; q < RP_RHOMIN; ++q) sum[q] += __shfl_down(sum[q], off, 64);
+      sum[RP_RHOMIN] = fmin(sum[RP_RHOMIN], __shfl_down(sum[RP_RHOMIN], off, 64));
+   }
+   if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < RP_N; ++q) R[(size_t)a * RP_N + q] = sum[q];
+}
+
 // CFL reduction over real cells (misc.c:39-84).  Positive doubles order like
 // their bit patterns, so the minimum is an unsigned 64-bit atomic min.
 template <int NQ, class H>
@@ -496,6 +528,18 @@ struct HipExec {
             k_diag<DH_DIAG_DISK2D><<<nb, TPB, 0, stream>>>(m, P, p1, prim, piph, dphi, dt, Qrz);
          else
             k_diag<DH_DIAG_MHDMRI><<<nb, TPB, 0, stream>>>(m, P, p1, prim, piph, dphi, dt, Qrz);
+      }
+      LAUNCH_CHECK();
+      return DH_OK;
+   }
+
+   int report(const Mesh &m, const dh_params &P, const DiagPlanet &p0, const DiagPlanet &p1,
+              int mhd, const double *prim, const double *piph, const double *dphi, double *R)
+   {
+      unsigned nb = (unsigned)((m.nann + 3) / 4);
+      {
+         Bracket b(*this, "report");
+         k_report<<<nb, TPB, 0, stream>>>(m, P, p0, p1, mhd, prim, piph, dphi, R);
       }
       LAUNCH_CHECK();
       return DH_OK;
@@ -899,6 +943,12 @@ int dh_enable_timers(dh_ctx *c, int on)
    if (!c) return fail(DH_EINVAL, "null context");
    c->x.timers = on != 0;
    return DH_OK;
+}
+
+int dh_report(dh_ctx *c, const double *planets, int npl, double *out)
+{
+   if (!c || !out) return fail(DH_EINVAL, "null argument");
+   return report(c->x, c->b, planets, npl, out);
 }
 
 int dh_enable_graphs(dh_ctx *c, int on)

@@ -1802,10 +1802,12 @@ struct KInitCells {
 // Diagnostics (analysis.c:52-106 with the shipped Diagnostics modules)
 // ---------------------------------------------------------------------------
 
-// planetaryForce, mode 0 (planet.c:99-149): force per unit mass of planet pl
-// (DH_PLANET_STRIDE layout: r, phi, M, omega, vr, eps, type) at (r, phi, z).
+// planetaryForce (planet.c:99-149): force per unit mass of planet pl
+// (DH_PLANET_STRIDE layout: r, phi, M, omega, vr, eps, type) at (r, phi, z),
+// its in-plane part along the cell's (r, phi) directions for mode 0 and along
+// the planet's for mode 1.
 DHD void planetary_force(const dh_params &P, DiagPlanet pl, double r, double phi, double z,
-                         double *fr, double *fp, double *fz)
+                         double *fr, double *fp, double *fz, int mode = 0)
 {
    double rp = pl.v[0], pp = pl.v[1];
    if (P.grav2d == 1) z = 0.0;
@@ -1822,6 +1824,10 @@ DHD void planetary_force(const dh_params &P, DiagPlanet pl, double r, double phi
    double sina = dy / script_r_perp;
    double cosap = cosa * cosp + sina * sinp;
    double sinap = sina * cosp - cosa * sinp;
+   if (mode == 1) {
+      cosap = cosa * cospp + sina * sinpp;
+      sinap = sina * cospp - cosa * sinpp;
+   }
    if (script_r_perp <= 0.0) { cosap = 0.0; sinap = 0.0; }
    double sint = script_r_perp / script_r;
    double cost = z / script_r;
@@ -1917,6 +1923,78 @@ DHD void diag_annulus(const Mesh &m, const dh_params &P, int kind, DiagPlanet p1
       vol += dV;
    }
    for (int q = 0; q < nq; ++q) Qrz[(size_t)a * nq + q] += sum[q] * dt / vol;
+}
+
+// report (report.c:7-202): the global sums of one annulus's real cells.
+// The host adds the annuli in the reference's (j outer, k inner) order and
+// forms the report.dat columns from the totals.
+enum {
+   RP_PSIR, RP_PSII, RP_L1ISEN, RP_L1RHO, RP_L1B, RP_BR2, RP_BRBP, RP_B2, RP_PDV, RP_MDOT,
+   RP_VOL, RP_MASS, RP_SR, RP_S0, RP_TORQUE, RP_POWER, RP_FR,
+   RP_RHOMIN,  // a minimum, not a sum
+   RP_N
+};
+
+// One cell's terms (report.c:76-145; rho0 = 1).  p0, p1: planets 0 and 1,
+// used when p1.on (Npl > 1).
+DHD void report_cell(const Mesh &m, const dh_params &P, const DiagPlanet &p0,
+                     const DiagPlanet &p1, int mhd, const double *prim, const double *piph,
+                     const double *dphi, long c, int j, int k, double *o)
+{
+   size_t nt = (size_t)m.ntot;
+   double phi = piph[c] - .5 * dphi[c];
+   double Pp = prim[PPP * nt + c], rho = prim[RHO * nt + c];
+   double phip = piph[c], phim = phip - dphi[c];
+   double rp = m.r_jph[j], rm = m.r_jph[j - 1];
+   double dV = cell_dV(rp, rm, phip, phim, m.z_kph[k], m.z_kph[k - 1], m.Pmax);
+   double r = centroid_r(rp, rm);
+   o[RP_PSIR] = rho * dV * cos(phi);
+   o[RP_PSII] = rho * dV * sin(phi);
+   o[RP_L1ISEN] = fabs(Pp / pow(rho, P.gamma_law) - 1.) * dV;
+   o[RP_L1RHO] = fabs(rho - 1.) * dV;
+   o[RP_L1B] = o[RP_BR2] = o[RP_BRBP] = o[RP_B2] = 0.0;
+   if (mhd) {
+      double Br = prim[BRR * nt + c], Bp = prim[BPP * nt + c], Bz = prim[BZZ * nt + c];
+      o[RP_L1B] = fabs(Br) * dV;
+      o[RP_BR2] = .5 * Br * Br * dV;
+      o[RP_BRBP] = Br * Bp * dV;
+      o[RP_B2] = .5 * (Br * Br + Bp * Bp + Bz * Bz) * dV;
+   }
+   o[RP_PDV] = Pp * dV;
+   o[RP_MDOT] = 2. * M_PI * r * rho * dV * prim[URR * nt + c];
+   o[RP_VOL] = dV;
+   o[RP_MASS] = rho * dV;
+   double rho4 = pow(rho, 4.);
+   o[RP_SR] = rho4 * r * dV;
+   o[RP_S0] = rho4 * dV;
+   o[RP_TORQUE] = o[RP_POWER] = o[RP_FR] = 0.0;
+   if (p1.on) {
+      double fr, fp, fz, fp2;
+      planetary_force(P, p0, r, phi, 0.0, &fr, &fp2, &fz, 1);
+      planetary_force(P, p1, r, phi, 0.0, &fr, &fp, &fz, 1);
+      double rpl = p1.v[0], om = p1.v[3], vr = p1.v[4];
+      o[RP_TORQUE] = -((rho - 1.0) * rpl * fp * dV);
+      o[RP_POWER] = -((rho - 1.0) * (rpl * om * fp + vr * fr) * dV);
+      o[RP_FR] = -((rho - 1.0) * fr * dV);
+   }
+   o[RP_RHOMIN] = rho;
+}
+
+// One annulus a, its cells in order: R[a * RP_N + q].
+DHD void report_annulus(const Mesh &m, const dh_params &P, const DiagPlanet &p0,
+                        const DiagPlanet &p1, int mhd, const double *prim, const double *piph,
+                        const double *dphi, int a, double *R)
+{
+   int j = a % m.Nr, k = a / m.Nr;
+   double sum[RP_N], o[RP_N];
+   for (int q = 0; q < RP_RHOMIN; ++q) sum[q] = 0.0;
+   sum[RP_RHOMIN] = HUGE_VAL;
+   for (int i = 0; i < m.Np[a]; ++i) {
+      report_cell(m, P, p0, p1, mhd, prim, piph, dphi, m.off[a] + i, j, k, o);
+      for (int q = 0; q < RP_RHOMIN; ++q) sum[q] += o[q];
+      if (sum[RP_RHOMIN] > o[RP_RHOMIN]) sum[RP_RHOMIN] = o[RP_RHOMIN];
+   }
+   for (int q = 0; q < RP_N; ++q) R[(size_t)a * RP_N + q] = sum[q];
 }
 
 // avg_diagnostics (analysis.c:28-49): every accumulated value over t_avg.

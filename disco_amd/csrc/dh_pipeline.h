@@ -79,6 +79,9 @@ struct Buffers {
    // first use; the kind that accumulated it
    double *diagQ = nullptr;
    int diag_kind = 0;
+   // report (report.c:7-202): per-annulus sums R[nann * RP_N], allocated on
+   // first use
+   double *repQ = nullptr;
    // grmhd2 cons2prim: every real cell's solve record (C2P_INFO_*); with the
    // learned guess its model (float parameters and bfloat16 MFMA fragments)
    // and the per-cell guesses (v^2, eta)
@@ -708,6 +711,51 @@ int add_diagnostics(X &x, Buffers &b, int kind, double dt, const double *planets
    if (p1.on)
       for (int q = 0; q < DH_PLANET_STRIDE; ++q) p1.v[q] = planets[DH_PLANET_STRIDE + q];
    return x.diag(b.m, b.P, kind, p1, b.prim, b.piph, b.dphi, dt, b.diagQ);
+}
+
+// report (report.c:7-202) of the current state: out[16] receives report.dat's
+// columns after t: Torque, Power, Fr, rho_min, rhoavg_min, PsiR, PsiI, Mass,
+// Mdot, S_R, L1_rho, L1_isen, L1_B, Br2, aM, bM.  The device sums each real
+// annulus; the annuli are added here in the reference's order (j outer, k
+// inner), so the totals differ from its cell-by-cell sums by rounding only.
+template <class X>
+int report(X &x, Buffers &b, const double *planets, int npl, double *out)
+{
+   if (npl < 0 || npl > DH_MAX_PLANETS) return fail(DH_EINVAL, "npl %d out of range", npl);
+   if (npl > 1 && !planets) return fail(DH_EINVAL, "null planet list");
+   const Mesh &m = b.m;
+   size_t n = (size_t)m.nann * RP_N;
+   if (!b.repQ) DH_TRY(x.alloc((void **)&b.repQ, sizeof(double) * n));
+   DiagPlanet p0{}, p1{};
+   p1.on = npl > 1;
+   if (p1.on)
+      for (int q = 0; q < DH_PLANET_STRIDE; ++q) {
+         p0.v[q] = planets[q];
+         p1.v[q] = planets[DH_PLANET_STRIDE + q];
+      }
+   DH_TRY(x.report(m, b.P, p0, p1, b.mhd ? 1 : 0, b.prim, b.piph, b.dphi, b.repQ));
+   std::vector<double> R(n);
+   DH_TRY(x.get(R.data(), b.repQ, sizeof(double) * n));
+   double S[RP_N] = {0.0}, rho_min = HUGE_VAL, rhoavg_min = HUGE_VAL;
+   for (int j = m.NgRa; j < m.Nr - m.NgRb; ++j) {
+      double mass = 0.0, vol = 0.0;
+      for (int k = m.NgZa; k < m.Nz - m.NgZb; ++k) {
+         const double *r = &R[(size_t)(j + m.Nr * k) * RP_N];
+         for (int q = 0; q < RP_RHOMIN; ++q) S[q] += r[q];
+         mass += r[RP_MASS];
+         vol += r[RP_VOL];
+         if (rho_min > r[RP_RHOMIN]) rho_min = r[RP_RHOMIN];
+      }
+      double rho_avg = mass / vol;
+      if (rhoavg_min > rho_avg) rhoavg_min = rho_avg;
+   }
+   double vol = S[RP_VOL];
+   double v[16] = {S[RP_TORQUE], S[RP_POWER], S[RP_FR], rho_min, rhoavg_min, S[RP_PSIR],
+                   S[RP_PSII], S[RP_MASS], S[RP_MDOT] / vol, S[RP_SR] / S[RP_S0],
+                   S[RP_L1RHO] / vol, S[RP_L1ISEN] / vol, S[RP_L1B] / vol, S[RP_BR2],
+                   S[RP_BRBP] / S[RP_PDV], S[RP_PDV] / S[RP_B2]};
+   for (int q = 0; q < 16; ++q) out[q] = v[q];
+   return DH_OK;
 }
 
 template <class X>

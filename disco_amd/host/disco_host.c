@@ -59,13 +59,18 @@ struct disco_accel {
    /* in.par Graphs: 0 launches every kernel of a timestep eagerly; otherwise
     * (the default) the RK2 step is recorded once and replayed */
    int graphs;
+   /* possiblyOutput's report counter (domain.c:103, 316-335) */
+   int nrpt;
 };
 
 static struct disco_accel *accel_of(struct domain *d)
 {
    if (!d->accel) {
       struct disco_accel *A = calloc(1, sizeof(struct disco_accel));
-      if (A) A->graphs = 1;
+      if (A) {
+         A->graphs = 1;
+         A->nrpt = -1;
+      }
       d->accel = A;
    }
    return (struct disco_accel *)d->accel;
@@ -723,6 +728,42 @@ int disco_get_diagnostics(struct domain *d, double *Qrz)
    struct disco_accel *A = (struct disco_accel *)d->accel;
    return dh_get_diagnostics(A->ctx, Qrz) ? host_fail("dh_get_diagnostics: %s", dh_last_error())
                                           : 0;
+}
+
+/* report (report.c:7-202): appends t and the 16 global quantities of the
+ * current state to report.dat, in the reference's format. */
+void report(struct domain *d)
+{
+   struct disco_accel *A = (struct disco_accel *)d->accel;
+   double pl[DH_MAX_PLANETS * DH_PLANET_STRIDE], v[16];
+   planets_pack(d, pl);
+   if (dh_report(A->ctx, pl, d->Npl, v)) {
+      host_fail("dh_report: %s", dh_last_error());
+      exit(1);
+   }
+   FILE *f = fopen("report.dat", "a");
+   if (!f) return;
+   fprintf(f, "%le %le %le %le %le %le %le %le %le %le %le %le %le %le %le %le %le\n", d->t,
+           v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], v[9], v[10], v[11], v[12],
+           v[13], v[14], v[15]);
+   fclose(f);
+}
+
+/* The report branch of possiblyOutput (domain.c:316-335): a report whenever
+ * t passes the next of Num_Reports even (or logarithmic) marks, and always
+ * with override. */
+void possibly_report(struct domain *d, int override)
+{
+   struct disco_accel *A = accel_of(d);
+   double Nrpt = d->theParList.NumRepts;
+   int n0 = (int)(d->t * Nrpt / d->t_fin);
+   if (d->theParList.Out_LogTime)
+      n0 = (int)(Nrpt * log(d->t / d->t_init) / log(d->t_fin / d->t_init));
+   if (A->nrpt < n0 || override) {
+      A->nrpt = n0;
+      report(d);
+      printf("t = %.3e\n", d->t);
+   }
 }
 
 void add_diagnostics(struct domain *d, double dt)

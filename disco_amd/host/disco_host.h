@@ -43,6 +43,10 @@ double getmindt(struct domain *d);
 void timestep(struct domain *d, double dt);
 void check_dt(struct domain *d, double *dt);
 void add_diagnostics(struct domain *d, double dt);
+/* report (report.c:7-202) appended to report.dat, and possiblyOutput's
+ * schedule of it (domain.c:316-335). */
+void report(struct domain *d);
+void possibly_report(struct domain *d, int override);
 void avg_diagnostics(struct domain *d);
 void zero_diagnostics(struct domain *d);
 /* theTools.Qrz after the last step (disco_sync_to_host refreshes it). */

@@ -1,8 +1,10 @@
 /*
  * disco -- driver of one Disco run on the MI355X timestep (main.c:26-97 of the
  * reference, serial): reads ./in.par, sets the mesh up, then loops
- *     dt = getmindt(); check_dt(&dt); timestep(dt);
- * until T_End or an `abort` file, and writes the final state to
+ *     dt = getmindt(); check_dt(&dt); possibly_report(); timestep(dt);
+ * until T_End or an `abort` file, appending report() lines to report.dat
+ * on possiblyOutput's schedule (and one for the final state), and writes
+ * the final state to
  * output.dstate (DSTATE01 layout, disco_amd/state_io.py) and the Analysis
  * module's accumulated sums to output.diag (int32 nq, int32 Nr*Nz, double
  * t_avg, double Qrz[Nr*Nz*nq], as the reference keeps theTools).
@@ -33,14 +35,20 @@ int main(int argc, char **argv)
    if (disco_read_par(&D, "in.par")) return 1;
    if (disco_setup(&D)) return 1;
    if (argc > 2 && disco_restart_dstate(&D, argv[2])) return 1;
+   if (argc <= 2) {
+      FILE *rf = fopen("report.dat", "w");  /* main.c:64-67 */
+      if (rf) fclose(rf);
+   }
    remove("abort");
    double t0 = wall();
    while (!D.final_step && (max_steps < 0 || D.count_steps < max_steps)) {
       double dt = getmindt(&D);
       check_dt(&D, &dt);
+      possibly_report(&D, 0);
       timestep(&D, dt);
    }
    double secs = wall() - t0;
+   possibly_report(&D, 1);
    long ncells = 0;
    for (int k = D.NgZa; k < D.Nz - D.NgZb; ++k)
       for (int j = D.NgRa; j < D.Nr - D.NgRb; ++j) ncells += D.Np[j + D.Nr * k];

@@ -209,6 +209,15 @@ int dh_zero_diagnostics(dh_ctx *ctx);
 /* Divides the accumulated Qrz by t_avg (avg_diagnostics, analysis.c:28-49). */
 int dh_avg_diagnostics(dh_ctx *ctx, double t_avg);
 
+/* report (report.c:7-202) of the current state: out[16] receives the
+ * report.dat columns after t, in its order: Torque, Power, Fr, rho_min,
+ * rhoavg_min, PsiR, PsiI, Mass, Mdot, S_R, L1_rho, L1_isen, L1_B, Br2, aM,
+ * bM.  The planet terms use planets 0 and 1 (DH_PLANET_STRIDE each) when
+ * npl > 1, as the reference does when Npl > 1.  The sums run per annulus on
+ * the device and over the annuli in the reference's order on the host, so
+ * they equal the reference's to rounding. */
+int dh_report(dh_ctx *ctx, const double *planets, int npl, double *out);
+
 /* The learned initial guess of grmhd2's cons2prim (no counterpart in the
  * reference: an MLP 8 -> 64 -> 64 -> 2 run on MFMA).
  * dh_set_c2p_model loads its n = 4886 float parameters (layout in

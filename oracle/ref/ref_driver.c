@@ -15,7 +15,8 @@
  *      skipped: T_End is never reached in these runs)
  *   -> dump state_final.bin, dt.txt, diag.bin (the Analysis module's
  *      accumulated Qrz: int32 nq, int32 Nr*Nz, double t_avg,
- *      double Qrz[Nr*Nz*nq], analysis.c:52-106)
+ *      double Qrz[Nr*Nz*nq], analysis.c:52-106), and report_ref.dat (the
+ *      reference's report() line of the final state, report.c:7-202)
  *
  * Usage: disco_ref_<cfg> <outdir> <nsteps> [dump_every]
  *   in.par is read from the current directory (readpar.c:60).
@@ -29,6 +30,7 @@ void setupGrid(struct domain *);
 void setupDomain(struct domain *);
 void setupCells(struct domain *);
 void freeDomain(struct domain *);
+void report(struct domain *);
 void timestep(struct domain *, double);
 double getmindt(struct domain *);
 void boundary_trans(struct domain *, int);
@@ -163,6 +165,13 @@ int main(int argc, char *argv[])
    fwrite(&theDomain.theTools.t_avg, sizeof(double), 1, fq);
    fwrite(theDomain.theTools.Qrz, sizeof(double), (size_t)qhdr[0] * qhdr[1], fq);
    fclose(fq);
+
+   /* report (report.c:7-202) of the final state: the reference appends it
+    * to ./report.dat; kept as <outdir>/report_ref.dat. */
+   remove("report.dat");
+   report(&theDomain);
+   snprintf(path, sizeof path, "%s/report_ref.dat", outdir);
+   if (rename("report.dat", path)) { perror(path); return 2; }
 
    /* Real cells, as profiler.c:12-37 counts them. */
    long ncells = 0;

@@ -77,6 +77,14 @@ struct HostExec {
          diag_annulus(m, P, kind, p1, prim, piph, dphi, dt, a, Qrz);
       return DH_OK;
    }
+   int report(const Mesh &m, const dh_params &P, const DiagPlanet &p0, const DiagPlanet &p1,
+              int mhd, const double *prim, const double *piph, const double *dphi, double *R)
+   {
+      for (int a = 0; a < m.nann; ++a)
+         if (is_real(m, a % m.Nr, a / m.Nr))
+            report_annulus(m, P, p0, p1, mhd, prim, piph, dphi, a, R);
+      return DH_OK;
+   }
    template <int NQ, class H>
    int min_dt(const Mesh &m, const dh_params &P, const double *prim, const double *piph,
               const double *dphi, const double *wiph, double *out)
@@ -433,6 +441,12 @@ int dh_c2p_stats(dh_ctx *c, long long *out)
 
 int dh_enable_timers(dh_ctx *c, int) { return c ? DH_OK : fail(DH_EINVAL, "null context"); }
 int dh_enable_graphs(dh_ctx *c, int) { return c ? DH_OK : fail(DH_EINVAL, "null context"); }
+
+int dh_report(dh_ctx *c, const double *planets, int npl, double *out)
+{
+   if (!c || !out) return fail(DH_EINVAL, "null argument");
+   return report(c->x, c->b, planets, npl, out);
+}
 
 int dh_read_timers(dh_ctx *c, int, char (*)[48], double *, long *, int *nrows)
 {

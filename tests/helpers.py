@@ -148,3 +148,26 @@ def div_b(s: MeshState, faces) -> np.ndarray:
         np.add.at(mag, L, np.abs(f))
         np.add.at(mag, R, np.abs(f))
     return np.abs(net) / np.where(mag > 0, mag, 1.0)
+
+
+REPORT_COLUMNS = ("t", "Torque", "Power", "Fr", "rho_min", "rhoavg_min", "PsiR", "PsiI", "Mass",
+                  "Mdot", "S_R", "L1_rho", "L1_isen", "L1_B", "Br2", "aM", "bM")
+
+
+def read_report(path: str) -> np.ndarray:
+    """report.dat rows (report.c:188-191): t and 16 columns each."""
+    with open(path) as f:
+        rows = [[float(x) for x in line.split()] for line in f if line.strip()]
+    return np.array(rows).reshape(-1, len(REPORT_COLUMNS))
+
+
+def assert_reports_match(got, want, rtol=2e-6):
+    """Two report rows, equal to the 7 significant digits report.c prints
+    (%le), with a floor of 1e-12 of the disk's mass for columns that are
+    sums of cancelling terms (PsiR, PsiI of a nearly axisymmetric disk)."""
+    floor = 1e-12 * max(abs(want[8]), 1.0)
+    for name, a, b in zip(REPORT_COLUMNS, got, want):
+        if not np.isfinite(b):
+            assert a == b or (np.isnan(a) and np.isnan(b)), (name, a, b)
+            continue
+        assert abs(a - b) <= rtol * max(abs(a), abs(b)) + floor, (name, a, b)

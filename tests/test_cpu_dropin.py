@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 from disco_amd import configs, state_io
+from helpers import assert_reports_match, read_report
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpu_engine"))
 import build as cpu_build  # noqa: E402
@@ -80,6 +81,13 @@ def test_driver_bit_exact_vs_reference(cname, over, steps):
     assert t_got == t_want and np.array_equal(q_got, q_want)
     if len(want.planets) > 1:
         assert np.any(q_want[:, 3] != 0.0)  # the torque column reads planet 1
+    # report.c's line for the final state: the reference's (report_ref.dat)
+    # and the driver's last report.dat line (its first is t = 0)
+    rep = read_report(os.path.join(d, "report.dat"))
+    assert rep[0, 0] == 0.0 and abs(rep[-1, 0] - want.t) <= 1e-6 * want.t
+    assert_reports_match(rep[-1], read_report(os.path.join(d, "report_ref.dat"))[-1])
+    if len(want.planets) > 1:
+        assert rep[-1, 1] != 0.0  # the torque reads planets 0 and 1
 
 
 @pytest.mark.parametrize("cname,over,n1,n2", [

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from disco_amd import configs, state_io
-from helpers import PARITY_TOL, compare_states, rel_l1
+from helpers import PARITY_TOL, assert_reports_match, compare_states, read_report, rel_l1
 
 pytestmark = pytest.mark.gpu
 HOST = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
@@ -75,3 +75,8 @@ def test_disco_driver_matches_reference(cname, over, steps):
     err = np.abs(q_got - q_want).sum(axis=0) / scale
     print(f"  diagnostics ({cfg.analysis}) worst column {err.max():.2e}", flush=True)
     assert err.max() <= PARITY_TOL
+    # report.c's line for the final state (report_ref.dat) against the
+    # driver's last report.dat line, to the digits report.c prints
+    rep = read_report(os.path.join(d, "report.dat"))
+    assert rep[0, 0] == 0.0
+    assert_reports_match(rep[-1], read_report(os.path.join(d, "report_ref.dat"))[-1])
