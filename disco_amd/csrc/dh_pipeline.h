@@ -75,6 +75,11 @@ struct Buffers {
    int bc_maxnp[4][8] = {{0}};      // largest Np of each side's ghost layers
    PlanetSet pl[2];
    StepArgs *step = nullptr;  // device copy of {dt, planets} of the current timestep
+   // The ring starts, cell ranges, faces and edge-line steps in the buffers
+   // are those of the current face positions (set by an RK2 step, whose
+   // second substep builds them for positions it does not move; cleared by
+   // anything that changes the positions from the host)
+   bool topo_valid = false;
    // diagnostics (analysis.c:52-106): Qrz[nann * DH_DIAG_MAXQ], allocated on
    // first use; the kind that accumulated it
    double *diagQ = nullptr;
@@ -385,22 +390,24 @@ struct Engine {
    Buffers &b;
 
    // Ring starts, cell ranges and faces of both directions, and the r/z
-   // slopes when grads is set (plm.c:56-232).
+   // slopes when grads is set (plm.c:56-232).  With reuse the buffers
+   // already hold the faces of the current positions and only the slopes
+   // are formed.
    template <int NQ>
-   int topology(bool grads)
+   int topology(bool grads, bool reuse = false)
    {
       const Mesh &m = b.m;
       if (!dim_active(m, 1) && !dim_active(m, 2)) return DH_OK;
-      DH_TRY(x.ring_start(m, b.piph, b.I0));
+      if (!reuse) DH_TRY(x.ring_start(m, b.piph, b.I0));
       for (int dim = 1; dim <= 2; ++dim) {
          int d = dim - 1;
          if (!dim_active(m, dim)) continue;
          double *grad = dim == 1 ? b.gradr : b.gradz;
          bool slopes = grads && b.P.plm != 0.0;
-         if (dim == 1) {
+         if (!reuse && dim == 1) {
             DH_TRY(x.run("ranges_r", KCellRanges<1>{m, b.piph, b.rng[d]}, m.ntot));
             DH_TRY(x.run("faces_r", KFaces<1>{m, b.F[d], b.rng[d], b.piph, b.dphi}, m.ntot));
-         } else {
+         } else if (!reuse) {
             DH_TRY(x.run("ranges_z", KCellRanges<2>{m, b.piph, b.rng[d]}, m.ntot));
             DH_TRY(run_in(x, "faces_z", KFaces<2>{m, b.F[d], b.rng[d], b.piph, b.dphi}, m.ntot,
                           b.zord_c));
@@ -447,9 +454,10 @@ struct Engine {
    }
 
    // One substep (onestep.c:84-228); its dt and planets are b.step's
-   // entries sub.
+   // entries sub.  reuse: the faces of the current positions are already
+   // built (buildfaces would rebuild the same ones).
    template <int NQ, class H>
-   int substep(double RK, int first_step, int sub)
+   int substep(double RK, int first_step, int sub, bool reuse)
    {
       const double *dt = &b.step->dt[sub];
       constexpr bool mhd = IsMHD<H>::value;
@@ -460,7 +468,7 @@ struct Engine {
          if (P.mesh_motion == 3) DH_TRY(x.run("wcell", KWcellAvg{m, b.wiph}, m.nann));
       }
       DH_TRY(x.run("plm_phi", KPlmPhi<NQ>{m, P.plm, b.prim, b.dphi, b.gradp}, m.ntot));
-      DH_TRY(topology<NQ>(true));
+      DH_TRY(topology<NQ>(true, reuse));
       KFluxPhi<NQ, H> kp{m, P, dt, b.prim, b.gradp, b.gradr, b.gradz, b.piph, b.dphi, b.wiph,
                          b.dFp, b.emf_phi};
       DH_TRY(x.run("flux_phi", kp, m.ntot));
@@ -496,9 +504,10 @@ struct Engine {
          KUpdateB kb{m, RK, dt, b.F[0], b.F[1], b.rng[0], b.rng[1], b.Eo, b.Phi, b.rkPhi};
          DH_TRY(x.run("update_B", kb, m.ntot));
          if (dim_active(m, 2)) {
-            if (dim_active(m, 1))
+            if (dim_active(m, 1) && !reuse)
                DH_TRY(x.run("line_steps_r", KLineSteps<1>{m, b.F[0], b.estep[0]}, b.F[0].n));
-            DH_TRY(x.run("line_steps_z", KLineSteps<2>{m, b.F[1], b.estep[1]}, b.F[1].n));
+            if (!reuse)
+               DH_TRY(x.run("line_steps_z", KLineSteps<2>{m, b.F[1], b.estep[1]}, b.F[1].n));
             KEdgeSegments ks{m, dt, b.F[0], b.F[1], b.estep[0], b.estep[1], b.lbase[0], b.lbase[1],
                              b.piph, b.dphi, b.Eown, b.seg[0], b.seg[1]};
             DH_TRY(x.run("edge_segments", ks, (long)b.F[0].n + b.F[1].n));
@@ -550,6 +559,9 @@ struct Engine {
    // may replay it as a recorded graph.  An RK2 step swaps the face
    // positions twice and so leaves every buffer pointer where it was; a
    // forward-Euler step does not (then the step is never recorded).
+   // The first substep starts from the positions the last RK2 step's second
+   // substep left unmoved, so it takes that substep's faces instead of
+   // rebuilding them; a step with nothing to take is not recorded either.
    template <int NQ, class H>
    int timestep(double dt)
    {
@@ -563,13 +575,17 @@ struct Engine {
       sa.pl[1] = b.pl[1];
       DH_TRY(x.set_step(b.step, sa));
       bool rk2 = b.P.timestep_mode != 1;
-      return x.replayable(rk2 ? (const void *)b.piph : nullptr, [&]() -> int {
+      bool reuse = b.topo_valid;
+      b.topo_valid = false;
+      DH_TRY(x.replayable(rk2 && reuse ? (const void *)b.piph : nullptr, [&]() -> int {
          DH_TRY(x.copy(b.rkcons, b.cons, sizeof(double) * NQ * b.m.ntot));
          if (b.mhd) DH_TRY(x.copy(b.rkPhi, b.Phi, sizeof(double) * 5 * b.m.ntot));
-         DH_TRY((substep<NQ, H>(0.0, 1, 0)));
-         if (rk2) DH_TRY((substep<NQ, H>(0.5, 0, 1)));
+         DH_TRY((substep<NQ, H>(0.0, 1, 0, reuse)));
+         if (rk2) DH_TRY((substep<NQ, H>(0.5, 0, 1, false)));
          return DH_OK;
-      });
+      }));
+      b.topo_valid = rk2;
+      return DH_OK;
    }
 
    // setupCells after the initial prims are in place (domain.c:139-242):
@@ -580,6 +596,7 @@ struct Engine {
    int init()
    {
       const Mesh &m = b.m;
+      b.topo_valid = false;
       DH_TRY(x.run("init", KInitCells<NQ, H>{m, b.P, b.prim, b.cons, b.piph, b.dphi}, m.ntot));
       if constexpr (IsMHD<H>::value) {
          // The reference also sets the primitive field of every cell here
@@ -654,6 +671,7 @@ int upload_state(X &x, Buffers &b, const double *prim, const double *cons, const
    if (prim) DH_TRY(x.put(b.prim, prim, qb));
    if (cons) DH_TRY(x.put(b.cons, cons, qb));
    if (piph) DH_TRY(x.put(b.piph, piph, sizeof(double) * nt));
+   if (piph || dphi) b.topo_valid = false;
    if (wiph) DH_TRY(x.put(b.wiph, wiph, sizeof(double) * nt));
    if (Phi && b.nf) DH_TRY(x.put(b.Phi, Phi, sizeof(double) * nt * b.nf));
    if (dphi) {
