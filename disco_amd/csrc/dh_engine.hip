@@ -266,15 +266,22 @@ __global__ __launch_bounds__(TPB) void k_report(Mesh m, dh_params P, DiagPlanet 
 }
 
 // CFL reduction over real cells (misc.c:39-84).  Positive doubles order like
-// their bit patterns, so the minimum is an unsigned 64-bit atomic min.
+// their bit patterns, so the minimum is an unsigned 64-bit atomic min, one
+// per workgroup; the grid is capped (MINDT_BLOCKS) and each thread takes
+// every stride-th cell, so that few atomics meet at the one address.
+constexpr unsigned MINDT_BLOCKS = 2048;
 template <int NQ, class H>
 __global__ __launch_bounds__(TPB) void k_mindt(Mesh m, dh_params P, const double *prim,
                                                const double *piph, const double *dphi,
                                                const double *wiph, unsigned long long *out)
 {
    __shared__ double red[TPB / 64];
-   long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-   double dtc = c < m.ntot ? cell_mindt<NQ, H>(m, P, prim, piph, dphi, wiph, c) : HUGE_VAL;
+   long stride = (long)gridDim.x * blockDim.x;
+   double dtc = HUGE_VAL;
+   for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < m.ntot; c += stride) {
+      double v = cell_mindt<NQ, H>(m, P, prim, piph, dphi, wiph, c);
+      dtc = v < dtc ? v : dtc;
+   }
    for (int o = 32; o > 0; o >>= 1) {
       double v = __shfl_down(dtc, o, 64);
       dtc = v < dtc ? v : dtc;
@@ -553,7 +560,8 @@ struct HipExec {
       HIPCHK(hipMemcpyAsync(d_min, &init, sizeof init, hipMemcpyHostToDevice, stream));
       {
          Bracket b(*this, "mindt");
-         k_mindt<NQ, H><<<nblocks(m.ntot), TPB, 0, stream>>>(m, P, prim, piph, dphi, wiph, d_min);
+         unsigned nb = nblocks(m.ntot) < MINDT_BLOCKS ? nblocks(m.ntot) : MINDT_BLOCKS;
+         k_mindt<NQ, H><<<nb, TPB, 0, stream>>>(m, P, prim, piph, dphi, wiph, d_min);
       }
       LAUNCH_CHECK();
       unsigned long long bits = 0;
