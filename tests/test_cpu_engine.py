@@ -115,6 +115,31 @@ def test_cpu_build_bit_exact_vs_oracle(cpu_lib, cname, over, steps):
     g.close()
 
 
+@pytest.mark.parametrize("cname,over,steps", [
+    ("euler_yee", {"Num_R": 24}, 5),
+    ("mhd_kep", {"Num_R": 12, "Num_Z": 6, "Max_DT": 0.02}, 5),
+])
+def test_face_reuse_equals_rebuild(cpu_lib, cname, over, steps):
+    """An RK2 step's first substep takes the faces the previous step's
+    second substep built (no substep between them moves the mesh); a run
+    whose state is uploaded again before every step, which makes every step
+    build its faces afresh, is bit-identical."""
+    cfg, par, a = _ref_initial(cname, over)
+    g = capi.DiscoHip(cfg, par, a, library=cpu_lib)
+    h = capi.DiscoHip(cfg, par, a, library=cpu_lib)
+    for _ in range(steps):
+        dt = g.getmindt()
+        assert h.getmindt() == dt
+        g.timestep(dt)
+        h.upload(h.download(a))
+        h.timestep(dt)
+    x, y = g.download(a), h.download(a)
+    g.close()
+    h.close()
+    for key in ("prim", "cons", "piph", "dphi", "phi_flux"):
+        assert np.array_equal(getattr(x, key), getattr(y, key)), key
+
+
 def test_cpu_build_faces_match_oracle(cpu_lib):
     cfg, par, a = _ref_initial("mhd_kep", {"Num_R": 8, "Num_Z": 4})
     g = capi.DiscoHip(cfg, par, a, library=cpu_lib)
