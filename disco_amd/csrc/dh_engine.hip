@@ -193,25 +193,26 @@ __global__ __launch_bounds__(TPB) void k_bc_scan(Mesh m, int side, const double 
    if (s_miss && tid == 0) KBcScan{m, side, piph, dphi, start}(t);
 }
 
-// add_diagnostics (analysis.c:52-106): one wavefront per annulus; each lane
-// sums get_diagnostics x dV and dV over every 64th cell, then the wave adds
-// the lanes' sums.  (The host build sums the cells in order, as the
-// reference does; the two differ by rounding only.)
+// add_diagnostics (analysis.c:52-106): one workgroup per annulus; each
+// thread sums get_diagnostics x dV and dV over every TPB-th cell, each wave
+// adds its lanes' sums and thread 0 the waves'.  (The host build sums the
+// cells in order, as the reference does; the two differ by rounding only.)
 template <int KIND>
 __global__ __launch_bounds__(TPB) void k_diag(Mesh m, dh_params P, DiagPlanet p1,
                                               const double *prim, const double *piph,
                                               const double *dphi, double dt, double *Qrz)
 {
    constexpr int NQD = KIND == DH_DIAG_EULER ? 8 : KIND == DH_DIAG_DISK2D ? 5 : 18;
-   int a = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-   int lane = threadIdx.x & 63;
+   __shared__ double red[TPB / 64][NQD + 1];
+   int a = blockIdx.x;  // the whole workgroup's annulus
+   int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
    if (a >= m.nann) return;
    size_t nt = (size_t)m.ntot;
    int j = a % m.Nr, k = a / m.Nr;
    double sum[NQD], vol = 0.0;
 #pragma unroll
    for (int q = 0; q < NQD; ++q) sum[q] = 0.0;
-   for (int i = lane; i < m.Np[a]; i += 64) {
+   for (int i = tid; i < m.Np[a]; i += TPB) {
       long c = m.off[a] + i;
       double x[3], dV, p[BZZ + 1], Q[NQD];
       diag_geometry(m, piph, dphi, c, j, k, x, &dV);
@@ -228,9 +229,21 @@ __global__ __launch_bounds__(TPB) void k_diag(Mesh m, dh_params P, DiagPlanet p1
       for (int q = 0; q < NQD; ++q) sum[q] += __shfl_down(sum[q], o, 64);
       vol += __shfl_down(vol, o, 64);
    }
-   if (lane == 0)
+   if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < NQD; ++q) red[wv][q] = sum[q];
+      red[wv][NQD] = vol;
+   }
+   __syncthreads();
+   if (tid == 0) {
+      for (int w = 1; w < TPB / 64; ++w) {
+#pragma unroll
+         for (int q = 0; q < NQD; ++q) sum[q] += red[w][q];
+         vol += red[w][NQD];
+      }
 #pragma unroll
       for (int q = 0; q < NQD; ++q) Qrz[(size_t)a * NQD + q] += sum[q] * dt / vol;
+   }
 }
 
 // report (report.c:7-202): one wavefront per real annulus; each lane sums
@@ -526,7 +539,7 @@ struct HipExec {
    int diag(const Mesh &m, const dh_params &P, int kind, const DiagPlanet &p1,
             const double *prim, const double *piph, const double *dphi, double dt, double *Qrz)
    {
-      unsigned nb = (unsigned)((m.nann + 3) / 4);
+      unsigned nb = (unsigned)m.nann;
       {
          Bracket b(*this, "diagnostics");
          if (kind == DH_DIAG_EULER)
