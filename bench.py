@@ -10,16 +10,25 @@ MHD Keplerian disk (Nr=256, Nz=64, constrained transport).  The timed region
 is exactly K steps between device synchronisations.  Prints one JSON line:
 
   value          zone-updates/s = real cells x K / seconds
-  roofline       the dominant kernel (most device time per step): algorithmic
-                 bytes per launch / its mean launch time (HIP events on the
-                 library's stream), against the 8 TB/s HBM3E peak; traffic is
-                 the measured HBM bytes per launch of that kernel from the
-                 rocprofv3 counter summary under profiles/, when there is one
+  roofline       the dominant kernel (most device time per step) against the
+                 8 TB/s HBM3E peak: achieved = B_alg / 2 x real cells (the
+                 state bytes of one substep, which one launch of a per-substep
+                 kernel advances) / its mean launch time (HIP events on the
+                 library's stream); traffic is the measured HBM bytes per
+                 launch of that kernel from the rocprofv3 counter summary under
+                 profiles/, when there is one.  kernel_model_GBps beside it is
+                 the kernel's own byte model (every array it touches read or
+                 written once) over the same time
   step_roofline  B_alg x zone-updates/s against 8 TB/s, with
                  B_alg = 40 (NQ + NF + 1) bytes per zone-update (the state
                  {cons, Phi, piph} read and written over the RK2 step)
-  cpu_baseline   the reference build (oracle/_ref) on this host, one core, on a
-                 bounded sample of the same workload
+  cpu_baseline   the reference build (oracle/_ref) on this host's cores: one
+                 serial process per core, each stepping the same bounded
+                 sample of the workload; value = their summed zone-updates/s
+  grmhd_c2p      the GRMHD workload stepped with the plain (previous
+                 primitives) and the learned cons2prim guess, both with the
+                 residual stopping rule (both residuals below 1e-10): the
+                 ratio of their zone-updates/s
 """
 from __future__ import annotations
 
@@ -53,16 +62,14 @@ WORKLOADS = {
         cpu_over={}, cpu_steps=150),
     "mhd_c5": dict(
         config="mhd_kep",
-        over={"Num_R": 512, "Num_Z": 160, "R_Min": 0.5, "R_Max": 3.0, "Z_Min": -0.3,
-              "Z_Max": 0.3, "Init_Par0": 1, "Init_Par1": 100.0},
+        over={**configs.C5_PAR, "Num_R": 512, "Num_Z": 160},
         steps=5, warmup=1,
         desc="3-D Newtonian MHD Keplerian disk at the one-GPU memory-filling size: r in [0.5,3], "
              "z in [-0.3,0.3], Nr=512, Nz=160, Np=int(2 pi r/dr) per annulus, rho=1, "
              "cs=v_K/20, zero-net-flux toroidal field B_phi ~ sin(2 pi (r-0.5)/2.5) at beta=100, "
              "velocity noise 5e-5 v_phi; HLLD, PLM 1.5, constrained transport, Keplerian mesh "
              "motion, RK2, CFL 0.5",
-        cpu_over={"Num_R": 64, "Num_Z": 20, "R_Min": 0.5, "R_Max": 3.0, "Z_Min": -0.3,
-                  "Z_Max": 0.3, "Init_Par0": 1, "Init_Par1": 100.0},
+        cpu_over={**configs.C5_PAR, "Num_R": 64, "Num_Z": 20},
         cpu_steps=3),
     "disk_ensemble": dict(
         config="disk_planet", over={}, steps=200, warmup=2, members=64, seed=314159,
@@ -172,25 +179,89 @@ def kernel_bytes(name: str, nq: int, mhd: bool, nt: int, nr: int, nfr: int, nfz:
     return 0.0
 
 
-def cpu_baseline(cfg, over, steps):
+def host_cpu():
+    """(model name, logical CPUs this process may run on) of the host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    return model, ncpu
+
+
+# The GPU box gives one GPU's share of its host: 16 cores.
+CPU_BASELINE_MAX_CORES = 16
+
+
+def cpu_baseline(cfg, over, steps, cores=None):
+    """The reference build (oracle/_ref, serial: USE_MPI=0) on this host:
+    `cores` processes, one per core, each stepping the same sample of the
+    workload; value = the sum of their zone-updates/s (cells x steps over
+    each process's own stepping time), i.e. the host's throughput with the
+    work split into independent pieces, an upper bound for the reference's
+    MPI decomposition of the same cells."""
     exe = configs.ref_binary(cfg)
     if not os.path.exists(exe):
         return None
+    model, ncpu = host_cpu()
+    cores = cores or min(ncpu, CPU_BASELINE_MAX_CORES)
     par = cfg.par_dict(**over)
     with tempfile.TemporaryDirectory(prefix="bench_ref_") as d:
-        state_io.write_par(d, par)
-        r = subprocess.run([exe, d, str(steps)], cwd=d, capture_output=True, text=True,
-                           timeout=900)
-        if r.returncode != 0:
-            return None
-        line = [ln for ln in r.stdout.splitlines() if ln.startswith("{")][-1]
-        res = json.loads(line)
+        procs = []
+        for i in range(cores):
+            wd = os.path.join(d, str(i))
+            state_io.write_par(wd, par)
+            procs.append(subprocess.Popen([exe, wd, str(steps)], cwd=wd, stdout=subprocess.PIPE,
+                                          stderr=subprocess.DEVNULL, text=True))
+        results = []
+        for p in procs:
+            out, _ = p.communicate(timeout=900)
+            if p.returncode != 0:
+                return None
+            line = [ln for ln in out.splitlines() if ln.startswith("{")][-1]
+            results.append(json.loads(line))
+    value = sum(r["zone_updates_per_s"] for r in results)
+    secs = max(r["seconds"] for r in results)
     size = ", ".join(f"{k}={v}" for k, v in over.items()) or "the full mesh"
-    return {"value": res["zone_updates_per_s"], "unit": "zone-updates/s", "cores": 1,
-            "kind": "reference",
-            "sample": f"{steps} RK2 steps of the same workload at {size} ({res['cells']} cells) "
-                      f"with the reference timestep built from its sources (gcc -O3, serial, "
-                      f"USE_MPI=0), {res['seconds']:.1f} s of stepping"}
+    return {"value": value, "unit": "zone-updates/s", "cores": cores, "kind": "reference",
+            "cpu_model": model, "host_logical_cpus": ncpu,
+            "per_core": value / cores,
+            "sample": f"{cores} concurrent serial runs (one per core) of {steps} RK2 steps of the "
+                      f"same workload at {size} ({results[0]['cells']} cells each) with the "
+                      f"reference timestep built from its sources (gcc -O3, USE_MPI=0); "
+                      f"{secs:.1f} s of stepping; value is their summed zone-updates/s"}
+
+
+def dominant_roofline(kernels, ba, ncell, workload):
+    """Roofline of the kernel with the most device time per step.  Its
+    algorithmic bytes per launch are the survey's per-zone-update figure
+    B_alg taken over one substep (B_alg / 2: the RK2 step reads and writes
+    the state {cons, Phi, piph} twice) times the real cells a launch covers;
+    the kernel's own byte model (every array it touches once) is reported
+    beside it as kernel_model_GBps."""
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    row = kernels[dom]
+    mean_s = row["mean_launch_ms"] * 1e-3
+    alg = ba / 2 * ncell
+    ach = alg / mean_s / 1e9 if mean_s > 0 else 0.0
+    return {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
+            "traffic": measured_traffic(dom, workload),
+            "algorithmic_bytes_per_launch": alg,
+            "algorithmic_bytes_rule": f"B_alg/2 = {ba / 2:g} B per real cell per substep x "
+                                      f"{ncell} cells",
+            "mean_launch_ms": row["mean_launch_ms"],
+            "kernel_model_bytes_per_launch": row["model_bytes_per_launch"],
+            "kernel_model_GBps": row["model_GBps"],
+            "share_of_step": row["ms_per_step"] / sum(v["ms_per_step"] for v in kernels.values())}
 
 
 def measured_traffic(kernel: str, workload: str):
@@ -290,6 +361,67 @@ def run_ensemble(args, w, cfg):
     return out
 
 
+# The GRMHD workload of the cons2prim comparison and its steps.
+C2P_COMPARE = dict(workload="gr_disk", steps=10, warmup=2)
+
+
+def timed_host_run(config, par, steps, warmup, graphs=True, label=""):
+    """Steps a host run `warmup` + `steps` times; returns (zone-updates/s,
+    seconds, real cells, c2p stats of the last cons2prim or None)."""
+    import ctypes as C
+    from disco_amd import capi
+    from disco_amd.host_lib import HostRun
+    tmp = tempfile.mkdtemp(prefix="bench_")
+    run = HostRun(config, state_io.write_par(tmp, par))
+    lib = capi.lib()
+    ctx = C.c_void_p(run.ctx)
+    try:
+        if lib.dh_enable_graphs(ctx, 1 if graphs else 0):
+            raise RuntimeError(lib.dh_last_error().decode())
+        for _ in range(warmup):
+            run.timestep(run.getmindt())
+        if lib.dh_sync(ctx):
+            raise RuntimeError(lib.dh_last_error().decode())
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run.timestep(run.getmindt())
+        if lib.dh_sync(ctx):
+            raise RuntimeError(lib.dh_last_error().decode())
+        secs = time.perf_counter() - t0
+        st = (C.c_longlong * 4)()
+        stats = None
+        if lib.dh_c2p_stats(ctx, st) == 0 and st[0]:
+            stats = {"iterations_per_cell": st[1] / st[0], "learned_fraction": st[2] / st[0],
+                     "fallback_fraction": st[3] / st[0]}
+        ncell = run.real_cells
+    finally:
+        run.close()
+    print(f"bench {label}: {steps} timed steps in {secs:.2f} s", file=sys.stderr, flush=True)
+    return ncell * steps / secs, secs, ncell, stats
+
+
+def grmhd_c2p_compare(args):
+    """The metric's second half: the GRMHD step with the learned cons2prim
+    guess against the plain one (the previous primitives), both stopping
+    when both Newton residuals are below 1e-10 (C2P_Stop 1)."""
+    from disco_amd import c2p_model
+    w = WORKLOADS[C2P_COMPARE["workload"]]
+    cfg = configs.get(w["config"])
+    base = cfg.par_dict(**w["over"])
+    base["C2P_Stop"] = 1
+    res = {}
+    for name, over in (("plain", {}),
+                       ("learned", {"C2P_Guess": 1, "C2P_Model": c2p_model.MODEL_PATH})):
+        par = dict(base, **over)
+        v, secs, ncell, stats = timed_host_run(w["config"], par, C2P_COMPARE["steps"],
+                                               C2P_COMPARE["warmup"], label=f"c2p {name}")
+        res[name] = {"value": v, "ms_per_step": secs / C2P_COMPARE["steps"] * 1e3, "c2p": stats}
+    return {"workload": w["desc"], "name": C2P_COMPARE["workload"], "real_cells": ncell,
+            "stop_rule": "both residuals below 1e-10 (C2P_Stop 1)",
+            "plain": res["plain"], "learned": res["learned"],
+            "learned_over_plain": res["learned"]["value"] / res["plain"]["value"]}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -304,6 +436,8 @@ def main(argv=None):
                     help="grmhd2 cons2prim's stopping rule")
     ap.add_argument("--graphs", choices=["on", "off"], default="on",
                     help="record the RK2 timestep once as a HIP graph and replay it")
+    ap.add_argument("--no-c2p-compare", action="store_true",
+                    help="skip the GRMHD plain vs learned cons2prim guess comparison")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(json.dumps({"error": "disco_amd runs one mesh on one GPU; sharding the annulus "
@@ -399,17 +533,9 @@ def main(argv=None):
             mean_s = ms[k] / max(calls[k], 1) * 1e-3
             kernels[nm] = {"ms_per_step": ms[k] / tsteps, "launches_per_step": calls[k] / tsteps,
                            "mean_launch_ms": mean_s * 1e3,
-                           "alg_bytes_per_launch": alg,
-                           "alg_GBps": alg / mean_s / 1e9 if mean_s > 0 and alg else None}
-        dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
-        row = kernels[dom]
-        ach = row["alg_GBps"] or 0.0
-        traffic = measured_traffic(dom, args.workload)
-        roofline = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS,
-                    "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": traffic,
-                    "algorithmic_bytes_per_launch": row["alg_bytes_per_launch"],
-                    "mean_launch_ms": row["mean_launch_ms"],
-                    "share_of_step": row["ms_per_step"] / sum(v["ms_per_step"] for v in kernels.values())}
+                           "model_bytes_per_launch": alg,
+                           "model_GBps": alg / mean_s / 1e9 if mean_s > 0 and alg else None}
+        roofline = dominant_roofline(kernels, b_alg(nq, nf), ncell, args.workload)
 
     ba = b_alg(nq, nf)
     step_ach = ba * value / 1e9
@@ -443,6 +569,8 @@ def main(argv=None):
                           "iterations_per_cell": st[1] / st[0],
                           "learned_fraction": st[2] / st[0], "fallback_fraction": st[3] / st[0]}
     run.close()
+    if not args.no_c2p_compare:
+        out["grmhd_c2p"] = grmhd_c2p_compare(args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, w["cpu_over"], w["cpu_steps"])
     print(json.dumps(out), flush=True)

@@ -152,6 +152,13 @@ CONFIGS["mhd_kep_p2"] = dataclasses.replace(
     par=CONFIGS["mhd_kep"].par + (("Mass_Ratio", 1e-3),))
 
 
+# The memory-filling MHD disk's in.par on top of mhd_kep: r in [0.5, 3],
+# z in [-0.3, 0.3], a zero-net-flux toroidal field at beta = 100
+# (kep_mhd_disk with Init_Par0 = 1); the full size is Nr = 512, Nz = 160.
+C5_PAR = {"R_Min": 0.5, "R_Max": 3.0, "Z_Min": -0.3, "Z_Max": 0.3, "Init_Par0": 1,
+          "Init_Par1": 100.0}
+
+
 def get(name: str) -> Config:
     return CONFIGS[name]
 

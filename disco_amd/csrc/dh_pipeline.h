@@ -77,8 +77,10 @@ struct Buffers {
    StepArgs *step = nullptr;  // device copy of {dt, planets} of the current timestep
    // The ring starts, cell ranges, faces and edge-line steps in the buffers
    // are those of the current face positions (set by an RK2 step, whose
-   // second substep builds them for positions it does not move; cleared by
-   // anything that changes the positions from the host)
+   // second substep builds them for positions it does not move).  Every
+   // path that writes piph or dphi other than the timestep itself (state
+   // upload, cell set-up, a restart, any new entry point) must clear it, or
+   // the next step would take faces of the old positions.
    bool topo_valid = false;
    // diagnostics (analysis.c:52-106): Qrz[nann * DH_DIAG_MAXQ], allocated on
    // first use; the kind that accumulated it

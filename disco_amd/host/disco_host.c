@@ -732,7 +732,7 @@ int disco_get_diagnostics(struct domain *d, double *Qrz)
 
 /* report (report.c:7-202): appends t and the 16 global quantities of the
  * current state to report.dat, in the reference's format. */
-void report(struct domain *d)
+void disco_report(struct domain *d)
 {
    struct disco_accel *A = (struct disco_accel *)d->accel;
    double pl[DH_MAX_PLANETS * DH_PLANET_STRIDE], v[16];
@@ -752,7 +752,7 @@ void report(struct domain *d)
 /* The report branch of possiblyOutput (domain.c:316-335): a report whenever
  * t passes the next of Num_Reports even (or logarithmic) marks, and always
  * with override. */
-void possibly_report(struct domain *d, int override)
+void disco_possibly_report(struct domain *d, int override)
 {
    struct disco_accel *A = accel_of(d);
    double Nrpt = d->theParList.NumRepts;
@@ -761,7 +761,7 @@ void possibly_report(struct domain *d, int override)
       n0 = (int)(Nrpt * log(d->t / d->t_init) / log(d->t_fin / d->t_init));
    if (A->nrpt < n0 || override) {
       A->nrpt = n0;
-      report(d);
+      disco_report(d);
       printf("t = %.3e\n", d->t);
    }
 }
@@ -870,9 +870,9 @@ void timestep(struct domain *d, double dt)
    d->count_steps += 1;
 }
 
-/* check_dt, domain.c:284-311 (serial; the abort/latest files as in the
+/* check_dt, domain.c:284-311, under its own name (serial; the abort/latest files as in the
  * reference). */
-void check_dt(struct domain *d, double *dt)
+void disco_check_dt(struct domain *d, double *dt)
 {
    int final = 0;
    if (d->t + *dt > d->t_fin) {

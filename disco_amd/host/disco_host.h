@@ -5,9 +5,15 @@
  * Reference-signature entry points (what a Disco driver links against):
  *   double getmindt(struct domain *)            misc.c:39-84
  *   void   timestep(struct domain *, double)    timestep.c:7-45
- *   void   check_dt(struct domain *, double *)  domain.c:284-311
  *   void   add_diagnostics / avg_diagnostics / zero_diagnostics
- *                                               analysis.c:8-106
+ *                                               analysis.c:8-106 (the sums
+ *          live on the device: a binding leaves analysis.o out)
+ * Helpers under their own names, so that a binding that keeps the
+ * reference's domain.o and report.o (which define check_dt and report)
+ * links without two definitions of one symbol:
+ *   disco_check_dt        check_dt, domain.c:284-311
+ *   disco_report          report, report.c:7-202, from the device's sums
+ *   disco_possibly_report possiblyOutput's report schedule, domain.c:316-335
  * Set-up (readpar.c:44-188, gridsetup.c:15-144, domain.c:26-242):
  *   disco_read_par, disco_setup_grid, disco_setup
  * Device state <-> host mirror: disco_sync_to_host, disco_export_state.
@@ -41,12 +47,12 @@ int disco_restart_dstate(struct domain *d, const char *path);
 
 double getmindt(struct domain *d);
 void timestep(struct domain *d, double dt);
-void check_dt(struct domain *d, double *dt);
+void disco_check_dt(struct domain *d, double *dt);
 void add_diagnostics(struct domain *d, double dt);
 /* report (report.c:7-202) appended to report.dat, and possiblyOutput's
  * schedule of it (domain.c:316-335). */
-void report(struct domain *d);
-void possibly_report(struct domain *d, int override);
+void disco_report(struct domain *d);
+void disco_possibly_report(struct domain *d, int override);
 void avg_diagnostics(struct domain *d);
 void zero_diagnostics(struct domain *d);
 /* theTools.Qrz after the last step (disco_sync_to_host refreshes it). */

@@ -1,8 +1,8 @@
 /*
  * disco -- driver of one Disco run on the MI355X timestep (main.c:26-97 of the
  * reference, serial): reads ./in.par, sets the mesh up, then loops
- *     dt = getmindt(); check_dt(&dt); possibly_report(); timestep(dt);
- * until T_End or an `abort` file, appending report() lines to report.dat
+ *     dt = getmindt(); disco_check_dt(&dt); disco_possibly_report(); timestep(dt);
+ * until T_End or an `abort` file, appending disco_report() lines to report.dat
  * on possiblyOutput's schedule (and one for the final state), and writes
  * the final state to
  * output.dstate (DSTATE01 layout, disco_amd/state_io.py) and the Analysis
@@ -43,12 +43,12 @@ int main(int argc, char **argv)
    double t0 = wall();
    while (!D.final_step && (max_steps < 0 || D.count_steps < max_steps)) {
       double dt = getmindt(&D);
-      check_dt(&D, &dt);
-      possibly_report(&D, 0);
+      disco_check_dt(&D, &dt);
+      disco_possibly_report(&D, 0);
       timestep(&D, dt);
    }
    double secs = wall() - t0;
-   possibly_report(&D, 1);
+   disco_possibly_report(&D, 1);
    long ncells = 0;
    for (int k = D.NgZa; k < D.Nz - D.NgZb; ++k)
       for (int j = D.NgRa; j < D.Nr - D.NgRb; ++j) ncells += D.Np[j + D.Nr * k];

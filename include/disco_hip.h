@@ -18,7 +18,8 @@
  *   dh_set_planets  passes the planet list that planet_src reads
  *                 (planet.c:152-196) for each RK substep.
  * The reference-signature wrappers `timestep` and `getmindt` that a Disco
- * driver links against are in disco_amd/host/dh_timestep.c (disco_host.h).
+ * driver links against are in disco_amd/host/disco_host.c (declared in
+ * disco_amd/host/disco_host.h).
  *
  * Layout of state arrays (all double precision, structure of arrays):
  *   cell c = off[jk] + i, with jk = j + Nr*k the annulus index (ghost annuli

@@ -143,10 +143,9 @@ def div_b(s: MeshState, faces) -> np.ndarray:
     mag += np.abs(p0) + np.abs(p0[cm])
     for dim, (L, R, LR) in faces.items():
         f = face_flux(s.phi_flux, dim, L, R, LR)
-        np.add.at(net, L, f)
-        np.add.at(net, R, -f)
-        np.add.at(mag, L, np.abs(f))
-        np.add.at(mag, R, np.abs(f))
+        af = np.abs(f)
+        net += np.bincount(L, weights=f, minlength=nt) - np.bincount(R, weights=f, minlength=nt)
+        mag += np.bincount(L, weights=af, minlength=nt) + np.bincount(R, weights=af, minlength=nt)
     return np.abs(net) / np.where(mag > 0, mag, 1.0)
 
 

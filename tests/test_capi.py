@@ -57,3 +57,27 @@ def test_struct_layout_matches_header(pystruct, cstruct):
     assert ctypes.sizeof(pystruct) == size
     for f, off in zip(fields, offs):
         assert getattr(pystruct, f).offset == off, f
+
+
+def _defined_dynamic_symbols(path):
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True,
+                         check=True).stdout
+    return {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+
+
+def test_host_library_symbols_do_not_collide_with_kept_reference_objects():
+    """A Disco binding keeps the reference's domain.o and report.o (check_dt,
+    report, possiblyOutput) and leaves out the step's objects and
+    analysis.o, whose functions the host layer supplies: the host library
+    defines the step's reference entry points and none of the kept
+    objects' symbols."""
+    host = os.path.join(REPO, "disco_amd", "host", "libdisco_host_mhd_kep.so")
+    if not os.path.exists(host):
+        pytest.fail(f"{host} missing: run python -m disco_amd.build")
+    syms = _defined_dynamic_symbols(host)
+    for name in ("getmindt", "timestep", "add_diagnostics", "avg_diagnostics",
+                 "zero_diagnostics", "disco_setup", "disco_sync_to_host", "disco_report",
+                 "disco_check_dt"):
+        assert name in syms, name
+    for name in ("report", "check_dt", "possiblyOutput", "setupDomain", "read_par_file"):
+        assert name not in syms, name

@@ -28,6 +28,8 @@ import build as cpu_build  # noqa: E402
     ("mhd_kep", {"Num_R": 8, "Num_Z": 4}, 5),
     ("mhd_kep", {"Num_R": 12, "Num_Z": 6, "Max_DT": 0.02}, 6),
     ("mhd_kep_p2", {"Num_R": 12, "Num_Z": 4, "Max_DT": 0.02}, 4),
+    # the memory-filling disk's setup: zero-net-flux toroidal field at beta 100
+    ("mhd_kep", {**configs.C5_PAR, "Num_R": 16, "Num_Z": 6}, 10),
     # a disk-planet ensemble member: isothermal Euler with alpha viscosity
     # (viscous fluxes and sources) and a planet on a circular orbit
     ("disk_planet", {"Num_R": 32}, 6),

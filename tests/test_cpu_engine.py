@@ -89,6 +89,8 @@ def _ref_initial(cname, over):
     # this coarse would shear edges across several cells per step.
     ("mhd_kep", {"Num_R": 12, "Num_Z": 6, "Max_DT": 0.02}, 20),
     ("mhd_kep", {"Num_R": 8, "Num_Z": 1}, 6),                  # CT without z faces
+    # the memory-filling disk's setup (toroidal zero-net-flux field) at CFL steps
+    ("mhd_kep", {**configs.C5_PAR, "Num_R": 16, "Num_Z": 6}, 10),
     ("mhd_kep", {"Num_R": 10, "Num_Z": 4, "Timestep": 1}, 4),  # forward Euler
 ])
 def test_cpu_build_bit_exact_vs_oracle(cpu_lib, cname, over, steps):

@@ -80,16 +80,18 @@ def test_gpu_model_matches_host_build(model):
 def test_gpu_learned_guess_and_residual_rule(model, solver, guess, stop):
     """Each of four steps, taken from the reference rule's run's state with
     the learned guess and/or the residual rule, agrees with the reference
-    rule's step: to 1e-13 with HLL (measured: 2e-15); with the relativistic
-    HLLD to 1e-10 over three steps, because its pressure Newton stops at
-    |dp/p| < 1e-12 (hlld_grmhd.c), so a last-bit change of a face's states
-    can change its iteration count and its flux by ~1e-12 (measured on the
-    GPU: 5e-13, 5e-12 and 2e-11 in B_r on the first three steps of this
-    trajectory, growing as the disk evolves; 2e-16 with a one-stage step)."""
+    rule's step.  With HLL to 1e-13 (measured: 2e-15).  With the
+    relativistic HLLD the pressure Newton stops at |dp/p| < 1e-12
+    (hlld_grmhd.c), so a last-bit change of a face's states can change its
+    iteration count and its flux by ~1e-12; the one-step difference grows as
+    the disk evolves (measured on the GPU over this trajectory's steps
+    0..3: 1.4e-14, 4.5e-12, 1.3e-11, 1.08e-10; 2e-16 with a one-stage
+    step).  The bound per step is therefore 1e-10 on steps 0-2 and 1e-9 on
+    step 3, the measured value with a tenfold margin."""
     cfg, par, a = _initial()
     par = {**par, "Riemann_Solver": solver}
-    tol = 1e-13 if solver == 0 else 1e-10
-    nsteps = 4 if solver == 0 else 3
+    tols = [1e-13] * 4 if solver == 0 else [1e-10, 1e-10, 1e-10, 1e-9]
+    nsteps = len(tols)
     print(f"case solver {solver} guess {guess} stop {stop}: {a.n_real()} cells, {nsteps} steps "
           f"from common states", flush=True)
     g = capi.DiscoHip(cfg, par, a)
@@ -108,7 +110,7 @@ def test_gpu_learned_guess_and_residual_rule(model, solver, guess, stop):
         print(f"  step {n}: worst rel L1 {worst:.2e}; iterations/cell {it:.2f} "
               f"(reference rule, previous primitives {rit:.2f}); learned {st['learned']}, "
               f"fallbacks {st['fallbacks']}", flush=True)
-        assert worst < tol
+        assert worst < tols[n], (n, worst, tols[n])
         if guess:
             assert st["learned"] > 0.9 * st["cells"], st
         assert (it >= 11.0) if stop == 0 else (it < 6.0)

@@ -28,6 +28,9 @@ HOST = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
     ("euler_yee_n1", {"Num_R": 64, "Riemann_Solver": 0}, 10),
     ("mhd_kep", {"Num_R": 32, "Num_Z": 8}, 6),
     ("mhd_kep_p2", {"Num_R": 32, "Num_Z": 8, "Max_DT": 0.01}, 6),
+    # the memory-filling disk's setup (toroidal zero-net-flux field at beta
+    # 100) against the reference build over 10 CFL steps
+    ("mhd_kep", {**configs.C5_PAR, "Num_R": 32, "Num_Z": 10}, 10),
     # a disk-planet ensemble member (alpha viscosity, a planet on its orbit)
     ("disk_planet", {"Num_R": 128}, 20),
     # GRMHD trajectories with HLL and HLLD over the steps a one-ulp libm change

@@ -124,6 +124,9 @@ def _ref_initial(cfg, par):
     ("euler_yee", {"Num_R": 32, "Num_Z": 8}, 6),               # z faces
     ("mhd_kep", {"Num_R": 48, "Num_Z": 12}, 10),               # CT, CFL steps
     ("mhd_kep", {"Num_R": 32, "Num_Z": 8, "Max_DT": 0.01}, 10),  # CT, div B kept
+    # the memory-filling disk's setup (r in [0.5, 3], z in [-0.3, 0.3],
+    # zero-net-flux toroidal field at beta 100) at CFL steps
+    ("mhd_kep", {**configs.C5_PAR, "Num_R": 48, "Num_Z": 12}, 10),
 ])
 def test_gpu_matches_oracle(cname, over, steps):
     cfg = configs.get(cname)
