@@ -91,6 +91,15 @@ WORKLOADS = {
         cpu_over={"Num_R": 32, "Num_Z": 8, "R_Min": 8.0, "R_Max": 20.0, "Riemann_Solver": 2,
                   "CFL": 0.2},
         cpu_steps=10),
+    "kerr_torus": dict(
+        config="kerr_torus", over={}, steps=10, warmup=2,
+        desc="3-D GRMHD Fishbone-Moncrief torus around a Kerr black hole (a=0.9, M=1) in "
+             "Kerr-Schild cylindrical coordinates: r in [2,40], z in [-20,20], Nr=192, Nz=96, "
+             "aspect 2.5 (Np up to 512), torus inner edge r=6, pressure maximum r=12, "
+             "gamma 4/3, poloidal loops on density contours at beta=100 plus a uniform "
+             "B^z=1e-4, 1% pressure noise, atmosphere rho 1e-3, P 1e-5; relativistic HLLD, "
+             "PLM 1.5, constrained transport, mesh moving with the gas, RK2, CFL 0.2",
+        cpu_over={"Num_R": 48, "Num_Z": 24}, cpu_steps=4),
 }
 DEFAULT_WORKLOAD = "mhd_kep"
 
@@ -300,7 +309,7 @@ def run_ensemble(args, w, cfg):
     t0 = time.perf_counter()
     runs = []
     for i, (nr, q) in enumerate(members):
-        par = cfg.par_dict(Num_R=nr, Mass_Ratio=q)
+        par = cfg.par_dict(Num_R=nr, Mass_Ratio=q, Autotune=args.autotune)
         runs.append(HostRun(w["config"], state_io.write_par(os.path.join(tmp, str(i)), par)))
     t_setup = time.perf_counter() - t0
     ncell = sum(r.real_cells for r in runs)
@@ -321,6 +330,8 @@ def run_ensemble(args, w, cfg):
             if lib.dh_sync(C.c_void_p(r.ctx)):
                 raise RuntimeError(lib.dh_last_error().decode())
 
+    for _ in range(args.autotune):  # the members' launch tuning (set-up, untimed)
+        step_all()
     for _ in range(warmup):
         step_all()
     sync_all()
@@ -361,65 +372,93 @@ def run_ensemble(args, w, cfg):
     return out
 
 
-# The GRMHD workload of the cons2prim comparison and its steps.
-C2P_COMPARE = dict(workload="gr_disk", steps=10, warmup=2)
-
-
-def timed_host_run(config, par, steps, warmup, graphs=True, label=""):
-    """Steps a host run `warmup` + `steps` times; returns (zone-updates/s,
-    seconds, real cells, c2p stats of the last cons2prim or None)."""
-    import ctypes as C
-    from disco_amd import capi
-    from disco_amd.host_lib import HostRun
-    tmp = tempfile.mkdtemp(prefix="bench_")
-    run = HostRun(config, state_io.write_par(tmp, par))
-    lib = capi.lib()
-    ctx = C.c_void_p(run.ctx)
-    try:
-        if lib.dh_enable_graphs(ctx, 1 if graphs else 0):
-            raise RuntimeError(lib.dh_last_error().decode())
-        for _ in range(warmup):
-            run.timestep(run.getmindt())
-        if lib.dh_sync(ctx):
-            raise RuntimeError(lib.dh_last_error().decode())
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            run.timestep(run.getmindt())
-        if lib.dh_sync(ctx):
-            raise RuntimeError(lib.dh_last_error().decode())
-        secs = time.perf_counter() - t0
-        st = (C.c_longlong * 4)()
-        stats = None
-        if lib.dh_c2p_stats(ctx, st) == 0 and st[0]:
-            stats = {"iterations_per_cell": st[1] / st[0], "learned_fraction": st[2] / st[0],
-                     "fallback_fraction": st[3] / st[0]}
-        ncell = run.real_cells
-    finally:
-        run.close()
-    print(f"bench {label}: {steps} timed steps in {secs:.2f} s", file=sys.stderr, flush=True)
-    return ncell * steps / secs, secs, ncell, stats
+# The GRMHD workload of the cons2prim comparison: its first `harvest` steps
+# (plain guess) give the training data, then `steps` are timed.
+C2P_COMPARE = dict(workload="kerr_torus", harvest=50, every=5, rows=20000, epochs=30,
+                   steps=10)
 
 
 def grmhd_c2p_compare(args):
     """The metric's second half: the GRMHD step with the learned cons2prim
     guess against the plain one (the previous primitives), both stopping
-    when both Newton residuals are below 1e-10 (C2P_Stop 1)."""
-    from disco_amd import c2p_model
-    w = WORKLOADS[C2P_COMPARE["workload"]]
+    when both Newton residuals are below 1e-10 (C2P_Stop 1).  The model is
+    trained online: the plain run's first `harvest` steps give (features,
+    targets) pairs of sampled cells every `every` steps, the engine's own
+    training kernels fit the MLP on them (dh_c2p_train), and the learned run
+    takes the same steps untimed before its timed ones, so both time the
+    same stretch of the torus's evolution."""
+    import ctypes as C
+    import numpy as np
+    from disco_amd import c2p_model, capi
+    from disco_amd.host_lib import HostRun
+    cc = C2P_COMPARE
+    w = WORKLOADS[cc["workload"]]
     cfg = configs.get(w["config"])
     base = cfg.par_dict(**w["over"])
-    base["C2P_Stop"] = 1
-    res = {}
-    for name, over in (("plain", {}),
-                       ("learned", {"C2P_Guess": 1, "C2P_Model": c2p_model.MODEL_PATH})):
-        par = dict(base, **over)
-        v, secs, ncell, stats = timed_host_run(w["config"], par, C2P_COMPARE["steps"],
-                                               C2P_COMPARE["warmup"], label=f"c2p {name}")
-        res[name] = {"value": v, "ms_per_step": secs / C2P_COMPARE["steps"] * 1e3, "c2p": stats}
-    return {"workload": w["desc"], "name": C2P_COMPARE["workload"], "real_cells": ncell,
+    base.update(C2P_Stop=1, Autotune=args.autotune)
+    lib = capi.lib()
+    rng = np.random.default_rng(314159)
+    tmp = tempfile.mkdtemp(prefix="bench_c2p_")
+
+    def timed(run):
+        ctx = C.c_void_p(run.ctx)
+        t0 = time.perf_counter()
+        for _ in range(cc["steps"]):
+            run.timestep(run.getmindt())
+        if lib.dh_sync(ctx):
+            raise RuntimeError(lib.dh_last_error().decode())
+        secs = time.perf_counter() - t0
+        st = (C.c_longlong * 4)()
+        lib.dh_c2p_stats(ctx, st)
+        stats = {"iterations_per_cell": st[1] / max(st[0], 1),
+                 "learned_fraction": st[2] / max(st[0], 1),
+                 "fallback_fraction": st[3] / max(st[0], 1)}
+        return run.real_cells * cc["steps"] / secs, secs, stats
+
+    run = HostRun(w["config"], state_io.write_par(os.path.join(tmp, "plain"), base))
+    ncell = run.real_cells
+    xs, ys = [], []
+    t0 = time.perf_counter()
+    for n in range(cc["harvest"]):
+        run.timestep(run.getmindt())
+        if (n + 1) % cc["every"] == 0:
+            f, t = c2p_model.harvest_ctx(lib, C.c_void_p(run.ctx), run.ntot)
+            if len(f) > cc["rows"]:
+                pick = rng.choice(len(f), cc["rows"], replace=False)
+                f, t = f[pick], t[pick]
+            xs.append(f)
+            ys.append(t)
+    t_harvest = time.perf_counter() - t0
+    v_plain, s_plain, st_plain = timed(run)
+    run.close()
+    X, Y = np.concatenate(xs), np.concatenate(ys)
+    t0 = time.perf_counter()
+    params, loss = c2p_model.fit(X, Y, epochs=cc["epochs"], batch=2048)
+    t_train = time.perf_counter() - t0
+    model = os.path.join(tmp, "c2p_online.f32")
+    c2p_model.save(params, model)
+    print(f"bench c2p: {len(X)} harvested pairs, trained in {t_train:.1f} s (mse {loss:.2e})",
+          file=sys.stderr, flush=True)
+    run = HostRun(w["config"], state_io.write_par(os.path.join(tmp, "learned"),
+                                                  dict(base, C2P_Guess=1, C2P_Model=model)))
+    for _ in range(cc["harvest"]):
+        run.timestep(run.getmindt())
+    v_learned, s_learned, st_learned = timed(run)
+    run.close()
+    print(f"bench c2p: plain {v_plain:.3e}, learned {v_learned:.3e} zone-updates/s",
+          file=sys.stderr, flush=True)
+    return {"workload": w["desc"], "name": cc["workload"], "real_cells": ncell,
             "stop_rule": "both residuals below 1e-10 (C2P_Stop 1)",
-            "plain": res["plain"], "learned": res["learned"],
-            "learned_over_plain": res["learned"]["value"] / res["plain"]["value"]}
+            "training": {"pairs": int(len(X)), "harvest_steps": cc["harvest"],
+                         "harvest_every": cc["every"], "epochs": cc["epochs"],
+                         "mse": loss, "train_s": t_train, "harvest_s": t_harvest,
+                         "kernels": "dh_c2p_train (float32 forward, backward, Adam on the GPU)"},
+            "timed_steps": cc["steps"],
+            "plain": {"value": v_plain, "ms_per_step": s_plain / cc["steps"] * 1e3,
+                      "c2p": st_plain},
+            "learned": {"value": v_learned, "ms_per_step": s_learned / cc["steps"] * 1e3,
+                        "c2p": st_learned},
+            "learned_over_plain": v_learned / v_plain}
 
 
 def main(argv=None):
@@ -436,6 +475,8 @@ def main(argv=None):
                     help="grmhd2 cons2prim's stopping rule")
     ap.add_argument("--graphs", choices=["on", "off"], default="on",
                     help="record the RK2 timestep once as a HIP graph and replay it")
+    ap.add_argument("--autotune", type=int, default=8,
+                    help="steps of launch tuning before the warmup (0: untuned launches)")
     ap.add_argument("--no-c2p-compare", action="store_true",
                     help="skip the GRMHD plain vs learned cons2prim guess comparison")
     args = ap.parse_args(argv)
@@ -462,6 +503,8 @@ def main(argv=None):
         par.update(C2P_Guess=1, C2P_Model=c2p_model.MODEL_PATH)
     if gr and args.c2p_stop == "residual":
         par["C2P_Stop"] = 1
+    tune_steps = args.autotune
+    par["Autotune"] = tune_steps
     tmp = tempfile.mkdtemp(prefix="bench_")
     par_path = state_io.write_par(tmp, par)
     t_setup = time.perf_counter()
@@ -478,6 +521,14 @@ def main(argv=None):
         if lib.dh_sync(ctx):
             raise RuntimeError(lib.dh_last_error().decode())
 
+    # the launch tuner's steps (dh_autotune, in.par Autotune) are part of
+    # the set-up: untimed, before the warmup
+    t_tune = time.perf_counter()
+    for _ in range(tune_steps):
+        run.timestep(run.getmindt())
+    sync()
+    t_tune = time.perf_counter() - t_tune
+    tuning = capi.read_tuning(lib, ctx)
     for _ in range(warmup):
         run.timestep(run.getmindt())
     sync()
@@ -554,7 +605,8 @@ def main(argv=None):
         "data": "synthetic (initial conditions from the workload's Initial plugin, seeded noise)",
         "config": {"workload": w["desc"], "name": args.workload, "graphs": args.graphs == "on", "real_cells": ncell,
                    "cells_with_ghosts": ntot, "r_faces": nfr, "z_faces": nfz,
-                   "setup_s": t_setup},
+                   "setup_s": t_setup, "tuning_steps": tune_steps, "tuning_s": t_tune},
+        "tuning": tuning,
         "roofline": roofline,
         "step_roofline": {"b_alg_bytes_per_zone_update": ba, "achieved": step_ach,
                           "peak": PEAK_HBM_GBS, "unit": "GB/s",

@@ -67,6 +67,10 @@ HOST_CONFIGS = {
     "gr_disk3d": dict(hydro="grmhd2", metric="schw_ks_rpz", initial="gr_disk", planets="none",
                       boundary="fixed_r_outflow_z", analysis="diag_euler",
                       num_c=8, num_n=0, ct_mode=2),
+    # the Fishbone-Moncrief torus around a spinning black hole (Kerr-Schild)
+    "kerr_torus": dict(hydro="grmhd2", metric="kerr_ks_rpz", initial="fm_torus", planets="none",
+                       boundary="fixed_r_outflow_z", analysis="diag_euler",
+                       num_c=8, num_n=0, ct_mode=2),
 }
 
 
@@ -94,7 +98,7 @@ def build_host(configs=None) -> None:
                 os.path.join(plug, "Initial", c["initial"] + ".c"),
                 os.path.join(plug, "Planets", c["planets"] + ".c")]
         common = ["gcc", "-O2", "-fPIC", "-Wall", "-Wno-unused-parameter", *defs,
-                  f"-I{os.path.join(INCLUDE, 'disco')}", f"-I{INCLUDE}", f"-I{host}"]
+                  f"-I{os.path.join(INCLUDE, 'disco')}", f"-I{INCLUDE}", f"-I{host}", f"-I{CSRC}"]
         link = [f"-L{HERE}", "-ldisco_hip", "-lm", "-Wl,-rpath,$ORIGIN/.."]
         _run(common + ["-shared", *srcs, "-o", os.path.join(host, f"libdisco_host_{name}.so"),
                        *link])

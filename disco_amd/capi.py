@@ -43,7 +43,8 @@ _PARAM_DOUBLES = ["gamma_law", "plm", "cfl", "max_dt", "rho_floor", "p_floor",
 class DhParams(C.Structure):
     _fields_ = ([(n, C.c_int) for n in _PARAM_INTS]
                 + [(n, C.c_double) for n in _PARAM_DOUBLES]
-                + [("init_par", C.c_double * 9), ("phi_order", C.c_double)])
+                + [("init_par", C.c_double * 9), ("phi_order", C.c_double),
+                   ("metric_par3", C.c_double)])
 
 
 PLANET_STRIDE = 8
@@ -64,7 +65,7 @@ _PAR_DBL_KEYS = {
     "Adiabatic_Index": "gamma_law", "PLM": "plm", "CFL": "cfl", "Max_DT": "max_dt",
     "Density_Floor": "rho_floor", "Pressure_Floor": "p_floor", "Viscosity": "viscosity",
     "Mach_Number": "mach", "Metric_Par2": "metric_par2", "RotOmega": "rot_omega",
-    "RotD": "rot_d",
+    "RotD": "rot_d", "Metric_Par3": "metric_par3",
 }
 
 
@@ -151,6 +152,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                           C.POINTER(C.c_float)], C.c_int),
         "dh_c2p_grad": ([C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
                          C.c_long, C.POINTER(C.c_float), C.POINTER(C.c_float)], C.c_int),
+        "dh_autotune": ([vp, C.c_int], C.c_int),
+        "dh_read_tuning": ([vp, C.c_int, C.c_void_p, _ip, _ip, _dp, _dp, _ip], C.c_int),
         "dh_last_error": ([], C.c_char_p),
     }
     for name, (args, res) in sig.items():
@@ -166,7 +169,32 @@ EXPORTED_SYMBOLS = ["dh_create", "dh_destroy", "dh_sizes", "dh_upload_state",
                     "dh_read_timers", "dh_add_diagnostics", "dh_get_diagnostics",
                     "dh_zero_diagnostics", "dh_avg_diagnostics", "dh_report", "dh_set_c2p_model",
                     "dh_c2p_harvest", "dh_c2p_predict", "dh_c2p_stats", "dh_c2p_train",
-                    "dh_c2p_grad", "dh_last_error"]
+                    "dh_c2p_grad", "dh_autotune", "dh_read_tuning", "dh_last_error"]
+
+VARIANT_NAMES = {0: "storage order, round-robin XCDs", 1: "storage order, XCD-contiguous"}
+
+
+def read_tuning(lib, ctx):
+    """{kernel: {variant, strip, ms_chosen, ms_default}} of a context's tuner
+    (dh_read_tuning)."""
+    cap = 64
+    names = (C.c_char * 48 * cap)()
+    var = (C.c_int * cap)()
+    strip = (C.c_int * cap)()
+    msc = (C.c_double * cap)()
+    msd = (C.c_double * cap)()
+    n = C.c_int()
+    if lib.dh_read_tuning(ctx, cap, names, var, strip, msc, msd, C.byref(n)):
+        raise DhError(f"dh_read_tuning: {lib.dh_last_error().decode()}")
+    out = {}
+    for i in range(n.value):
+        v = var[i]
+        out[bytes(names[i]).split(b"\0")[0].decode()] = {
+            "variant": v,
+            "order": VARIANT_NAMES.get(v, f"strips of {strip[i]} layers"),
+            "ms_chosen": msc[i], "ms_default": msd[i]}
+    return out
+
 
 _LIB: Optional[C.CDLL] = None
 
@@ -315,6 +343,14 @@ class DiscoHip:
     def enable_graphs(self, on: bool = True):
         """Record an RK2 timestep once as a HIP graph and replay it."""
         self._check(self.lib.dh_enable_graphs(self.ctx, 1 if on else 0))
+
+    def autotune(self, steps: int):
+        """Tune the launch variant of every kernel over the next `steps`
+        timesteps (dh_autotune)."""
+        self._check(self.lib.dh_autotune(self.ctx, steps))
+
+    def read_tuning(self):
+        return read_tuning(self.lib, self.ctx)
 
     def read_timers(self):
         cap = 64

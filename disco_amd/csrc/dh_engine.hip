@@ -449,9 +449,32 @@ struct TimerRow {
    long calls = 0;
 };
 
+// Launch variants of a kernel that runs over cells or faces (run_tuned):
+// 0 storage order with the workgroups dealt to the XCDs round-robin (the
+// hardware's order), 1 storage order with each XCD given a contiguous
+// eighth (k_run_xcd), 2 + s the group order s of the kernel's OrderSet
+// (strips of annuli, XCD-contiguous).  All give the same results.
+struct TuneRow {
+   std::string name;
+   int nvar = 0, next = 0, chosen = -1, dflt = 0;
+   double ms[2 + MAX_ORDERS] = {0};
+   int n[2 + MAX_ORDERS] = {0};
+   int strip[MAX_ORDERS] = {0};
+};
+struct TunePending {
+   int row, var;
+   hipEvent_t e0, e1;
+};
+
 struct HipExec {
    hipStream_t stream = nullptr;
    bool timers = false;
+   // Tuner (dh_autotune): while tune_left timesteps remain, every launch of
+   // a tunable kernel takes its next variant in turn and is timed; then
+   // each kernel keeps its fastest variant.
+   int tune_left = 0;
+   std::vector<TuneRow> tunes;
+   std::vector<TunePending> tpend;
    // Recorded timestep (dh_enable_graphs): replayed while the key it was
    // recorded under (the current face positions' buffer) is unchanged.
    bool graphs = false;
@@ -500,6 +523,78 @@ struct HipExec {
             k_run<K><<<nblocks(n), TPB, 0, stream>>>(k, n);
       }
       LAUNCH_CHECK();
+      return DH_OK;
+   }
+
+   TuneRow &tune_row(const char *name, int nvar, const int *strips, int dflt)
+   {
+      for (auto &t : tunes)
+         if (t.name == name) return t;
+      tunes.emplace_back();
+      TuneRow &t = tunes.back();
+      t.name = name;
+      t.nvar = nvar;
+      t.dflt = dflt;
+      for (int s = 0; s + 2 < nvar; ++s) t.strip[s] = strips[s];
+      return t;
+   }
+
+   // A kernel over cells or faces in one of its launch variants (TuneRow).
+   template <class K>
+   int run_tuned(const char *name, const K &k, long n, const GroupOrder *ords, int nord,
+                 const int *strips)
+   {
+      if (n <= 0) return DH_OK;
+      int row = (int)(&tune_row(name, 2 + nord, strips, XcdContiguous<K>::value ? 1 : 0) -
+                      tunes.data());
+      TuneRow &t = tunes[row];
+      int v = t.chosen >= 0 ? t.chosen : t.dflt;
+      bool timing = tune_left > 0;
+      if (timing) v = t.next++ % t.nvar;
+      hipEvent_t e0{}, e1{};
+      if (timing) {
+         HIPCHK(hipEventCreate(&e0));
+         HIPCHK(hipEventCreate(&e1));
+         HIPCHK(hipEventRecord(e0, stream));
+      }
+      {
+         Bracket b(*this, name);
+         if (v == 0)
+            k_run<K><<<nblocks(n), TPB, 0, stream>>>(k, n);
+         else if (v == 1)
+            k_run_xcd<K><<<nblocks_xcd(n), TPB, 0, stream>>>(k, n);
+         else
+            k_run_xcd<Grouped<K>><<<nblocks_xcd(n), TPB, 0, stream>>>(Grouped<K>{k, ords[v - 2]}, n);
+      }
+      LAUNCH_CHECK();
+      if (timing) {
+         HIPCHK(hipEventRecord(e1, stream));
+         tpend.push_back({row, v, e0, e1});
+      }
+      return DH_OK;
+   }
+
+   // After a tuned timestep: the launch times of its variants; on the last
+   // tuned step, every kernel's choice.
+   int tune_collect()
+   {
+      HIPCHK(hipStreamSynchronize(stream));
+      for (auto &p : tpend) {
+         float ms = 0.0f;
+         HIPCHK(hipEventElapsedTime(&ms, p.e0, p.e1));
+         tunes[p.row].ms[p.var] += ms;
+         tunes[p.row].n[p.var] += 1;
+         (void)hipEventDestroy(p.e0);
+         (void)hipEventDestroy(p.e1);
+      }
+      tpend.clear();
+      if (--tune_left == 0)
+         for (auto &t : tunes) {
+            int best = -1;
+            for (int v = 0; v < t.nvar; ++v)
+               if (t.n[v] && (best < 0 || t.ms[v] / t.n[v] < t.ms[best] / t.n[best])) best = v;
+            if (best >= 0) t.chosen = best;
+         }
       return DH_OK;
    }
 
@@ -674,6 +769,10 @@ struct HipExec {
    template <class F>
    int replayable(const void *key, F &&body)
    {
+      if (tune_left > 0) {
+         DH_TRY(body());
+         return tune_collect();
+      }
       if (!graphs || timers || !key) return body();
       if (!gexec || gkey != key) {
          drop_graph();
@@ -714,6 +813,10 @@ static void release(dh_ctx *c)
    if (!c) return;
    if (c->x.stream) (void)hipStreamSynchronize(c->x.stream);
    c->x.drop_graph();
+   for (auto &p : c->x.tpend) {
+      (void)hipEventDestroy(p.e0);
+      (void)hipEventDestroy(p.e1);
+   }
    for (auto &row : c->x.rows)
       for (auto &pe : row.pending) {
          (void)hipEventDestroy(pe.first);
@@ -980,6 +1083,41 @@ int dh_enable_graphs(dh_ctx *c, int on)
       HIPCHK(hipStreamSynchronize(c->x.stream));
       c->x.drop_graph();
    }
+   return DH_OK;
+}
+
+int dh_autotune(dh_ctx *c, int steps)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   if (steps < 0) return fail(DH_EINVAL, "negative step count");
+   c->x.drop_graph();
+   c->x.tune_left = steps;
+   if (steps > 0)
+      for (auto &t : c->x.tunes) {
+         t.next = 0;
+         for (int v = 0; v < t.nvar; ++v) t.ms[v] = 0.0, t.n[v] = 0;
+      }
+   return DH_OK;
+}
+
+int dh_read_tuning(dh_ctx *c, int cap, char (*names)[48], int *variant, int *strip,
+                   double *ms_chosen, double *ms_default, int *nrows)
+{
+   if (!c || cap < 0 || !nrows) return fail(DH_EINVAL, "bad arguments");
+   int n = 0;
+   for (auto &t : c->x.tunes) {
+      if (n >= cap) break;
+      int v = t.chosen >= 0 ? t.chosen : t.dflt;
+      if (names) {
+         snprintf(names[n], 48, "%s", t.name.c_str());
+      }
+      if (variant) variant[n] = v;
+      if (strip) strip[n] = v >= 2 ? t.strip[v - 2] : 0;
+      if (ms_chosen) ms_chosen[n] = t.n[v] ? t.ms[v] / t.n[v] : -1.0;
+      if (ms_default) ms_default[n] = t.n[t.dflt] ? t.ms[t.dflt] / t.n[t.dflt] : -1.0;
+      ++n;
+   }
+   *nrows = n;
    return DH_OK;
 }
 

@@ -19,6 +19,8 @@
 #pragma once
 #include "dh_geom.h"
 #include "dh_grhlld.h"
+#define KKS_FN DHD
+#include "kerr_ks.h"
 
 namespace dh {
 
@@ -32,8 +34,9 @@ struct GRMetric {
    double al, be[3], gam[9], igam[9], jac, sqrtgam, U[4];
 };
 
-// The z at which the metric module evaluates a position.
-DHD double gr_z(const dh_params &P, double z) { return P.metric == DH_METRIC_SCHW_KS ? z : 0.0; }
+// The z at which the metric module evaluates a position (the equatorial
+// module evaluates every position at z = 0).
+DHD double gr_z(const dh_params &P, double z) { return P.metric == DH_METRIC_SCHW_KS_EQ ? 0.0 : z; }
 
 DHD double gr_lapse(double M, double r, double z)
 {
@@ -171,19 +174,63 @@ DHD void gr_frame_der_U(double M, double r, double z, double *dU)
 }
 
 // The metric at a position (r, z); z is the reference's x[2], which the
-// equatorial module ignores.
+// equatorial module ignores.  The Kerr metric (DH_METRIC_KERR_KS,
+// kerr_ks.h) has the same metric_jacobian (= r) and the same normal frame.
 DHD GRMetric gr_metric(const dh_params &P, double r, double z_pos)
 {
    double M = P.metric_par2, z = gr_z(P, z_pos);
    GRMetric g;
-   g.al = gr_lapse(M, r, z);
-   gr_shift(M, r, z, g.be);
-   gr_gam(M, r, z, g.gam);
-   gr_igam(M, r, z, g.igam);
+   if (P.metric == DH_METRIC_KERR_KS) {
+      kks_point p = kks_eval(M, P.metric_par3, r, z, 0);
+      g.al = kks_lapse(&p);
+      kks_shift(&p, g.be);
+      kks_gam(&p, g.gam);
+      kks_igam(&p, g.igam);
+      double ia = 1.0 / g.al;
+      g.U[0] = ia;
+      for (int i = 0; i < 3; ++i) g.U[i + 1] = -ia * g.be[i];
+   } else {
+      g.al = gr_lapse(M, r, z);
+      gr_shift(M, r, z, g.be);
+      gr_gam(M, r, z, g.gam);
+      gr_igam(M, r, z, g.igam);
+      gr_frame_U(M, r, z, g.U);
+   }
    g.jac = r / r;  // metric_jacobian (= r) over r
    g.sqrtgam = g.jac / g.al;
-   gr_frame_U(M, r, z, g.U);
    return g;
+}
+
+// d g_{mu nu} / d x^i of the metric module at (r, z) (z as gr_z gives it).
+DHD void gr_metric_der_g(const dh_params &P, double r, double z, int i, double *dg)
+{
+   if (P.metric == DH_METRIC_KERR_KS) {
+      kks_point p = kks_eval(P.metric_par2, P.metric_par3, r, z, 1);
+      kks_der_g(&p, i, dg);
+   } else {
+      gr_der_g(P.metric_par2, r, z, i, dg);
+   }
+}
+
+// frame_der_U (normal.c:23-42) of the metric module at (r, z).
+DHD void gr_metric_frame_der_U(const dh_params &P, double r, double z, double *dU)
+{
+   if (P.metric != DH_METRIC_KERR_KS) {
+      gr_frame_der_U(P.metric_par2, r, z, dU);
+      return;
+   }
+   kks_point p = kks_eval(P.metric_par2, P.metric_par3, r, z, 1);
+   double da[4], b[3], db[12];
+   double ia = 1.0 / kks_lapse(&p);
+   kks_der_lapse(&p, da);
+   kks_shift(&p, b);
+   kks_der_shift(&p, db);
+   for (int i = 0; i < 16; ++i) dU[i] = 0.0;
+   for (int i = 0; i < 4; ++i) {
+      if (gr_killing(i)) continue;
+      dU[4 * i] = -ia * ia * da[i];
+      for (int j = 0; j < 3; ++j) dU[4 * i + j + 1] = b[j] * ia * ia * da[i] - db[3 * i + j] * ia;
+   }
 }
 
 // The fluid of a primitive state in a metric: the quantities prim2cons,
@@ -799,7 +846,7 @@ struct GRMHD {
       double al = g.al, jac = g.jac, sqrtgam = g.sqrtgam;
       const double *be = g.be, *gam = g.gam, *igam = g.igam, *U = g.U;
       double dU[16];
-      gr_frame_der_U(M, r, z, dU);
+      gr_metric_frame_der_U(P, r, z, dU);
       double rho = prim[RHO], Pp = prim[PPP];
       double l[4] = {0.0, prim[URR], prim[UPP], prim[UZZ]};
       double B[3] = {prim[BRR] / sqrtgam, prim[BPP] / (r * sqrtgam), prim[BZZ] / sqrtgam};
@@ -842,7 +889,7 @@ struct GRMHD {
          Sk[i] = 0.0;
          if (gr_killing(i + 1)) continue;
          double dg[16];
-         gr_der_g(M, r, z, i + 1, dg);
+         gr_metric_der_g(P, r, z, i + 1, dg);
          for (int mu = 0; mu < 4; ++mu)
             for (int nu = 0; nu < 4; ++nu)
                Sk[i] += (rhoh * u[mu] * u[nu] * rcorr * pcorr + ig[4 * mu + nu] * Pt -

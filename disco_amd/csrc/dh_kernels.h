@@ -57,17 +57,35 @@ struct IsGR<GRMHD<NQ>> {
    static constexpr bool value = true;
 };
 
-// A kernel run in another order: thread t does element order[t].  Every
-// element's work is independent of when it runs (owner-computes, no
-// atomics), so only memory locality changes; the order is a permutation of
-// the kernel's elements, and its workgroups go to the XCDs in contiguous
-// runs so that neighbouring elements of the order share an L2.
+// Traversal orders of a kernel's elements.  The elements come in groups
+// that are contiguous in storage (the cells of an annulus, the faces of an
+// annulus pair), and an order visits whole groups in some sequence: thread
+// t does element elem(t).  Every element's work is independent of when it
+// runs (owner-computes, no atomics), so an order changes only which
+// elements share an L2 at a time, never a result.
+//   first[p]  storage index of the first element of the p-th visited group
+//   pos[p]    thread index at which that group starts (pos[ng] = n)
+//   chunk[c]  the p whose range holds thread ORDER_CHUNK * c, so that a
+//             thread finds its group after a step or two from there
+constexpr int ORDER_CHUNK = 256;
+struct GroupOrder {
+   const int *first, *pos, *chunk;
+   int ng;
+   DHD long elem(long t) const
+   {
+      int p = chunk[t / ORDER_CHUNK];
+      while (pos[p + 1] <= t) ++p;
+      return first[p] + (t - pos[p]);
+   }
+};
+
+// A kernel run in a group order (its workgroups go to the XCDs in
+// contiguous runs, so that neighbouring groups of the order share an L2).
 template <class K>
-struct Ordered {
+struct Grouped {
    K k;
-   const int *order;
-   static constexpr bool xcd_contiguous = true;
-   DHD void operator()(long t) const { k(order[t]); }
+   GroupOrder o;
+   DHD void operator()(long t) const { k(o.elem(t)); }
 };
 
 // One direction's face list: faces of annulus pair P are fI[P] .. fI[P+1]-1.

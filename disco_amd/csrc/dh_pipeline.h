@@ -41,6 +41,25 @@ inline int fail(int code, const char *fmt, ...)
       if (rc_) return rc_;                                                    \
    } while (0)
 
+// Traversal orders of the four element kinds the kernels run over: the
+// cells (groups: annuli), the r faces and the z faces (groups: annulus
+// pairs), and both face lists as one range (the edge kernels; an r pair's
+// faces, then the z pair's of the same (j, k)).  Order s of a kind visits
+// its groups (j, k) in strips of strip[s] layers of k: strip after strip,
+// j after j within a strip, k fastest within j.  Layer k + 1 of an
+// annulus is Nr annuli away in storage and j + 1 only one, so in storage
+// order the z neighbours of a gather are far apart in time and in the
+// strip orders both neighbours are close; the strip height trades the
+// annuli between the two.  With one layer every strip order is the storage
+// order, and a set has none.
+enum { ORD_CELLS = 0, ORD_FACES_R = 1, ORD_FACES_Z = 2, ORD_EDGES = 3, ORD_KINDS = 4 };
+constexpr int MAX_ORDERS = 4;
+struct OrderSet {
+   GroupOrder o[MAX_ORDERS];
+   int strip[MAX_ORDERS];
+   int n = 0;
+};
+
 struct Buffers {
    dh_params P;
    int nq = 0, nf = 0;
@@ -66,12 +85,8 @@ struct Buffers {
    int *lbase[2] = {nullptr, nullptr};  // first segment of the edge line whose first list is pair JK
    double *seg[2] = {nullptr, nullptr}; // E dl dt of every segment of every edge line
    int *bcstart = nullptr;          // where each ghost cell's overlap sum starts
-   // Traversal order of the cell kernels that gather from the next annulus
-   // in z (Nr annuli away in memory): the annuli taken z-fastest, so that
-   // neighbouring workgroups share those annuli in L2 (measured: grad_z
-   // 6% and faces_z 8% faster; the face-ordered z kernels were slower in
-   // that order and keep their own)
-   int *zord_c = nullptr;
+   // Traversal orders the executor may launch a kernel in (OrderSet)
+   OrderSet ord[ORD_KINDS];
    int bc_maxnp[4][8] = {{0}};      // largest Np of each side's ghost layers
    PlanetSet pl[2];
    StepArgs *step = nullptr;  // device copy of {dt, planets} of the current timestep
@@ -132,8 +147,12 @@ inline int validate(const dh_grid *g, const dh_params &P, long *ntot_out)
          return fail(DH_EINVAL, "grmhd2 needs ghost annuli on every side");
       if (P.mesh_motion == 3 || P.energy_omega)
          return fail(DH_EINVAL, "grmhd2 takes Mesh_Motion 0, 1 or 4 and no energy frame");
-      if (P.metric != DH_METRIC_SCHW_KS_EQ && P.metric != DH_METRIC_SCHW_KS)
-         return fail(DH_EINVAL, "grmhd2 runs with the schw_ks_eq_rpz or schw_ks_rpz metric");
+      if (P.metric != DH_METRIC_SCHW_KS_EQ && P.metric != DH_METRIC_SCHW_KS &&
+          P.metric != DH_METRIC_KERR_KS)
+         return fail(DH_EINVAL, "grmhd2 runs with the schw_ks_eq_rpz, schw_ks_rpz or kerr_ks_rpz metric");
+      if (P.metric == DH_METRIC_KERR_KS && !(fabs(P.metric_par3) < P.metric_par2))
+         return fail(DH_EINVAL, "kerr_ks_rpz needs a spin below the mass (|a| = %g, M = %g)",
+                     P.metric_par3, P.metric_par2);
    } else {
       return fail(DH_EINVAL, "hydro module %d not supported", P.hydro);
    }
@@ -176,30 +195,96 @@ int alloc_n(X &x, void *pp, size_t bytes)
    return x.alloc((void **)pp, bytes);
 }
 
-// The z-major traversal order of the cells (Buffers::zord_c).
-template <class X>
-int build_orders(X &x, Buffers &b, const dh_grid *g)
+// The strip heights of the orders of a kind whose groups have K layers:
+// 4, 8 and 16 layers where they are fewer than K, and all K (k fastest
+// over the whole column).
+inline std::vector<int> strip_heights(int K)
 {
-   int Nr = g->Nr, Nz = g->Nz;
-   if (Nz < 2) return DH_OK;
-   std::vector<int> oc;
-   oc.reserve((size_t)b.m.ntot);
-   for (int j = 0; j < Nr; ++j)
-      for (int k = 0; k < Nz; ++k) {
-         int a = j + Nr * k;
-         for (int c = b.h_off[a]; c < b.h_off[a + 1]; ++c) oc.push_back(c);
-      }
-   DH_TRY(alloc_n(x, &b.zord_c, sizeof(int) * oc.size()));
-   DH_TRY(x.put(b.zord_c, oc.data(), sizeof(int) * oc.size()));
+   std::vector<int> out;
+   if (K < 2) return out;
+   for (int S : {4, 8, 16})
+      if (S < K) out.push_back(S);
+   out.push_back(K);
+   return out;
+}
+
+// One group order on the device from the host lists of the visited groups'
+// storage starts and sizes.
+template <class X>
+int put_order(X &x, GroupOrder &o, const std::vector<int> &first, const std::vector<int> &size)
+{
+   int ng = (int)first.size();
+   std::vector<int> pos(ng + 1, 0);
+   for (int p = 0; p < ng; ++p) pos[p + 1] = pos[p] + size[p];
+   long n = pos[ng];
+   std::vector<int> chunk((size_t)(n / ORDER_CHUNK + 1), 0);
+   for (int p = 0, c = 0; c < (int)chunk.size(); ++c) {
+      long t = (long)c * ORDER_CHUNK;
+      while (p < ng - 1 && pos[p + 1] <= t) ++p;
+      chunk[c] = p;
+   }
+   int *d[3] = {nullptr, nullptr, nullptr};
+   DH_TRY(alloc_n(x, &d[0], sizeof(int) * (size_t)(ng > 0 ? ng : 1)));
+   DH_TRY(alloc_n(x, &d[1], sizeof(int) * (size_t)(ng + 1)));
+   DH_TRY(alloc_n(x, &d[2], sizeof(int) * chunk.size()));
+   if (ng > 0) DH_TRY(x.put(d[0], first.data(), sizeof(int) * (size_t)ng));
+   DH_TRY(x.put(d[1], pos.data(), sizeof(int) * (size_t)(ng + 1)));
+   DH_TRY(x.put(d[2], chunk.data(), sizeof(int) * chunk.size()));
+   o.first = d[0];
+   o.pos = d[1];
+   o.chunk = d[2];
+   o.ng = ng;
    return DH_OK;
 }
 
-// Runs k over its n elements, in the given order when there is one.
-template <class X, class K>
-int run_in(X &x, const char *name, const K &k, long n, const int *order)
+// The strip orders of every kind (OrderSet).  fI holds the host copies of
+// both face lists' pair offsets.
+template <class X>
+int build_orders(X &x, Buffers &b, const std::vector<int> *fI)
 {
-   if (order) return x.run(name, Ordered<K>{k, order}, n);
-   return x.run(name, k, n);
+   const Mesh &m = b.m;
+   int Nr = m.Nr, Nz = m.Nz;
+   for (int S : strip_heights(Nz)) {
+      // (j, k) of the cells' groups in this strip order
+      std::vector<std::pair<int, int>> seq;
+      for (int k0 = 0; k0 < Nz; k0 += S)
+         for (int j = 0; j < Nr; ++j)
+            for (int k = k0; k < k0 + S && k < Nz; ++k) seq.push_back({j, k});
+      std::vector<int> first[ORD_KINDS], size[ORD_KINDS];
+      int nfr = fI[0].empty() ? 0 : fI[0].back();
+      for (auto jk : seq) {
+         int j = jk.first, k = jk.second;
+         int a = j + Nr * k;
+         first[ORD_CELLS].push_back(b.h_off[a]);
+         size[ORD_CELLS].push_back(b.h_Np[a]);
+         for (int d = 0; d < 2; ++d) {
+            int J = d == 0 ? Nr - 1 : Nr, K = d == 0 ? Nz : Nz - 1;
+            if (j >= J || k >= K || fI[d].empty()) continue;
+            int P = j + J * k;
+            int f0 = fI[d][P], nf = fI[d][P + 1] - fI[d][P];
+            first[ORD_FACES_R + d].push_back(f0);
+            size[ORD_FACES_R + d].push_back(nf);
+            first[ORD_EDGES].push_back(d == 0 ? f0 : nfr + f0);
+            size[ORD_EDGES].push_back(nf);
+         }
+      }
+      for (int kind = 0; kind < ORD_KINDS; ++kind) {
+         OrderSet &os = b.ord[kind];
+         if (os.n >= MAX_ORDERS) return fail(DH_EINVAL, "too many traversal orders");
+         DH_TRY(put_order(x, os.o[os.n], first[kind], size[kind]));
+         os.strip[os.n] = S;
+         ++os.n;
+      }
+   }
+   return DH_OK;
+}
+
+// Runs k over its n elements; the executor may take any order of the set
+// (results do not depend on it).
+template <class X, class K>
+int run_ord(X &x, const char *name, const K &k, long n, const OrderSet &os)
+{
+   return x.run_tuned(name, k, n, os.o, os.n, os.strip);
 }
 
 // Allocates and fills every buffer of a validated mesh.
@@ -249,6 +334,7 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
 
    // Face lists: the pair offsets are fixed because Np never changes
    // (buildfaces mode 0, faces_alt.c:80-96).
+   std::vector<int> h_fI[2];
    for (int dim = 1; dim <= 2; ++dim) {
       int d = dim - 1;
       Faces &F = b.F[d];
@@ -267,6 +353,7 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
          }
       if (n >= (1L << 31)) return fail(DH_EINVAL, "too many faces (%ld)", n);
       fI[Nrmax * Nzmax] = (int)n;
+      h_fI[d] = fI;
       F.npairs = Nrmax * Nzmax;
       F.n = (int)n;
       int *dfI = nullptr;
@@ -313,7 +400,7 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
          DH_TRY(x.zero(b.emf_f[d], sizeof(double) * 3 * n));
       }
    }
-   DH_TRY(build_orders(x, b, g));
+   DH_TRY(build_orders(x, b, h_fI));
    size_t qb = sizeof(double) * (size_t)ntot * nq;
    size_t cb = sizeof(double) * (size_t)ntot;
    DH_TRY(alloc_n(x, &b.prim, qb));
@@ -407,12 +494,13 @@ struct Engine {
          double *grad = dim == 1 ? b.gradr : b.gradz;
          bool slopes = grads && b.P.plm != 0.0;
          if (!reuse && dim == 1) {
-            DH_TRY(x.run("ranges_r", KCellRanges<1>{m, b.piph, b.rng[d]}, m.ntot));
-            DH_TRY(x.run("faces_r", KFaces<1>{m, b.F[d], b.rng[d], b.piph, b.dphi}, m.ntot));
+            DH_TRY(run_ord(x, "ranges_r", KCellRanges<1>{m, b.piph, b.rng[d]}, m.ntot, b.ord[ORD_CELLS]));
+            DH_TRY(run_ord(x, "faces_r", KFaces<1>{m, b.F[d], b.rng[d], b.piph, b.dphi}, m.ntot,
+                           b.ord[ORD_CELLS]));
          } else if (!reuse) {
-            DH_TRY(x.run("ranges_z", KCellRanges<2>{m, b.piph, b.rng[d]}, m.ntot));
-            DH_TRY(run_in(x, "faces_z", KFaces<2>{m, b.F[d], b.rng[d], b.piph, b.dphi}, m.ntot,
-                          b.zord_c));
+            DH_TRY(run_ord(x, "ranges_z", KCellRanges<2>{m, b.piph, b.rng[d]}, m.ntot, b.ord[ORD_CELLS]));
+            DH_TRY(run_ord(x, "faces_z", KFaces<2>{m, b.F[d], b.rng[d], b.piph, b.dphi}, m.ntot,
+                           b.ord[ORD_CELLS]));
          }
          if (!grads) continue;
          if (!slopes) {
@@ -422,11 +510,11 @@ struct Engine {
          if (dim == 1) {
             KGrad<NQ, 1> kg{m, b.P.plm, b.P.num_c, b.F[d], b.rng[d], b.piph, b.dphi, b.prim,
                             b.gradp, grad};
-            DH_TRY(x.run("grad_r", kg, m.ntot));
+            DH_TRY(run_ord(x, "grad_r", kg, m.ntot, b.ord[ORD_CELLS]));
          } else {
             KGrad<NQ, 2> kg{m, b.P.plm, b.P.num_c, b.F[d], b.rng[d], b.piph, b.dphi, b.prim,
                             b.gradp, grad};
-            DH_TRY(run_in(x, "grad_z", kg, m.ntot, b.zord_c));
+            DH_TRY(run_ord(x, "grad_z", kg, m.ntot, b.ord[ORD_CELLS]));
          }
       }
       return DH_OK;
@@ -466,72 +554,83 @@ struct Engine {
       const Mesh &m = b.m;
       const dh_params &P = b.P;
       if (first_step) {
-         DH_TRY(x.run("wcell", KWcell<NQ, H>{m, P, b.prim, b.wiph}, m.ntot));
+         DH_TRY(run_ord(x, "wcell", KWcell<NQ, H>{m, P, b.prim, b.wiph}, m.ntot, b.ord[ORD_CELLS]));
          if (P.mesh_motion == 3) DH_TRY(x.run("wcell", KWcellAvg{m, b.wiph}, m.nann));
       }
-      DH_TRY(x.run("plm_phi", KPlmPhi<NQ>{m, P.plm, b.prim, b.dphi, b.gradp}, m.ntot));
+      DH_TRY(run_ord(x, "plm_phi", KPlmPhi<NQ>{m, P.plm, b.prim, b.dphi, b.gradp}, m.ntot,
+                     b.ord[ORD_CELLS]));
       DH_TRY(topology<NQ>(true, reuse));
       KFluxPhi<NQ, H> kp{m, P, dt, b.prim, b.gradp, b.gradr, b.gradz, b.piph, b.dphi, b.wiph,
                          b.dFp, b.emf_phi};
-      DH_TRY(x.run("flux_phi", kp, m.ntot));
+      DH_TRY(run_ord(x, "flux_phi", kp, m.ntot, b.ord[ORD_CELLS]));
       for (int dim = 1; dim <= 2; ++dim) {
          int d = dim - 1;
          if (!dim_active(m, dim)) continue;
          if (dim == 1) {
             KFluxTrans<NQ, H, 1> kt{m, P, dt, b.F[d], b.prim, b.gradp, b.gradr, b.gradz,
                                     b.piph, b.dphi, b.dF[d], b.emf_f[d]};
-            DH_TRY(x.run("flux_r", kt, b.F[d].n));
+            DH_TRY(run_ord(x, "flux_r", kt, b.F[d].n, b.ord[ORD_FACES_R]));
          } else {
             KFluxTrans<NQ, H, 2> kt{m, P, dt, b.F[d], b.prim, b.gradp, b.gradr, b.gradz,
                                     b.piph, b.dphi, b.dF[d], b.emf_f[d]};
-            DH_TRY(x.run("flux_z", kt, b.F[d].n));
+            DH_TRY(run_ord(x, "flux_z", kt, b.F[d].n, b.ord[ORD_FACES_Z]));
          }
       }
       if (mhd) {
          int ct = P.ct;
          KEdgesCollect kc{m, ct, b.dphi, b.emf_phi, b.F[0], b.F[1], b.rng[0], b.rng[1],
                           b.emf_f[0], b.emf_f[1], b.Ecarry, b.Epre};
-         DH_TRY(x.run("edges_collect", kc, m.ntot));
+         DH_TRY(run_ord(x, "edges_collect", kc, m.ntot, b.ord[ORD_CELLS]));
          for (int dim = 1; dim <= 2; ++dim) {
             int d = dim - 1;
             if (!dim_active(m, dim)) continue;
             if (dim == 1)
-               DH_TRY(x.run("face_emf_r", KFaceEmf<1>{m, b.F[d], b.piph, b.dphi, b.Epre, b.fE[d], b.fB[d]}, b.F[d].n));
+               DH_TRY(run_ord(x, "face_emf_r",
+                              KFaceEmf<1>{m, b.F[d], b.piph, b.dphi, b.Epre, b.fE[d], b.fB[d]},
+                              b.F[d].n, b.ord[ORD_FACES_R]));
             else
-               DH_TRY(x.run("face_emf_z", KFaceEmf<2>{m, b.F[d], b.piph, b.dphi, b.Epre, b.fE[d], b.fB[d]}, b.F[d].n));
+               DH_TRY(run_ord(x, "face_emf_z",
+                              KFaceEmf<2>{m, b.F[d], b.piph, b.dphi, b.Epre, b.fE[d], b.fB[d]},
+                              b.F[d].n, b.ord[ORD_FACES_Z]));
          }
          EdgeArgs ea{m, b.F[0], b.F[1], b.rng[0], b.rng[1], b.Epre, b.fE[0], b.fB[0], b.fE[1], b.fB[1]};
          KEdgesFinal kef{ea, ct, b.wiph, b.emf_f[0], b.emf_f[1], b.Eo, b.Eown, b.Ecarry};
-         DH_TRY(x.run("edges_final", kef, m.ntot));
+         DH_TRY(run_ord(x, "edges_final", kef, m.ntot, b.ord[ORD_CELLS]));
          KUpdateB kb{m, RK, dt, b.F[0], b.F[1], b.rng[0], b.rng[1], b.Eo, b.Phi, b.rkPhi};
-         DH_TRY(x.run("update_B", kb, m.ntot));
+         DH_TRY(run_ord(x, "update_B", kb, m.ntot, b.ord[ORD_CELLS]));
          if (dim_active(m, 2)) {
             if (dim_active(m, 1) && !reuse)
-               DH_TRY(x.run("line_steps_r", KLineSteps<1>{m, b.F[0], b.estep[0]}, b.F[0].n));
+               DH_TRY(run_ord(x, "line_steps_r", KLineSteps<1>{m, b.F[0], b.estep[0]}, b.F[0].n,
+                              b.ord[ORD_FACES_R]));
             if (!reuse)
-               DH_TRY(x.run("line_steps_z", KLineSteps<2>{m, b.F[1], b.estep[1]}, b.F[1].n));
+               DH_TRY(run_ord(x, "line_steps_z", KLineSteps<2>{m, b.F[1], b.estep[1]}, b.F[1].n,
+                              b.ord[ORD_FACES_Z]));
             KEdgeSegments ks{m, dt, b.F[0], b.F[1], b.estep[0], b.estep[1], b.lbase[0], b.lbase[1],
                              b.piph, b.dphi, b.Eown, b.seg[0], b.seg[1]};
-            DH_TRY(x.run("edge_segments", ks, (long)b.F[0].n + b.F[1].n));
+            DH_TRY(run_ord(x, "edge_segments", ks, (long)b.F[0].n + b.F[1].n, b.ord[ORD_EDGES]));
             KEdgeAdjust ke{m, b.F[0], b.F[1], b.estep[0], b.estep[1], b.lbase[0], b.lbase[1],
                            b.seg[0], b.seg[1], b.Phi};
-            DH_TRY(x.run("edge_adjust", ke, (long)b.F[0].n + b.F[1].n));
+            DH_TRY(run_ord(x, "edge_adjust", ke, (long)b.F[0].n + b.F[1].n, b.ord[ORD_EDGES]));
          }
       }
       KUpdate<NQ, H> ku{m, P, RK, dt, first_step, &b.step->pl[sub], b.prim, b.prim, b.cons,
                         b.rkcons, b.gradr, b.gradp, b.gradz, b.piph, b.dphi, b.wiph,
                         b.piph2, b.dphi2, b.dFp, b.dF[0], b.dF[1], b.F[0].fI, b.F[1].fI,
                         b.rng[0], b.rng[1], b.F[0].n, b.F[1].n};
-      DH_TRY(x.run("update", ku, m.ntot));
+      DH_TRY(run_ord(x, "update", ku, m.ntot, b.ord[ORD_CELLS]));
       if (mhd) {
          if (first_step) {
             for (int dim = 1; dim <= 2; ++dim) {
                int d = dim - 1;
                if (!dim_active(m, dim)) continue;
                if (dim == 1)
-                  DH_TRY(x.run("flip_flags_r", KFlipFlags<1>{m, b.F[d], b.piph, b.wiph, dt, b.flag[d]}, b.F[d].n));
+                  DH_TRY(run_ord(x, "flip_flags_r",
+                                 KFlipFlags<1>{m, b.F[d], b.piph, b.wiph, dt, b.flag[d]},
+                                 b.F[d].n, b.ord[ORD_FACES_R]));
                else
-                  DH_TRY(x.run("flip_flags_z", KFlipFlags<2>{m, b.F[d], b.piph, b.wiph, dt, b.flag[d]}, b.F[d].n));
+                  DH_TRY(run_ord(x, "flip_flags_z",
+                                 KFlipFlags<2>{m, b.F[d], b.piph, b.wiph, dt, b.flag[d]},
+                                 b.F[d].n, b.ord[ORD_FACES_Z]));
                DH_TRY(x.flips(m, dim, b.F[d], b.flag[d], b.Phi, b.rkPhi));
             }
          }
@@ -540,13 +639,13 @@ struct Engine {
          bool learned = IsGR<H>::value && P.c2p_guess == DH_C2P_GUESS_LEARNED;
          KBfacesC2p<NQ, H> kbc{m, P, b.F[0], b.F[1], b.rng[0], b.rng[1], b.piph2, b.dphi2,
                                b.Phi, b.cons, b.prim, 1, learned ? 0 : 1, b.c2p_info};
-         DH_TRY(x.run("bfaces_c2p", kbc, m.ntot));
+         DH_TRY(run_ord(x, "bfaces_c2p", kbc, m.ntot, b.ord[ORD_CELLS]));
          if constexpr (IsGR<H>::value) {
             if (learned) {
                DH_TRY(x.template c2p_guess<NQ>(m, P, b.cons, b.piph2, b.dphi2, b.c2p_prm,
                                                b.c2p_frag, b.c2p_guess));
                KC2p<NQ> kc{m, P, b.piph2, b.dphi2, b.cons, b.prim, b.c2p_guess, b.c2p_info};
-               DH_TRY(x.run("c2p", kc, m.ntot));
+               DH_TRY(run_ord(x, "c2p", kc, m.ntot, b.ord[ORD_CELLS]));
             }
          }
       }

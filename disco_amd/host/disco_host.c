@@ -46,6 +46,11 @@ int planet_motion_analytic(void);
 void initializePlanets(struct planet *);
 void movePlanets(struct planet *, double, double);
 
+/* Timesteps of launch tuning at the start of a run: enough for every
+ * kernel (launched once or twice a step) to run each of its up to six
+ * variants at least once. */
+#define DISCO_AUTOTUNE_STEPS 8
+
 struct disco_accel {
    dh_ctx *ctx;
    long ntot;
@@ -59,6 +64,10 @@ struct disco_accel {
    /* in.par Graphs: 0 launches every kernel of a timestep eagerly; otherwise
     * (the default) the RK2 step is recorded once and replayed */
    int graphs;
+   /* in.par Autotune: the number of first timesteps over which every
+    * kernel's launch variants are timed (dh_autotune); 0 keeps the untuned
+    * variants */
+   int autotune;
    /* possiblyOutput's report counter (domain.c:103, 316-335) */
    int nrpt;
 };
@@ -69,6 +78,7 @@ static struct disco_accel *accel_of(struct domain *d)
       struct disco_accel *A = calloc(1, sizeof(struct disco_accel));
       if (A) {
          A->graphs = 1;
+         A->autotune = DISCO_AUTOTUNE_STEPS;
          A->nrpt = -1;
       }
       d->accel = A;
@@ -261,6 +271,7 @@ int disco_read_par(struct domain *d, const char *file)
    par_lookup(file, "C2P_Stop", V_INT, &A->c2p_stop);
    par_lookup_str(file, "C2P_Model", A->c2p_model, sizeof A->c2p_model);
    par_lookup(file, "Graphs", V_INT, &A->graphs);
+   par_lookup(file, "Autotune", V_INT, &A->autotune);
    if (t2pi) { L->t_min *= 2. * M_PI; L->t_max *= 2. * M_PI; }
    if (p2pi) L->phimax *= 2. * M_PI;
    if (zpi) { L->zmin *= M_PI; L->zmax *= M_PI; }
@@ -389,6 +400,7 @@ static void fill_params(struct domain *d, dh_params *P, const int *bc)
    P->metric = !strcmp(METRIC, "flat_rpz")         ? DH_METRIC_FLAT
                : !strcmp(METRIC, "schw_ks_eq_rpz") ? DH_METRIC_SCHW_KS_EQ
                : !strcmp(METRIC, "schw_ks_rpz")    ? DH_METRIC_SCHW_KS
+               : !strcmp(METRIC, "kerr_ks_rpz")    ? DH_METRIC_KERR_KS
                                                    : -1; /* refused at dh_create */
    const struct disco_accel *A = (const struct disco_accel *)d->accel;
    P->c2p_guess = A ? A->c2p_guess : DH_C2P_GUESS_PREVIOUS;
@@ -425,6 +437,7 @@ static void fill_params(struct domain *d, dh_params *P, const int *bc)
    P->viscosity = L->viscosity;
    P->mach = L->Disk_Mach;
    P->metric_par2 = L->metricPar2;
+   P->metric_par3 = L->metricPar3;
    P->rot_omega = L->RotOmega;
    P->rot_d = L->RotD;
    double ip[9] = {(double)L->initPar0, L->initPar1, L->initPar2, L->initPar3, L->initPar4,
@@ -642,6 +655,8 @@ int disco_setup(struct domain *d)
    if (P.c2p_guess == DH_C2P_GUESS_LEARNED && load_c2p_model(A)) return 1;
    if (dh_enable_graphs(A->ctx, A->graphs != 0))
       return host_fail("dh_enable_graphs: %s", dh_last_error());
+   if (dh_autotune(A->ctx, A->autotune > 0 ? A->autotune : 0))
+      return host_fail("dh_autotune: %s", dh_last_error());
    long nt = A->ntot;
    double *prim = (double *)malloc(sizeof(double) * nt * NUM_Q);
    double *piph = (double *)malloc(sizeof(double) * nt);

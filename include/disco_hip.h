@@ -51,8 +51,8 @@ enum {
 enum {
    DH_HYDRO_EULER = 0,  /* Hydro/euler.c */
    DH_HYDRO_MHD = 1,    /* Hydro/mhd.c */
-   DH_HYDRO_GRMHD = 2   /* Hydro/grmhd2.c with Frame/normal.c and a DH_METRIC_SCHW_* metric
-                           (M = metric_par2) */
+   DH_HYDRO_GRMHD = 2   /* Hydro/grmhd2.c with Frame/normal.c and a DH_METRIC_SCHW_* or
+                           DH_METRIC_KERR_KS metric (M = metric_par2) */
 };
 
 /* Metric modules (reference Hydro/Metric/<name>.c). */
@@ -60,7 +60,11 @@ enum {
    DH_METRIC_FLAT = 0,     /* flat_rpz.c (euler, mhd) */
    DH_METRIC_SCHW_KS_EQ = 1, /* schw_ks_eq_rpz.c: Schwarzschild in Kerr-Schild coordinates
                               evaluated on the equator (z = 0) */
-   DH_METRIC_SCHW_KS = 2   /* schw_ks_rpz.c: the same metric at (r, z) */
+   DH_METRIC_SCHW_KS = 2,  /* schw_ks_rpz.c: the same metric at (r, z) */
+   DH_METRIC_KERR_KS = 3   /* the Kerr metric in Kerr-Schild form at (r, z), mass
+                              metric_par2 and spin metric_par3 (no reference
+                              module; disco_amd/plugins/Metric/kerr_ks_rpz.c is
+                              the same metric written against Hydro/metric.h) */
 };
 
 /* The initial guess and stopping rule of grmhd2's cons2prim (noble2d,
@@ -131,6 +135,7 @@ typedef struct dh_params {
    double phi_order;            /* PHI_ORDER of the softened point-mass force,
                                    2.0 (planet.c:4); a run-time value as there,
                                    so pow() is evaluated as the reference does */
+   double metric_par3;          /* Metric_Par3 (the spin a of DH_METRIC_KERR_KS) */
 } dh_params;
 
 /* Planet record passed to dh_set_planets: 8 doubles per planet,
@@ -266,6 +271,25 @@ int dh_read_timers(dh_ctx *ctx, int cap, char (*names)[48], double *ms,
  * calls onestep twice); it removes the per-kernel launch cost of small
  * meshes.  The host build accepts and ignores it. */
 int dh_enable_graphs(dh_ctx *ctx, int on);
+
+/* Launch tuner (no counterpart in the reference, whose loops run in one
+ * order on one core).  Every kernel of the step that runs over cells or
+ * faces can be launched in several traversal orders: storage order with the
+ * workgroups dealt to the XCDs round-robin (variant 0) or in contiguous
+ * eighths (1), or whole annuli (annulus pairs) taken in strips of S layers
+ * of z, the strips one after another, r after r within a strip and z
+ * fastest (2, 3, ...; S = 4, 8, 16 or all layers).  Results are the same
+ * in every order.  dh_autotune makes the next `steps` timesteps launch each
+ * such kernel in its variants in turn, timed with events (eagerly, never
+ * recorded), after which each kernel keeps its fastest; steps = 0 keeps
+ * the choices made so far.  Until a kernel is tuned it runs in variant 0,
+ * or 1 for the kernels that gather from the next annulus in r.
+ * dh_read_tuning reports up to `cap` kernels: the chosen variant, its
+ * strip height (0 for variants 0 and 1), and the mean launch time of the
+ * chosen and of the untuned variant over the tuning steps (-1 if none). */
+int dh_autotune(dh_ctx *ctx, int steps);
+int dh_read_tuning(dh_ctx *ctx, int cap, char (*names)[48], int *variant, int *strip,
+                   double *ms_chosen, double *ms_default, int *nrows);
 
 const char *dh_last_error(void);
 

@@ -2,6 +2,7 @@
 compiled as host code (hipcc, host pass only, no FMA contraction)."""
 from __future__ import annotations
 
+import glob
 import os
 import subprocess
 import sys
@@ -10,9 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 OUT = os.path.join(HERE, "_build", "libdh_cpu.so")
 SRC = os.path.join(HERE, "dh_cpu.cpp")
-DEPS = [SRC] + [os.path.join(REPO, "disco_amd", "csrc", f) for f in
-                ("dh_kernels.h", "dh_pipeline.h", "dh_ct.h", "dh_mesh.h", "dh_geom.h",
-                 "dh_euler.h", "dh_mhd.h")] + [os.path.join(REPO, "include", "disco_hip.h")]
+DEPS = ([SRC] + sorted(glob.glob(os.path.join(REPO, "disco_amd", "csrc", "*.h")))
+        + [os.path.join(REPO, "include", "disco_hip.h")])
 
 
 def build(force: bool = False) -> str:
@@ -52,7 +52,8 @@ def build_driver(config: str) -> str:
         return out
     cmd = ["gcc", "-O2", "-Wall", "-Wno-unused-parameter", *host_defines(c),
            f"-I{os.path.join(REPO, 'include', 'disco')}",
-           f"-I{os.path.join(REPO, 'include')}", f"-I{host}", *srcs, "-o", out + ".tmp",
+           f"-I{os.path.join(REPO, 'include')}", f"-I{host}",
+           f"-I{os.path.join(REPO, 'disco_amd', 'csrc')}", *srcs, "-o", out + ".tmp",
            f"-L{os.path.dirname(lib)}", "-ldh_cpu", "-lm",
            f"-Wl,-rpath,{os.path.dirname(lib)}"]
     subprocess.run(cmd, check=True)

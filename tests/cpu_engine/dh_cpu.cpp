@@ -30,6 +30,23 @@ struct HostExec {
       for (long t = 0; t < n; ++t) k(t);
       return DH_OK;
    }
+   // A kernel with launch variants (the GPU tuner's numbering: 0 and 1 the
+   // storage order, 2 + s group order s).  DH_CPU_ORDER = v runs every such
+   // kernel in variant v where the kernel has it (strip orders visit every
+   // element once, in another sequence: the results must not change).
+   template <class K>
+   int run_tuned(const char *, const K &k, long n, const GroupOrder *ords, int nord, const int *)
+   {
+      const char *e = getenv("DH_CPU_ORDER");
+      int v = e ? atoi(e) : 0;
+      if (v >= 2 && v - 2 < nord) {
+         Grouped<K> g{k, ords[v - 2]};
+         for (long t = 0; t < n; ++t) g(t);
+         return DH_OK;
+      }
+      for (long t = 0; t < n; ++t) k(t);
+      return DH_OK;
+   }
    int ring_start(const Mesh &m, const double *piph, int *I0)
    {
       for (int a = 0; a < m.nann; ++a) I0[a] = ring_start_of(m, piph, a);
@@ -247,6 +264,31 @@ int dhx_riemann(dh_ctx *c, const double *pL, const double *pR, double r, const d
 }
 
 
+// Every group order of every kind visits each element of its range exactly
+// once: returns the number of orders checked, or -1 at the first that does
+// not.
+extern "C" long dhx_check_orders(dh_ctx *c)
+{
+   if (!c) return -1;
+   Buffers &b = c->b;
+   long nkind[ORD_KINDS] = {b.m.ntot, b.F[0].n, b.F[1].n, (long)b.F[0].n + b.F[1].n};
+   long checked = 0;
+   for (int kind = 0; kind < ORD_KINDS; ++kind)
+      for (int s = 0; s < b.ord[kind].n; ++s) {
+         const GroupOrder &o = b.ord[kind].o[s];
+         long n = nkind[kind];
+         if (o.pos[o.ng] != n) return -1;
+         std::vector<char> seen((size_t)n, 0);
+         for (long t = 0; t < n; ++t) {
+            long e = o.elem(t);
+            if (e < 0 || e >= n || seen[(size_t)e]) return -1;
+            seen[(size_t)e] = 1;
+         }
+         ++checked;
+      }
+   return checked;
+}
+
 extern "C" long dhx_flip_count(dh_ctx *c) { return c ? c->x.flipped : -1; }
 extern "C" long dhx_bc_rings(dh_ctx *c) { return c ? c->x.bc_rings : -1; }
 extern "C" long dhx_bc_misses(dh_ctx *c) { return c ? c->x.bc_misses : -1; }
@@ -441,6 +483,13 @@ int dh_c2p_stats(dh_ctx *c, long long *out)
 
 int dh_enable_timers(dh_ctx *c, int) { return c ? DH_OK : fail(DH_EINVAL, "null context"); }
 int dh_enable_graphs(dh_ctx *c, int) { return c ? DH_OK : fail(DH_EINVAL, "null context"); }
+int dh_autotune(dh_ctx *c, int) { return c ? DH_OK : fail(DH_EINVAL, "null context"); }
+int dh_read_tuning(dh_ctx *c, int, char (*)[48], int *, int *, double *, double *, int *nrows)
+{
+   if (!c || !nrows) return fail(DH_EINVAL, "bad arguments");
+   *nrows = 0;
+   return DH_OK;
+}
 
 int dh_report(dh_ctx *c, const double *planets, int npl, double *out)
 {

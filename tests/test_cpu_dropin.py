@@ -54,6 +54,11 @@ import build as cpu_build  # noqa: E402
     ("gr_disk3d", {"Num_R": 16, "Num_Z": 8, "Riemann_Solver": 2, "CFL": 0.3}, 6),
     ("gr_disk3d", {"Num_R": 16, "Num_Z": 8, "Riemann_Solver": 2, "Z_Min": 1.0, "Z_Max": 3.0,
                    "Mesh_Motion": 1}, 5),
+    # the Fishbone-Moncrief torus around a Kerr black hole (a = 0.9, the
+    # kerr_ks_rpz metric), relativistic HLLD, mesh moving with the gas; and
+    # with HLL and a static mesh
+    ("kerr_torus", {"Num_R": 32, "Num_Z": 16}, 4),
+    ("kerr_torus", {"Num_R": 24, "Num_Z": 12, "Riemann_Solver": 0, "Mesh_Motion": 0}, 4),
 ])
 def test_driver_bit_exact_vs_reference(cname, over, steps):
     cfg = configs.get(cname)

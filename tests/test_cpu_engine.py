@@ -225,3 +225,27 @@ def test_no_kernel_reads_memory_nothing_wrote(cname, over):
             os.environ.pop("DH_CPU_POISON", None)
     for key in ("prim", "cons", "piph", "phi_flux"):
         assert np.array_equal(getattr(outs[0], key), getattr(outs[1], key), equal_nan=True), key
+
+
+@pytest.mark.parametrize("order", [2, 3, 4])
+def test_strip_orders_are_permutations_and_change_nothing(cpu_lib, monkeypatch, order):
+    """The strip traversal orders the GPU tuner may pick visit every cell
+    and face exactly once, and a run with every kernel in one of them is
+    bit-identical with the oracle (owner-computes: the order of the work
+    changes nothing)."""
+    cpu_lib.dhx_check_orders.argtypes = [C.c_void_p]
+    cpu_lib.dhx_check_orders.restype = C.c_long
+    cfg, par, a = _ref_initial("mhd_kep", {"Num_R": 12, "Num_Z": 6, "Max_DT": 0.02})
+    monkeypatch.setenv("DH_CPU_ORDER", str(order))
+    g = capi.DiscoHip(cfg, par, a, library=cpu_lib)
+    # 6 real layers and 2 ghost layers on each side: strips of 4, 8 and all
+    # 10 layers, for cells, r faces, z faces and edges
+    assert cpu_lib.dhx_check_orders(g.ctx) == 12
+    o = Oracle(cfg, par, a)
+    for n in range(4):
+        dt = o.getmindt()
+        assert g.getmindt() == dt, f"step {n}: CFL dt"
+        g.timestep(dt)
+        o.timestep(dt)
+    _assert_same(g.download(a), o.get_state(), f"order {order}")
+    g.close()

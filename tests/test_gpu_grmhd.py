@@ -38,6 +38,8 @@ CASES = [
     # round-off (tests/test_conditioning.py), so it is compared over five
     ("gr_disk3d", {"Num_R": 48, "Num_Z": 12, "Riemann_Solver": 0}),
     ("gr_disk3d", {"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 2, "Z_Min": 1.0, "Z_Max": 3.0}, 5),
+    # the Fishbone-Moncrief torus in the Kerr metric (a = 0.9), relativistic HLLD
+    ("kerr_torus", {"Num_R": 48, "Num_Z": 24}),
 ]
 
 
