@@ -97,7 +97,7 @@ WORKLOADS = {
              "Kerr-Schild cylindrical coordinates: r in [2,40], z in [-20,20], Nr=192, Nz=96, "
              "aspect 2.5 (Np up to 512), torus inner edge r=6, pressure maximum r=12, "
              "gamma 4/3, poloidal loops on density contours at beta=100 plus a uniform "
-             "B^z=1e-4, 1% pressure noise, atmosphere rho 1e-3, P 1e-5; relativistic HLLD, "
+             "B^z=1e-4, 1% pressure noise, atmosphere rho 1e-2, P 1e-5; relativistic HLLD, "
              "PLM 1.5, constrained transport, mesh moving with the gas, RK2, CFL 0.2",
         cpu_over={"Num_R": 48, "Num_Z": 24}, cpu_steps=4),
 }

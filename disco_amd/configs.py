@@ -140,19 +140,20 @@ CONFIGS["gr_disk3d"] = dataclasses.replace(CONFIGS["gr_disk"], name="gr_disk3d",
 # the density contours at beta = 100 with divergence-free face fluxes plus a
 # weak uniform vertical field (B^z = 1e-4: the reference's relativistic HLLD
 # fails and leaves NaN where the field vanishes identically), 1% pressure
-# noise, an atmosphere at rest in the normal frame (rho 1e-3, P 1e-5) with
+# noise, an atmosphere at rest in the normal frame (rho 1e-2, P 1e-5) with
 # floors below it; r in [2, 40], z in [-20, 20], Nr = 192, Nz = 96, aspect
 # 2.5 (at most 512 cells per annulus, 4.9M cells), relativistic HLLD, PLM
 # 1.5, constrained transport, the mesh moving with the gas (Mesh_Motion 1),
-# CFL 0.2 (at 0.25 and above the reference's own run of the torus's thin
-# inner edge fails within 24 steps at half this resolution).
+# CFL 0.2.  With a thinner atmosphere (rho 1e-3 or less) the torus's edge
+# cells empty to the floors within a few steps, where the reference's own
+# run branches on round-off (and with rho 1e-4 fails).
 CONFIGS["kerr_torus"] = _cfg(
     "kerr_torus", "grmhd2", "fm_torus", "fixed_r_outflow_z", "none", 8, 0, 2,
     Num_R=192, Num_Z=96, R_Min=2.0, R_Max=40.0, Z_Min=-20.0, Z_Max=20.0, aspect=2.5,
     NoBC_Rmin=0, NoBC_Rmax=0, NoBC_Zmin=0, NoBC_Zmax=0, CFL=0.2, PLM=1.5,
     Riemann_Solver=2, Mesh_Motion=1, Energy_Omega=0, Constrained_Transport=1, Isothermal=0,
     Metric_Par2=1.0, Metric_Par3=0.9, Adiabatic_Index=4.0 / 3.0, Init_Par1=6.0,
-    Init_Par2=12.0, Init_Par3=100.0, Init_Par4=0.2, Init_Par5=1e-3, Init_Par6=1e-5,
+    Init_Par2=12.0, Init_Par3=100.0, Init_Par4=0.2, Init_Par5=1e-2, Init_Par6=1e-5,
     Init_Par7=1e-4, Density_Floor=1e-6, Pressure_Floor=1e-8, Noise_Type=2, Noise_Rel=1e-2,
     Noise_Abs=0.0)
 CONFIGS["kerr_torus"] = dataclasses.replace(CONFIGS["kerr_torus"], metric="kerr_ks_rpz")

@@ -44,7 +44,7 @@ HOST = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
     ("gr_disk3d", {"Num_R": 32, "Num_Z": 8, "Riemann_Solver": 0}, 3),
     # the Fishbone-Moncrief torus around a Kerr black hole (a = 0.9) with the
     # relativistic HLLD, and with HLL on a static mesh
-    ("kerr_torus", {"Num_R": 32, "Num_Z": 16}, 3),
+    ("kerr_torus", {"Num_R": 32, "Num_Z": 16}, 6),
     ("kerr_torus", {"Num_R": 32, "Num_Z": 16, "Riemann_Solver": 0, "Mesh_Motion": 0}, 6),
 ])
 def test_disco_driver_matches_reference(cname, over, steps):
