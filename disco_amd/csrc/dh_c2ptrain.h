@@ -30,7 +30,7 @@ struct MlpBatch {
    const float *prm;  // parameter block
    const float *X;    // features [n][8]
    const float *Y;    // targets [n][2]
-   const int *idx;    // rows of this batch: sample indices
+   const int *idx;    // rows of this batch: sample indices (null: rows 0..nb-1)
    int nb;            // rows in the batch
    float *act;        // per row: H1[64] H2[64] D1[64] D2[64] D3[2]
    float *loss;       // per row: squared error summed over the outputs
@@ -41,7 +41,7 @@ struct KMlpRow {
    DHD void operator()(long b) const
    {
       const float *p = B.prm;
-      int i = B.idx[b];
+      int i = B.idx ? B.idx[b] : (int)b;
       float x[C2PNET_IN], h1[C2PNET_H], h2[C2PNET_H];
       for (int k = 0; k < C2PNET_IN; ++k)
          x[k] = (B.X[(size_t)i * C2PNET_IN + k] - p[C2PNET_IN_MU + k]) / p[C2PNET_IN_SD + k];
@@ -130,7 +130,7 @@ struct KMlpGrad {
          if (in < 0) {
             s += d;
          } else if (aof < 0) {
-            int i = B.idx[b];
+            int i = B.idx ? B.idx[b] : b;
             float x = (B.X[(size_t)i * C2PNET_IN + in] - p[C2PNET_IN_MU + in]) / p[C2PNET_IN_SD + in];
             s += x * d;
          } else {
@@ -153,6 +153,31 @@ struct KAdam {
       m[t] = mt;
       v[t] = vt;
       prm[C2PNET_TRAIN0 + t] -= lr * (mt / c1) / (sqrtf(vt / c2) + eps);
+   }
+};
+
+// The entries of a gradient block outside the trainable range (the
+// normalisation constants), cleared: thread q < C2PNET_TRAIN0 takes entry q,
+// the others the entries after the trainable range.
+struct KGradFrame {
+   float *grad;  // [C2PNET_NPARAM]
+   DHD void operator()(long q) const
+   {
+      grad[q < C2PNET_TRAIN0 ? q : q + C2PNET_NTRAIN] = 0.0f;
+   }
+};
+
+// The mean loss of n rows, L = sum of the rows' squared errors / (2 n), summed
+// in double in row order (one thread).
+struct KLossMean {
+   const float *loss;
+   long n;
+   float *out;
+   DHD void operator()(long) const
+   {
+      double s = 0.0;
+      for (long b = 0; b < n; ++b) s += loss[b];
+      *out = (float)(s / (2.0 * n));
    }
 };
 

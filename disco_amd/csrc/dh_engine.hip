@@ -7,6 +7,7 @@
 // and per-kernel event timers.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -853,6 +854,41 @@ static int do_faces(dh_ctx *c)
 }
 
 // ---------------------------------------------------------------------------
+// The learned guess's loss gradient without allocation (dh_c2p_grad_noalloc).
+// The per-row activations and losses of dh_c2p_grad live in `scratch`, one
+// file-scope device array: DH_C2P_NOALLOC_MAX_ROWS rows of C2PNET_ROW
+// activations, then one loss per row.  Host code reaches it through the
+// address hipGetSymbolAddress gives on each device (each GPU holds its own
+// copy), looked up once per device with that device current.  Every row the
+// kernels read is written first in the same call, so nothing needs clearing.
+// ---------------------------------------------------------------------------
+constexpr long C2P_NOALLOC_ROWS = DH_C2P_NOALLOC_MAX_ROWS;
+constexpr size_t C2P_SCRATCH_FLOATS = (size_t)C2P_NOALLOC_ROWS * (C2PNET_ROW + 1);
+static __device__ __align__(256) float scratch[C2P_SCRATCH_FLOATS];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static float *g_scratch_ptr[SCRATCH_MAX_DEVICES];
+
+// The device address of `scratch` on the current device.
+static int scratch_of_current_device(float **out)
+{
+   int dev = -1;
+   HIPCHK(hipGetDevice(&dev));
+   if (dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+      return fail(DH_EINVAL, "device %d out of range for the scratch table", dev);
+   std::lock_guard<std::mutex> lock(g_scratch_mu);
+   if (!g_scratch_ptr[dev]) {
+      void *p = nullptr;
+      HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      if (!p) return fail(DH_EHIP, "hipGetSymbolAddress(scratch) gave a null address");
+      g_scratch_ptr[dev] = (float *)p;
+   }
+   *out = g_scratch_ptr[dev];
+   return DH_OK;
+}
+
+// ---------------------------------------------------------------------------
 // C ABI (include/disco_hip.h)
 // ---------------------------------------------------------------------------
 
@@ -1054,6 +1090,30 @@ int dh_c2p_grad(const float *params, const float *X, const float *Y, long n, flo
       GradArgs *t = (GradArgs *)p;
       return c2p_grad(c->x, t->params, t->X, t->Y, t->n, t->grad, t->loss);
    }, &a);
+}
+
+int dh_c2p_grad_noalloc(const float *params, const float *X, const float *Y, long n, float *grad,
+                        float *loss, void *stream)
+{
+   if (!params || !X || !Y || !grad || !loss) return fail(DH_EINVAL, "null argument");
+   if (n < 1 || n > C2P_NOALLOC_ROWS)
+      return fail(DH_EINVAL, "dh_c2p_grad_noalloc takes 1 to %ld rows (got %ld)",
+                  C2P_NOALLOC_ROWS, n);
+   float *s = nullptr;
+   DH_TRY(scratch_of_current_device(&s));
+   hipStream_t st = (hipStream_t)stream;
+   MlpBatch B{params, X, Y, nullptr, (int)n, s, s + (size_t)n * C2PNET_ROW};
+   k_run<KMlpRow><<<nblocks(n), TPB, 0, st>>>(KMlpRow{B}, n);
+   LAUNCH_CHECK();
+   k_run<KMlpGrad><<<nblocks(C2PNET_NTRAIN), TPB, 0, st>>>(KMlpGrad{B, grad + C2PNET_TRAIN0},
+                                                           C2PNET_NTRAIN);
+   LAUNCH_CHECK();
+   long nframe = C2PNET_NPARAM - C2PNET_NTRAIN;
+   k_run<KGradFrame><<<nblocks(nframe), TPB, 0, st>>>(KGradFrame{grad}, nframe);
+   LAUNCH_CHECK();
+   k_run<KLossMean><<<1, TPB, 0, st>>>(KLossMean{B.loss, n, loss}, 1);
+   LAUNCH_CHECK();
+   return DH_OK;
 }
 
 int dh_c2p_stats(dh_ctx *c, long long *out)

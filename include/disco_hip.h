@@ -256,6 +256,20 @@ int dh_c2p_train(const float *X, const float *Y, long n, int epochs, int batch, 
 int dh_c2p_grad(const float *params, const float *X, const float *Y, long n, float *grad,
                 float *loss);
 
+/* dh_c2p_grad without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc); no counterpart in the reference.
+ * params[4886], X[n][8], Y[n][2], grad[4886] and loss[1] are DEVICE
+ * pointers; the kernels are queued on `stream` (a hipStream_t; NULL is the
+ * default stream) and the call returns without waiting for them.  n is at
+ * most DH_C2P_NOALLOC_MAX_ROWS: the per-row activations live in one fixed
+ * device array per GPU (4.2 MB, held until the process ends), shared by
+ * every call on that GPU, so calls that overlap on one GPU race.  The
+ * results equal dh_c2p_grad's bit for bit. */
+#define DH_C2P_NOALLOC_MAX_ROWS 4096
+int dh_c2p_grad_noalloc(const float *params, const float *X, const float *Y, long n,
+                        float *grad, float *loss, void *stream);
+
 /* Average device time (ms) of the kernels of the last `dh_timestep` calls,
  * accumulated per kernel family since dh_reset_timers; used by the benchmark
  * to report the dominant kernel.  names/ms/calls receive up to `cap` rows. */

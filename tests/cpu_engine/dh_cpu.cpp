@@ -484,6 +484,11 @@ int dh_c2p_stats(dh_ctx *c, long long *out)
 int dh_enable_timers(dh_ctx *c, int) { return c ? DH_OK : fail(DH_EINVAL, "null context"); }
 int dh_enable_graphs(dh_ctx *c, int) { return c ? DH_OK : fail(DH_EINVAL, "null context"); }
 int dh_autotune(dh_ctx *c, int) { return c ? DH_OK : fail(DH_EINVAL, "null context"); }
+// The host build has no device array to work in.
+int dh_c2p_grad_noalloc(const float *, const float *, const float *, long, float *, float *, void *)
+{
+   return fail(DH_ENODEV, "dh_c2p_grad_noalloc needs the GPU library");
+}
 int dh_read_tuning(dh_ctx *c, int, char (*)[48], int *, int *, double *, double *, int *nrows)
 {
    if (!c || !nrows) return fail(DH_EINVAL, "bad arguments");

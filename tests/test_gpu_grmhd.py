@@ -27,6 +27,12 @@ import build as cpu_build  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 STEP_TOL = 1e-12
+# The relativistic HLLD's total-pressure Newton stops at |dp / p| < 1e-12
+# (hlld_grmhd.c:1092-1171), so a last-bit difference in a face's states (the
+# GPU contracts multiply-adds into FMAs) can end a face's solve one
+# iteration earlier or later and move its flux, and a cell's field, by
+# ~1e-12 (measured: 1.6e-12 in Phi at the sixth step of the 48 x 12 disk).
+HLLD_STEP_TOL = 1e-11
 
 
 CASES = [
@@ -56,6 +62,7 @@ def test_gpu_grmhd_steps_match_host_build(cname, over, nsteps):
     a = state_io.read_state(os.path.join(d, "state_0000.bin"))
     print(f"case {cname} {over}: {a.n_real()} cells, {nsteps} steps from common states",
           flush=True)
+    tol = HLLD_STEP_TOL if par.get("Riemann_Solver") == 2 else STEP_TOL
     host = capi.load_library(cpu_build.build())
     g = capi.DiscoHip(cfg, par, a)
     worst = 0.0
@@ -76,7 +83,7 @@ def test_gpu_grmhd_steps_match_host_build(cname, over, nsteps):
         k = max(e, key=e.get)
         worst = max(worst, hydro)
         print(f"  step {n}: hydro {hydro:.2e}, worst {e[k]:.2e} ({k}), Phi {ephi:.2e}", flush=True)
-        assert hydro <= STEP_TOL, (n, hydro)
-        assert e[k] <= STEP_TOL and ephi <= STEP_TOL, (n, k, e[k], ephi)
+        assert hydro <= tol, (n, hydro)
+        assert e[k] <= tol and ephi <= tol, (n, k, e[k], ephi)
         g.timestep(dts[0])
     g.close()
