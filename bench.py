@@ -74,7 +74,7 @@ WORKLOADS = {
     "disk_ensemble": dict(
         config="disk_planet", over={}, steps=200, warmup=2, members=64, seed=314159,
         desc="64 independent 2-D isothermal disk-planet Euler runs, each its own mesh, dt and "
-             "stream: r in [0.4,2.5], Nr drawn from {64,128,256,512} by seed, "
+             "planet, stepped together: r in [0.4,2.5], Nr drawn from {64,128,256,512} by seed, "
              "Np=int(2 pi r/dr), Sigma=r^-0.5, h=0.05, alpha viscosity 1e-3, a planet of mass "
              "ratio q drawn log-uniform in [1e-5,1e-3] on a circular orbit at r=1 softened by "
              "0.6h; HLLC, PLM 1.5, RK2, CFL 0.5, Keplerian mesh motion",
@@ -247,6 +247,143 @@ def cpu_baseline(cfg, over, steps, cores=None):
                       f"same workload at {size} ({results[0]['cells']} cells each) with the "
                       f"reference timestep built from its sources (gcc -O3, USE_MPI=0); "
                       f"{secs:.1f} s of stepping; value is their summed zone-updates/s"}
+
+
+def dominant_roofline(kernels, ba, ncell, workload):
+    """Roofline of the kernel with the most device time per step.  Its
+    algorithmic bytes per launch are the survey's per-zone-update figure
+    B_alg taken over one substep (B_alg / 2: the RK2 step reads and writes
+    the state {cons, Phi, piph} twice) times the real cells a launch covers;
+    the kernel's own byte model (every array it touches once) is reported
+    beside it as kernel_model_GBps."""
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    row = kernels[dom]
+    mean_s = row["mean_launch_ms"] * 1e-3
+    alg = ba / 2 * ncell
+    ach = alg / mean_s / 1e9 if mean_s > 0 else 0.0
+    return {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
+            "traffic": measured_traffic(dom, workload),
+            "algorithmic_bytes_per_launch": alg,
+            "algorithmic_bytes_rule": f"B_alg/2 = {ba / 2:g} B per real cell per substep x "
+                                      f"{ncell} cells",
+            "mean_launch_ms": row["mean_launch_ms"],
+            "kernel_model_bytes_per_launch": row["model_bytes_per_launch"],
+            "kernel_model_GBps": row["model_GBps"],
+            "share_of_step": row["ms_per_step"] / sum(v["ms_per_step"] for v in kernels.values())}
+
+
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
+    summary (profiles/traffic_<workload>.json), or None."""
+    path = os.path.join(REPO, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    row = table.get("kernels", {}).get(kernel)
+    return row.get("hbm_bytes_per_launch") if row else None
+
+
+def ensemble_members(w):
+    """(Num_R, Mass_Ratio) of every member: Nr uniform in {64, 128, 256, 512}
+    and q log-uniform in [1e-5, 1e-3], from the workload's seed."""
+    import numpy as np
+    rng = np.random.default_rng(w["seed"])
+    return [(int(rng.choice([64, 128, 256, 512])), float(10.0 ** rng.uniform(-5.0, -3.0)))
+            for _ in range(w["members"])]
+
+
+def run_ensemble(args, w, cfg):
+    """Independent runs stepped together.  Packed (the default): one
+    ensemble (dh_ens_*) whose timestep launches each kernel once for all
+    members, with one CFL reduction over all of them and one copy of their
+    dts per step, recorded as one HIP graph.  Streams: each member its own
+    context, stream and recorded step, its dt fetched one by one.
+    value = all members' zone-updates per second."""
+    import ctypes as C
+    from disco_amd import capi
+    from disco_amd.host_lib import Ensemble, HostRun
+    steps = args.steps if args.steps is not None else w["steps"]
+    warmup = args.warmup if args.warmup is not None else w["warmup"]
+    members = ensemble_members(w)
+    packed = args.ensemble_mode == "packed"
+    tmp = tempfile.mkdtemp(prefix="bench_ens_")
+    t0 = time.perf_counter()
+    runs = []
+    for i, (nr, q) in enumerate(members):
+        par = cfg.par_dict(Num_R=nr, Mass_Ratio=q, Autotune=0 if packed else args.autotune)
+        runs.append(HostRun(w["config"], state_io.write_par(os.path.join(tmp, str(i)), par)))
+    t_setup = time.perf_counter() - t0
+    ncell = sum(r.real_cells for r in runs)
+    print(f"bench {args.workload}: set-up of {len(runs)} members, {ncell} cells in "
+          f"{t_setup:.1f} s", file=sys.stderr, flush=True)
+    lib = capi.lib()
+    ens = None
+    if packed:
+        ens = Ensemble(runs, graphs=args.graphs == "on")
+        step_all, sync_all, tune = ens.step, ens.sync, 0
+    else:
+        for r in runs:
+            if lib.dh_enable_graphs(C.c_void_p(r.ctx), 1 if args.graphs == "on" else 0):
+                raise RuntimeError(lib.dh_last_error().decode())
+
+        def step_all():
+            dts = [r.getmindt() for r in runs]
+            for r, dt in zip(runs, dts):
+                r.timestep(dt)
+
+        def sync_all():
+            for r in runs:
+                if lib.dh_sync(C.c_void_p(r.ctx)):
+                    raise RuntimeError(lib.dh_last_error().decode())
+        tune = args.autotune
+
+    for _ in range(tune):  # the members' launch tuning (set-up, untimed)
+        step_all()
+    for _ in range(warmup):
+        step_all()
+    sync_all()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step_all()
+    sync_all()
+    secs = time.perf_counter() - t0
+    print(f"bench {args.workload}: {steps} timed steps in {secs:.2f} s", file=sys.stderr, flush=True)
+    if ens is not None:
+        ens.close()
+    for r in runs:
+        r.close()
+    value = ncell * steps / secs
+    counts = {n: sum(1 for m in members if m[0] == n) for n in (64, 128, 256, 512)}
+    ba = b_alg(cfg.nq, cfg.nf)
+    out = {
+        "metric": "zone-updates/s on 1 MI355X (%HBM roofline)",
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": secs / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (initial conditions from the workload's Initial plugin)",
+        "config": {"workload": w["desc"], "name": args.workload, "graphs": args.graphs == "on",
+                   "ensemble": "one launch per kernel for all members (dh_ens)" if packed
+                   else "one context and stream per member",
+                   "members": len(members), "members_by_Nr": counts, "real_cells": ncell,
+                   "setup_s": t_setup},
+        "roofline": None,
+        "step_roofline": {"b_alg_bytes_per_zone_update": ba, "achieved": ba * value / 1e9,
+                          "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                          "frac": ba * value / 1e9 / PEAK_HBM_GBS},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, w["cpu_over"], w["cpu_steps"])
+    return out
 
 
 def dominant_roofline(kernels, ba, ncell, workload):
@@ -477,6 +614,9 @@ def main(argv=None):
                     help="record the RK2 timestep once as a HIP graph and replay it")
     ap.add_argument("--autotune", type=int, default=8,
                     help="steps of launch tuning before the warmup (0: untuned launches)")
+    ap.add_argument("--ensemble-mode", choices=["packed", "streams"], default="packed",
+                    help="disk_ensemble: one launch per kernel for all members, or a "
+                         "context and stream per member")
     ap.add_argument("--no-c2p-compare", action="store_true",
                     help="skip the GRMHD plain vs learned cons2prim guess comparison")
     args = ap.parse_args(argv)

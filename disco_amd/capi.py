@@ -154,6 +154,13 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                          C.c_long, C.POINTER(C.c_float), C.POINTER(C.c_float)], C.c_int),
         "dh_c2p_grad_noalloc": ([vp, vp, vp, C.c_long, vp, vp, vp], C.c_int),
         "dh_autotune": ([vp, C.c_int], C.c_int),
+        "dh_ens_create": ([C.POINTER(vp), C.c_int, C.POINTER(vp)], C.c_int),
+        "dh_ens_destroy": ([vp], C.c_int),
+        "dh_ens_getmindt": ([vp, _dp], C.c_int),
+        "dh_ens_timestep": ([vp, _dp], C.c_int),
+        "dh_ens_add_diagnostics": ([vp, C.c_int, _dp, _dp, C.c_int], C.c_int),
+        "dh_ens_enable_graphs": ([vp, C.c_int], C.c_int),
+        "dh_ens_sync": ([vp], C.c_int),
         "dh_read_tuning": ([vp, C.c_int, C.c_void_p, _ip, _ip, _dp, _dp, _ip], C.c_int),
         "dh_last_error": ([], C.c_char_p),
     }
@@ -171,6 +178,8 @@ EXPORTED_SYMBOLS = ["dh_create", "dh_destroy", "dh_sizes", "dh_upload_state",
                     "dh_zero_diagnostics", "dh_avg_diagnostics", "dh_report", "dh_set_c2p_model",
                     "dh_c2p_harvest", "dh_c2p_predict", "dh_c2p_stats", "dh_c2p_train",
                     "dh_c2p_grad", "dh_c2p_grad_noalloc", "dh_autotune", "dh_read_tuning",
+                    "dh_ens_create", "dh_ens_destroy", "dh_ens_getmindt", "dh_ens_timestep",
+                    "dh_ens_add_diagnostics", "dh_ens_enable_graphs", "dh_ens_sync",
                     "dh_last_error"]
 
 VARIANT_NAMES = {0: "storage order, round-robin XCDs", 1: "storage order, XCD-contiguous"}

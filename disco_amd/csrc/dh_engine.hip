@@ -75,9 +75,10 @@ struct KZeroWords {
 // Start cell of every annulus (faces_alt.c:59-78): one wavefront per
 // annulus scans 64 cells at a time with ballots for the first cell in the
 // first quadrant that follows a cell in the fourth.
-__global__ __launch_bounds__(TPB) void k_ring_start(Mesh m, const double *piph, int *I0)
+// Block b of the launch (its TPB / 64 wavefronts take annuli 4 b .. 4 b + 3).
+__device__ void ring_start_block(const Mesh &m, const double *piph, int *I0, unsigned blk)
 {
-   int a = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+   int a = blk * (TPB / 64) + threadIdx.x / 64;
    int lane = threadIdx.x & 63;
    if (a >= m.nann) return;
    int c0 = m.off[a], np = m.Np[a];
@@ -110,13 +111,17 @@ __global__ __launch_bounds__(TPB) void k_ring_start(Mesh m, const double *piph, 
    }
    if (lane == 0) I0[a] = found < 0 ? 0 : found;
 }
+__global__ __launch_bounds__(TPB) void k_ring_start(Mesh m, const double *piph, int *I0)
+{
+   ring_start_block(m, piph, I0, blockIdx.x);
+}
 
 // flip_fluxes (bfields.c:742-814): one wavefront per face track; the lanes
 // read 64 flags at a time and lane 0 hands the flagged faces over in order.
-__global__ __launch_bounds__(TPB) void k_flips(Mesh m, int dim, Faces F, const signed char *flag,
-                                               double *Phi, double *rkPhi)
+__device__ void flips_block(const Mesh &m, int dim, const Faces &F, const signed char *flag,
+                            double *Phi, double *rkPhi, unsigned blk)
 {
-   int JK = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+   int JK = blk * (TPB / 64) + threadIdx.x / 64;
    int lane = threadIdx.x & 63;
    if (JK >= F.npairs || track_skipped(F.npairs, JK)) return;
    int base = F.fI[JK], N = F.fI[JK + 1] - base;
@@ -132,6 +137,11 @@ __global__ __launch_bounds__(TPB) void k_flips(Mesh m, int dim, Faces F, const s
       }
    }
 }
+__global__ __launch_bounds__(TPB) void k_flips(Mesh m, int dim, Faces F, const signed char *flag,
+                                               double *Phi, double *rkPhi)
+{
+   flips_block(m, dim, F, flag, Phi, rkPhi, blockIdx.x);
+}
 
 // The ghost-ring walk (KBcScan), one workgroup per ring.  The walk's state
 // on entering ghost cell i is the source cell holding the cell's lower edge;
@@ -140,14 +150,14 @@ __global__ __launch_bounds__(TPB) void k_flips(Mesh m, int dim, Faces F, const s
 // then checks each guess against one step of the walk from the guess before
 // it.  A ring with any miss is walked again sequentially, so start[] is the
 // sequential walk's in every case.
-__global__ __launch_bounds__(TPB) void k_bc_scan(Mesh m, int side, const double *piph,
-                                                 const double *dphi, int *start)
+__device__ void bc_scan_block(const Mesh &m, int side, const double *piph, const double *dphi,
+                              int *start, unsigned blk)
 {
    __shared__ int s_first, s_miss;
    __shared__ double s_minv[TPB];
    __shared__ int s_mini[TPB];
    int lines = bc_lines(m, side);
-   int t = blockIdx.x;
+   int t = blk;
    int line = t % lines, g = t / lines;
    int dst, src;
    bc_pair(m, side, line, g, &dst, &src);
@@ -193,19 +203,24 @@ __global__ __launch_bounds__(TPB) void k_bc_scan(Mesh m, int side, const double 
    __syncthreads();
    if (s_miss && tid == 0) KBcScan{m, side, piph, dphi, start}(t);
 }
+__global__ __launch_bounds__(TPB) void k_bc_scan(Mesh m, int side, const double *piph,
+                                                 const double *dphi, int *start)
+{
+   bc_scan_block(m, side, piph, dphi, start, blockIdx.x);
+}
 
 // add_diagnostics (analysis.c:52-106): one workgroup per annulus; each
 // thread sums get_diagnostics x dV and dV over every TPB-th cell, each wave
 // adds its lanes' sums and thread 0 the waves'.  (The host build sums the
 // cells in order, as the reference does; the two differ by rounding only.)
 template <int KIND>
-__global__ __launch_bounds__(TPB) void k_diag(Mesh m, dh_params P, DiagPlanet p1,
-                                              const double *prim, const double *piph,
-                                              const double *dphi, double dt, double *Qrz)
+__device__ void diag_block(const Mesh &m, const dh_params &P, const DiagPlanet &p1,
+                           const double *prim, const double *piph, const double *dphi, double dt,
+                           double *Qrz, unsigned blk)
 {
    constexpr int NQD = KIND == DH_DIAG_EULER ? 8 : KIND == DH_DIAG_DISK2D ? 5 : 18;
    __shared__ double red[TPB / 64][NQD + 1];
-   int a = blockIdx.x;  // the whole workgroup's annulus
+   int a = blk;  // the whole workgroup's annulus
    int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
    if (a >= m.nann) return;
    size_t nt = (size_t)m.ntot;
@@ -246,6 +261,13 @@ __global__ __launch_bounds__(TPB) void k_diag(Mesh m, dh_params P, DiagPlanet p1
       for (int q = 0; q < NQD; ++q) Qrz[(size_t)a * NQD + q] += sum[q] * dt / vol;
    }
 }
+template <int KIND>
+__global__ __launch_bounds__(TPB) void k_diag(Mesh m, dh_params P, DiagPlanet p1,
+                                              const double *prim, const double *piph,
+                                              const double *dphi, double dt, double *Qrz)
+{
+   diag_block<KIND>(m, P, p1, prim, piph, dphi, dt, Qrz, blockIdx.x);
+}
 
 // report (report.c:7-202): one wavefront per real annulus; each lane sums
 // the terms of every 64th cell, then the wave combines the lanes (the host
@@ -284,15 +306,16 @@ __global__ __launch_bounds__(TPB) void k_report(Mesh m, dh_params P, DiagPlanet 
 // per workgroup; the grid is capped (MINDT_BLOCKS) and each thread takes
 // every stride-th cell, so that few atomics meet at the one address.
 constexpr unsigned MINDT_BLOCKS = 2048;
+// Block blk of nb blocks that share the cells (a grid-stride loop).
 template <int NQ, class H>
-__global__ __launch_bounds__(TPB) void k_mindt(Mesh m, dh_params P, const double *prim,
-                                               const double *piph, const double *dphi,
-                                               const double *wiph, unsigned long long *out)
+__device__ void mindt_block(const Mesh &m, const dh_params &P, const double *prim,
+                            const double *piph, const double *dphi, const double *wiph,
+                            unsigned long long *out, unsigned blk, unsigned nb)
 {
    __shared__ double red[TPB / 64];
-   long stride = (long)gridDim.x * blockDim.x;
+   long stride = (long)nb * TPB;
    double dtc = HUGE_VAL;
-   for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < m.ntot; c += stride) {
+   for (long c = (long)blk * TPB + threadIdx.x; c < m.ntot; c += stride) {
       double v = cell_mindt<NQ, H>(m, P, prim, piph, dphi, wiph, c);
       dtc = v < dtc ? v : dtc;
    }
@@ -308,6 +331,13 @@ __global__ __launch_bounds__(TPB) void k_mindt(Mesh m, dh_params P, const double
       for (int w = 1; w < TPB / 64; ++w) v = red[w] < v ? red[w] : v;
       if (v < HUGE_VAL && v > 0.0) atomicMin(out, (unsigned long long)__double_as_longlong(v));
    }
+}
+template <int NQ, class H>
+__global__ __launch_bounds__(TPB) void k_mindt(Mesh m, dh_params P, const double *prim,
+                                               const double *piph, const double *dphi,
+                                               const double *wiph, unsigned long long *out)
+{
+   mindt_block<NQ, H>(m, P, prim, piph, dphi, wiph, out, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -854,6 +884,332 @@ static int do_faces(dh_ctx *c)
 }
 
 // ---------------------------------------------------------------------------
+// Ensembles (dh_ens_*): independent meshes of one module selection, each
+// with its own size, time step and planets, stepped together so that every
+// kernel of their timestep is launched once for all of them.  A mesh's work
+// in one launch is an "op" (a functor over its elements, or a cooperative
+// kernel's per-block body); the grouped kernel k_group<Op> gives block b to
+// the mesh whose range of the concatenated block table holds it.  The
+// timestep of every mesh is recorded by the same launch sequence the single
+// mesh runs (Engine::timestep on a recording executor), and the recorded
+// ops of equal position and type become one launch.
+// ---------------------------------------------------------------------------
+template <class K>
+struct RunOp {
+   K k;
+   long n;
+   unsigned blocks() const { return nblocks(n); }
+   __device__ void run(unsigned b) const
+   {
+      long t = (long)b * TPB + threadIdx.x;
+      if (t < n) k(t);
+   }
+};
+struct RingStartOp {
+   Mesh m;
+   const double *piph;
+   int *I0;
+   unsigned blocks() const { return (unsigned)((m.nann + 3) / 4); }
+   __device__ void run(unsigned b) const { ring_start_block(m, piph, I0, b); }
+};
+struct FlipsOp {
+   Mesh m;
+   int dim;
+   Faces F;
+   const signed char *flag;
+   double *Phi, *rkPhi;
+   unsigned blocks() const { return (unsigned)((F.npairs + 3) / 4); }
+   __device__ void run(unsigned b) const { flips_block(m, dim, F, flag, Phi, rkPhi, b); }
+};
+struct BcScanOp {
+   Mesh m;
+   int side;
+   const double *piph, *dphi;
+   int *start;
+   unsigned blocks() const { return (unsigned)((long)bc_lines(m, side) * bc_layers(m, side)); }
+   __device__ void run(unsigned b) const { bc_scan_block(m, side, piph, dphi, start, b); }
+};
+template <int NQ, class H>
+struct MindtOp {
+   Mesh m;
+   dh_params P;
+   const double *prim, *piph, *dphi, *wiph;
+   unsigned long long *out;
+   unsigned nb;
+   unsigned blocks() const { return nb; }
+   __device__ void run(unsigned b) const
+   {
+      mindt_block<NQ, H>(m, P, prim, piph, dphi, wiph, out, b, nb);
+   }
+};
+// dt and planet 1 of a mesh's diagnostics, written per call
+struct EnsDiagArgs {
+   double dt;
+   DiagPlanet p1;
+};
+template <int KIND>
+struct DiagOp {
+   Mesh m;
+   dh_params P;
+   const EnsDiagArgs *a;
+   const double *prim, *piph, *dphi;
+   double *Qrz;
+   unsigned blocks() const { return (unsigned)m.nann; }
+   __device__ void run(unsigned b) const
+   {
+      diag_block<KIND>(m, P, a->p1, prim, piph, dphi, a->dt, Qrz, b);
+   }
+};
+
+// Block b of the concatenated table goes to mesh m with boff[m] <= b < boff[m + 1].
+template <class Op>
+__global__ __launch_bounds__(TPB) void k_group(const Op *ops, const unsigned *boff, int nm)
+{
+   unsigned b = blockIdx.x;
+   int lo = 0, hi = nm;
+   while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (boff[mid] <= b) lo = mid;
+      else hi = mid;
+   }
+   ops[lo].run(b - boff[lo]);
+}
+
+// Every mesh's {dt, planets} of the step, one thread per mesh.
+__global__ void k_set_steps(StepArgs *const *dst, const StepArgs *src, int nm)
+{
+   int m = blockIdx.x * blockDim.x + threadIdx.x;
+   if (m < nm) *dst[m] = src[m];
+}
+__global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
+{
+   int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) p[i] = v;
+}
+
+// One recorded op of one mesh.
+struct GroupOp {
+   const void *tag;  // the op's type
+   size_t size;
+   std::vector<unsigned char> bytes;
+   unsigned blocks;
+   void (*launch)(const void *ops, const unsigned *boff, int nm, unsigned total, hipStream_t s);
+};
+template <class Op>
+struct OpType {
+   static const void *tag()
+   {
+      static const char t = 0;
+      return &t;
+   }
+   static void launch(const void *ops, const unsigned *boff, int nm, unsigned total, hipStream_t s)
+   {
+      if (total) k_group<Op><<<total, TPB, 0, s>>>((const Op *)ops, boff, nm);
+   }
+};
+
+// The executor that records a mesh's timestep as ops (see Engine::timestep
+// for what a timestep asks of an executor).
+struct GroupExec {
+   std::vector<GroupOp> ops;
+   StepArgs *step_dst = nullptr;
+   StepArgs step{};
+   template <class Op>
+   int rec(const Op &op)
+   {
+      GroupOp g;
+      g.tag = OpType<Op>::tag();
+      g.size = sizeof(Op);
+      g.bytes.resize(sizeof(Op));
+      memcpy(g.bytes.data(), &op, sizeof(Op));
+      g.blocks = op.blocks();
+      g.launch = &OpType<Op>::launch;
+      ops.push_back(std::move(g));
+      return DH_OK;
+   }
+   template <class K>
+   int run(const char *, const K &k, long n)
+   {
+      return n > 0 ? rec(RunOp<K>{k, n}) : DH_OK;
+   }
+   template <class K>
+   int run_tuned(const char *name, const K &k, long n, const GroupOrder *, int, const int *)
+   {
+      return run(name, k, n);
+   }
+   int ring_start(const Mesh &m, const double *piph, int *I0)
+   {
+      return rec(RingStartOp{m, piph, I0});
+   }
+   int bc_scan(const Mesh &m, int side, const double *piph, const double *dphi, int *start)
+   {
+      if ((long)bc_lines(m, side) * bc_layers(m, side) <= 0) return DH_OK;
+      return rec(BcScanOp{m, side, piph, dphi, start});
+   }
+   int flips(const Mesh &m, int dim, const Faces &F, const signed char *flag, double *Phi,
+             double *rkPhi)
+   {
+      if (F.npairs <= 0) return DH_OK;
+      return rec(FlipsOp{m, dim, F, flag, Phi, rkPhi});
+   }
+   template <int NQ>
+   int c2p_guess(const Mesh &, const dh_params &, const double *, const double *, const double *,
+                 const float *, const unsigned short *, float *)
+   {
+      return fail(DH_EINVAL, "the learned cons2prim guess does not run in an ensemble");
+   }
+   int copy(void *dst, const void *src, size_t bytes)
+   {
+      if (!HipExec::words(dst, bytes) || !HipExec::words(src, bytes))
+         return fail(DH_EINVAL, "an ensemble copies whole 8-byte words only");
+      return run("copy", KCopyWords{(unsigned long long *)dst, (const unsigned long long *)src},
+                 (long)(bytes / 8));
+   }
+   int zero(void *p, size_t bytes)
+   {
+      if (!HipExec::words(p, bytes)) return fail(DH_EINVAL, "an ensemble clears whole 8-byte words only");
+      return run("zero", KZeroWords{(unsigned long long *)p}, (long)(bytes / 8));
+   }
+   int set_step(StepArgs *dst, const StepArgs &sa)
+   {
+      step_dst = dst;
+      step = sa;
+      return DH_OK;
+   }
+   template <class F>
+   int replayable(const void *, F &&body)
+   {
+      return body();
+   }
+};
+
+// One launch of an ensemble: the ops of every mesh in device memory and
+// their concatenated block table.
+struct EnsLaunch {
+   const void *tag = nullptr;
+   void (*launch)(const void *, const unsigned *, int, unsigned, hipStream_t) = nullptr;
+   std::vector<unsigned char> host;  // every mesh's op bytes, in mesh order
+   unsigned total = 0;
+   void *d_ops = nullptr;
+   unsigned *d_boff = nullptr;
+};
+
+template <int NQ, class H>
+static int ens_record_step(GroupExec *gx, Buffers *b, double dt)
+{
+   Engine<GroupExec> e{*gx, *b};
+   return e.template timestep<NQ, H>(dt);
+}
+template <int NQ, class H>
+static int ens_record_mindt(GroupExec *gx, Buffers *b, unsigned long long *out)
+{
+   unsigned nb = nblocks(b->m.ntot) < 512 ? nblocks(b->m.ntot) : 512;
+   return gx->rec(MindtOp<NQ, H>{b->m, b->P, b->prim, b->piph, b->dphi, b->wiph, out, nb});
+}
+
+template <int KIND>
+static int ens_record_diag(GroupExec *gx, Buffers *b, const EnsDiagArgs *a)
+{
+   return gx->rec(DiagOp<KIND>{b->m, b->P, a, b->prim, b->piph, b->dphi, b->diagQ});
+}
+
+struct dh_ens {
+   std::vector<dh_ctx *> mem;
+   std::vector<hipStream_t> own_stream;  // the members' streams before they joined
+   int nm = 0;
+   hipStream_t stream = nullptr;         // every member's work goes here while joined
+   std::vector<EnsLaunch> plan;          // the timestep's launches
+   bool graphs = false;
+   hipGraphExec_t gexec = nullptr;
+   StepArgs **d_dst = nullptr;           // each member's StepArgs
+   StepArgs *d_step = nullptr, *h_step = nullptr;  // this step's values (h_step pinned)
+   EnsLaunch mindt;
+   unsigned long long *d_min = nullptr, *h_min = nullptr;
+   EnsLaunch diag;
+   int diag_kind = 0;
+   EnsDiagArgs *d_diag = nullptr, *h_diag = nullptr;
+   std::vector<void *> dev, pinned;
+};
+
+static void ens_free_launch(EnsLaunch &L)
+{
+   if (L.d_ops) (void)hipFree(L.d_ops);
+   if (L.d_boff) (void)hipFree(L.d_boff);
+   L = EnsLaunch();
+}
+
+static void ens_drop_graph(dh_ens *e)
+{
+   if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+   e->gexec = nullptr;
+}
+
+static void ens_release(dh_ens *e)
+{
+   if (!e) return;
+   if (e->stream) (void)hipStreamSynchronize(e->stream);
+   ens_drop_graph(e);
+   for (auto &L : e->plan) ens_free_launch(L);
+   ens_free_launch(e->mindt);
+   ens_free_launch(e->diag);
+   for (void *p : e->dev) (void)hipFree(p);
+   for (void *p : e->pinned) (void)hipHostFree(p);
+   for (size_t m = 0; m < e->own_stream.size(); ++m) e->mem[m]->x.stream = e->own_stream[m];
+   delete e;
+}
+
+// The ops of position i of every member's record as one launch (device
+// copies of the ops and the block table).
+// *changed tells whether the launch differs from what L held (then L is
+// rebuilt; otherwise its device copies stay as they are).
+static int ens_build_launch(const std::vector<GroupExec> &rec, size_t i, EnsLaunch &L,
+                            bool *changed)
+{
+   int nm = (int)rec.size();
+   const GroupOp &g0 = rec[0].ops[i];
+   std::vector<unsigned char> host;
+   std::vector<unsigned> boff(nm + 1, 0);
+   for (int m = 0; m < nm; ++m) {
+      const GroupOp &g = rec[m].ops[i];
+      host.insert(host.end(), g.bytes.begin(), g.bytes.end());
+      boff[m + 1] = boff[m] + g.blocks;
+   }
+   *changed = !(L.d_ops && L.tag == g0.tag && L.host == host && L.total == boff[nm]);
+   if (!*changed) return DH_OK;
+   ens_free_launch(L);
+   L.tag = g0.tag;
+   L.launch = g0.launch;
+   L.host = std::move(host);
+   L.total = boff[nm];
+   HIPCHK(hipMalloc(&L.d_ops, L.host.size()));
+   HIPCHK(hipMalloc((void **)&L.d_boff, sizeof(unsigned) * (nm + 1)));
+   HIPCHK(hipMemcpy(L.d_ops, L.host.data(), L.host.size(), hipMemcpyHostToDevice));
+   HIPCHK(hipMemcpy(L.d_boff, boff.data(), sizeof(unsigned) * (nm + 1), hipMemcpyHostToDevice));
+   return DH_OK;
+}
+
+// The members' records agree op by op (same type at every position).
+static int ens_check_records(const std::vector<GroupExec> &rec)
+{
+   for (size_t m = 1; m < rec.size(); ++m) {
+      if (rec[m].ops.size() != rec[0].ops.size())
+         return fail(DH_EINVAL, "ensemble member %zu launches %zu kernels in a step, member 0 %zu",
+                     m, rec[m].ops.size(), rec[0].ops.size());
+      for (size_t i = 0; i < rec[0].ops.size(); ++i)
+         if (rec[m].ops[i].tag != rec[0].ops[i].tag)
+            return fail(DH_EINVAL, "ensemble member %zu differs from member 0 at launch %zu", m, i);
+   }
+   return DH_OK;
+}
+
+static int ens_launch(dh_ens *e, const EnsLaunch &L)
+{
+   L.launch(L.d_ops, L.d_boff, e->nm, L.total, e->stream);
+   LAUNCH_CHECK();
+   return DH_OK;
+}
+
+// ---------------------------------------------------------------------------
 // The learned guess's loss gradient without allocation (dh_c2p_grad_noalloc).
 // The per-row activations and losses of dh_c2p_grad live in `scratch`, one
 // file-scope device array: DH_C2P_NOALLOC_MAX_ROWS rows of C2PNET_ROW
@@ -1114,6 +1470,214 @@ int dh_c2p_grad_noalloc(const float *params, const float *X, const float *Y, lon
    k_run<KLossMean><<<1, TPB, 0, st>>>(KLossMean{B.loss, n, loss}, 1);
    LAUNCH_CHECK();
    return DH_OK;
+}
+
+int dh_ens_create(dh_ctx *const *members, int n, dh_ens **out)
+{
+   if (!members || n < 1 || !out) return fail(DH_EINVAL, "bad arguments");
+   for (int m = 0; m < n; ++m) {
+      if (!members[m]) return fail(DH_EINVAL, "null member %d", m);
+      for (int k = 0; k < m; ++k)
+         if (members[k] == members[m]) return fail(DH_EINVAL, "member %d appears twice", m);
+      const Buffers &a = members[0]->b, &b = members[m]->b;
+      if (b.P.hydro != a.P.hydro || b.nq != a.nq || b.nf != a.nf)
+         return fail(DH_EINVAL, "member %d has another hydro module or field count", m);
+   }
+   dh_ens *e = new dh_ens();
+   e->mem.assign(members, members + n);
+   e->nm = n;
+   e->stream = members[0]->x.stream;
+   for (int m = 0; m < n; ++m) {
+      dh_ctx *c = members[m];
+      e->own_stream.push_back(c->x.stream);
+      if (hipStreamSynchronize(c->x.stream) != hipSuccess) {
+         ens_release(e);
+         return fail(DH_EHIP, "member %d: stream synchronisation failed", m);
+      }
+      c->x.drop_graph();
+      c->x.stream = e->stream;
+   }
+   std::vector<StepArgs *> dst(n);
+   for (int m = 0; m < n; ++m) dst[m] = members[m]->b.step;
+   void *p = nullptr;
+   auto dev = [&](void **q, size_t bytes) -> int {
+      if (hipMalloc(q, bytes) != hipSuccess) return fail(DH_ENOMEM, "hipMalloc(%zu) failed", bytes);
+      e->dev.push_back(*q);
+      return DH_OK;
+   };
+   auto pin = [&](void **q, size_t bytes) -> int {
+      if (hipHostMalloc(q, bytes, 0) != hipSuccess)
+         return fail(DH_ENOMEM, "hipHostMalloc(%zu) failed", bytes);
+      e->pinned.push_back(*q);
+      return DH_OK;
+   };
+   int rc = dev(&p, sizeof(StepArgs *) * n);
+   e->d_dst = (StepArgs **)p;
+   if (!rc) rc = dev((void **)&e->d_step, sizeof(StepArgs) * n);
+   if (!rc) rc = pin((void **)&e->h_step, sizeof(StepArgs) * n);
+   if (!rc) rc = dev((void **)&e->d_min, sizeof(unsigned long long) * n);
+   if (!rc) rc = pin((void **)&e->h_min, sizeof(unsigned long long) * n);
+   if (!rc) rc = dev((void **)&e->d_diag, sizeof(EnsDiagArgs) * n);
+   if (!rc) rc = pin((void **)&e->h_diag, sizeof(EnsDiagArgs) * n);
+   if (!rc && hipMemcpy(e->d_dst, dst.data(), sizeof(StepArgs *) * n, hipMemcpyHostToDevice) !=
+                  hipSuccess)
+      rc = fail(DH_EHIP, "hipMemcpy of the step table failed");
+   if (rc) {
+      ens_release(e);
+      return rc;
+   }
+   *out = e;
+   return DH_OK;
+}
+
+int dh_ens_destroy(dh_ens *e)
+{
+   ens_release(e);
+   return DH_OK;
+}
+
+int dh_ens_enable_graphs(dh_ens *e, int on)
+{
+   if (!e) return fail(DH_EINVAL, "null ensemble");
+   e->graphs = on != 0;
+   if (!e->graphs) ens_drop_graph(e);
+   return DH_OK;
+}
+
+int dh_ens_sync(dh_ens *e)
+{
+   if (!e) return fail(DH_EINVAL, "null ensemble");
+   HIPCHK(hipStreamSynchronize(e->stream));
+   return DH_OK;
+}
+
+int dh_ens_getmindt(dh_ens *e, double *dts)
+{
+   if (!e || !dts) return fail(DH_EINVAL, "null argument");
+   std::vector<GroupExec> rec(e->nm);
+   for (int m = 0; m < e->nm; ++m) {
+      Buffers &b = e->mem[m]->b;
+      DH_TRY(DH_DISPATCH(b, ens_record_mindt, &rec[m], &b, e->d_min + m));
+   }
+   bool changed = false;
+   DH_TRY(ens_build_launch(rec, 0, e->mindt, &changed));
+   k_fill_u64<<<(e->nm + 63) / 64, 64, 0, e->stream>>>(e->d_min, e->nm, 0x7FF0000000000000ULL);
+   LAUNCH_CHECK();
+   DH_TRY(ens_launch(e, e->mindt));
+   HIPCHK(hipMemcpyAsync(e->h_min, e->d_min, sizeof(unsigned long long) * e->nm,
+                         hipMemcpyDeviceToHost, e->stream));
+   HIPCHK(hipStreamSynchronize(e->stream));
+   for (int m = 0; m < e->nm; ++m) {
+      const dh_params &P = e->mem[m]->b.P;
+      double v;
+      memcpy(&v, &e->h_min[m], sizeof v);
+      double d0 = P.max_dt / P.cfl;  // as Engine::mindt
+      if (d0 <= 0.0) d0 = 1.0e100;
+      if (v < d0) d0 = v;
+      dts[m] = d0 * P.cfl;
+   }
+   return DH_OK;
+}
+
+int dh_ens_timestep(dh_ens *e, const double *dts)
+{
+   if (!e || !dts) return fail(DH_EINVAL, "null argument");
+   for (int m = 0; m < e->nm; ++m)
+      if (!(dts[m] > 0.0)) return fail(DH_EINVAL, "dt of member %d must be positive", m);
+   std::vector<GroupExec> rec(e->nm);
+   for (int m = 0; m < e->nm; ++m) {
+      Buffers &b = e->mem[m]->b;
+      DH_TRY(DH_DISPATCH(b, ens_record_step, &rec[m], &b, dts[m]));
+      if (rec[m].step_dst != b.step) return fail(DH_EINVAL, "member %d recorded no step arguments", m);
+   }
+   DH_TRY(ens_check_records(rec));
+   size_t nops = rec[0].ops.size();
+   bool same = e->plan.size() == nops;
+   while (e->plan.size() > nops) {
+      ens_free_launch(e->plan.back());
+      e->plan.pop_back();
+   }
+   e->plan.resize(nops);
+   for (size_t i = 0; i < nops; ++i) {
+      bool changed = false;
+      DH_TRY(ens_build_launch(rec, i, e->plan[i], &changed));
+      if (changed) same = false;
+   }
+   if (!same) ens_drop_graph(e);
+   // the previous step's upload has left the pinned buffer (the stream
+   // runs in order; getmindt and sync wait for it)
+   HIPCHK(hipStreamSynchronize(e->stream));
+   for (int m = 0; m < e->nm; ++m) e->h_step[m] = rec[m].step;
+   HIPCHK(hipMemcpyAsync(e->d_step, e->h_step, sizeof(StepArgs) * e->nm, hipMemcpyHostToDevice,
+                         e->stream));
+   k_set_steps<<<(e->nm + 63) / 64, 64, 0, e->stream>>>(e->d_dst, e->d_step, e->nm);
+   LAUNCH_CHECK();
+   if (e->graphs && same) {
+      if (!e->gexec) {
+         HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+         int rc = DH_OK;
+         for (const EnsLaunch &L : e->plan)
+            if (!rc) rc = ens_launch(e, L);
+         hipGraph_t g = nullptr;
+         hipError_t er = hipStreamEndCapture(e->stream, &g);
+         if (rc || er != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : fail(DH_EHIP, "ensemble capture failed: %s", hipGetErrorString(er));
+         }
+         er = hipGraphInstantiateWithFlags(&e->gexec, g, 0);
+         (void)hipGraphDestroy(g);
+         if (er != hipSuccess) {
+            e->gexec = nullptr;
+            return fail(DH_EHIP, "ensemble graph instantiation failed: %s", hipGetErrorString(er));
+         }
+      }
+      HIPCHK(hipGraphLaunch(e->gexec, e->stream));
+      return DH_OK;
+   }
+   for (const EnsLaunch &L : e->plan) DH_TRY(ens_launch(e, L));
+   return DH_OK;
+}
+
+int dh_ens_add_diagnostics(dh_ens *e, int kind, const double *dts, const double *planets, int npl)
+{
+   if (!e || !dts) return fail(DH_EINVAL, "null argument");
+   if (!diag_count(kind)) return fail(DH_EINVAL, "unknown diagnostics kind %d", kind);
+   if (npl < 0 || npl > DH_MAX_PLANETS) return fail(DH_EINVAL, "npl %d out of range", npl);
+   if (npl > 1 && !planets) return fail(DH_EINVAL, "null planet list");
+   HIPCHK(hipStreamSynchronize(e->stream));  // h_diag is free again
+   std::vector<GroupExec> rec(e->nm);
+   for (int m = 0; m < e->nm; ++m) {
+      dh_ctx *c = e->mem[m];
+      Buffers &b = c->b;
+      if (kind == DH_DIAG_MHDMRI && !b.mhd)
+         return fail(DH_EINVAL, "diag_mhdmri needs the MHD fields (NUM_C 8)");
+      size_t bytes = sizeof(double) * (size_t)b.m.nann * DH_DIAG_MAXQ;
+      if (!b.diagQ) {
+         DH_TRY(c->x.alloc((void **)&b.diagQ, bytes));
+         DH_TRY(c->x.zero(b.diagQ, bytes));
+      } else if (b.diag_kind != kind) {
+         DH_TRY(c->x.zero(b.diagQ, bytes));
+      }
+      b.diag_kind = kind;
+      EnsDiagArgs &a = e->h_diag[m];
+      a.dt = dts[m];
+      a.p1 = DiagPlanet{};
+      a.p1.on = npl > 1;
+      if (a.p1.on)
+         for (int q = 0; q < DH_PLANET_STRIDE; ++q)
+            a.p1.v[q] = planets[(size_t)m * DH_MAX_PLANETS * DH_PLANET_STRIDE + DH_PLANET_STRIDE + q];
+      const EnsDiagArgs *da = e->d_diag + m;
+      int rc = kind == DH_DIAG_EULER    ? ens_record_diag<DH_DIAG_EULER>(&rec[m], &b, da)
+               : kind == DH_DIAG_DISK2D ? ens_record_diag<DH_DIAG_DISK2D>(&rec[m], &b, da)
+                                        : ens_record_diag<DH_DIAG_MHDMRI>(&rec[m], &b, da);
+      DH_TRY(rc);
+   }
+   bool changed = false;
+   DH_TRY(ens_build_launch(rec, 0, e->diag, &changed));
+   e->diag_kind = kind;
+   HIPCHK(hipMemcpyAsync(e->d_diag, e->h_diag, sizeof(EnsDiagArgs) * e->nm, hipMemcpyHostToDevice,
+                         e->stream));
+   return ens_launch(e, e->diag);
 }
 
 int dh_c2p_stats(dh_ctx *c, long long *out)

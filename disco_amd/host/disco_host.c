@@ -848,10 +848,10 @@ static void planets_clean(struct domain *d)
    }
 }
 
-/* timestep, timestep.c:7-45.  The planets each substep's sources see are
- * computed on the host first (onestep.c:164-168 moves them after the
- * sources), then the whole step runs on the device. */
-void timestep(struct domain *d, double dt)
+/* The host's part of a timestep before the device's (timestep.c:18-27 and
+ * onestep.c:164-168): the planets each substep's sources see, handed to the
+ * device context. */
+int disco_step_begin(struct domain *d, double dt)
 {
    struct disco_accel *A = (struct disco_accel *)d->accel;
    double pl0[DH_MAX_PLANETS * DH_PLANET_STRIDE], pl1[DH_MAX_PLANETS * DH_PLANET_STRIDE];
@@ -860,7 +860,6 @@ void timestep(struct domain *d, double dt)
       pl->RK_r = pl->r; pl->RK_phi = pl->phi; pl->RK_M = pl->M;
       pl->RK_omega = pl->omega; pl->RK_vr = pl->vr;
    }
-   int analytic = planet_motion_analytic();
    planets_pack(d, pl0);
    /* after substep 1 */
    planets_rk_adjust(d, 0.0);
@@ -869,20 +868,45 @@ void timestep(struct domain *d, double dt)
    planets_pack(d, pl1);
    int rc = dh_set_planets(A->ctx, 0, pl0, d->Npl);
    if (!rc) rc = dh_set_planets(A->ctx, 1, pl1, d->Npl);
-   if (!rc) rc = dh_timestep(A->ctx, dt);
-   if (rc) {
-      host_fail("dh_timestep: %s", dh_last_error());
-      exit(1);
-   }
+   return rc ? host_fail("dh_set_planets: %s", dh_last_error()) : 0;
+}
+
+/* The host's part after the device's step: t, the planets' second RK half
+ * (timestep.c:28-39) and the step count. */
+void disco_step_end(struct domain *d, double dt)
+{
+   struct disco_accel *A = (struct disco_accel *)d->accel;
    d->t += dt;
-   if (d->theParList.Timestep != 1 && !analytic) {
+   if (d->theParList.Timestep != 1 && !planet_motion_analytic()) {
       planets_rk_adjust(d, 0.5);
       movePlanets(d->thePlanets, d->t, 0.5 * dt);
       planets_clean(d);
    }
    A->host_dirty = 1;
-   add_diagnostics(d, dt);
    d->count_steps += 1;
+}
+
+/* For a host that steps several meshes together (dh_ens_*): the planets in
+ * the device's layout, the Analysis module's kind, and dt added to the
+ * averaging time as add_diagnostics does. */
+void disco_pack_planets(struct domain *d, double *out) { planets_pack(d, out); }
+int disco_diag_kind(struct domain *d) { return ((struct disco_accel *)d->accel)->diag_kind; }
+int disco_npl(struct domain *d) { return d->Npl; }
+void disco_add_diag_time(struct domain *d, double dt) { d->theTools.t_avg += dt; }
+
+/* timestep, timestep.c:7-45.  The planets each substep's sources see are
+ * computed on the host first (onestep.c:164-168 moves them after the
+ * sources), then the whole step runs on the device. */
+void timestep(struct domain *d, double dt)
+{
+   struct disco_accel *A = (struct disco_accel *)d->accel;
+   if (disco_step_begin(d, dt)) exit(1);
+   if (dh_timestep(A->ctx, dt)) {
+      host_fail("dh_timestep: %s", dh_last_error());
+      exit(1);
+   }
+   disco_step_end(d, dt);
+   add_diagnostics(d, dt);
 }
 
 /* check_dt, domain.c:284-311, under its own name (serial; the abort/latest files as in the

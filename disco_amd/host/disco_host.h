@@ -45,6 +45,17 @@ long disco_real_cells(const struct domain *d);
 int disco_write_dstate(struct domain *d, const char *path);
 int disco_restart_dstate(struct domain *d, const char *path);
 
+/* timestep's host parts around the device step, for a host that steps
+ * several meshes together (dh_ens_*): disco_step_begin (planets for the
+ * substeps), disco_step_end (t, planets, step count), and for the
+ * diagnostics disco_pack_planets, disco_diag_kind, disco_add_diag_time. */
+int disco_step_begin(struct domain *d, double dt);
+void disco_step_end(struct domain *d, double dt);
+void disco_pack_planets(struct domain *d, double *out);
+int disco_diag_kind(struct domain *d);
+int disco_npl(struct domain *d);
+void disco_add_diag_time(struct domain *d, double dt);
+
 double getmindt(struct domain *d);
 void timestep(struct domain *d, double dt);
 void disco_check_dt(struct domain *d, double *dt);

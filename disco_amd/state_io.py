@@ -187,8 +187,9 @@ PRESETS: Dict[str, Dict[str, object]] = {
 
 
 # Keys of this path with no reference counterpart (grmhd2 cons2prim's guess,
-# stopping rule and model file); written only when given.
-EXTRA_KEYS = ("C2P_Guess", "C2P_Stop", "C2P_Model", "Graphs")
+# stopping rule and model file, the recorded timestep, the launch tuner's
+# steps); written only when given.
+EXTRA_KEYS = ("C2P_Guess", "C2P_Stop", "C2P_Model", "Graphs", "Autotune")
 
 
 def format_par(overrides: Optional[Dict[str, object]] = None) -> str:

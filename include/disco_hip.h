@@ -286,6 +286,37 @@ int dh_read_timers(dh_ctx *ctx, int cap, char (*names)[48], double *ms,
  * meshes.  The host build accepts and ignores it. */
 int dh_enable_graphs(dh_ctx *ctx, int on);
 
+/* Ensembles: independent meshes stepped together (no counterpart in the
+ * reference, which runs one mesh per process).  The members are contexts of
+ * one hydro module and field count, created and set up as usual (each with
+ * its own grid, parameters, time step and planets); while they belong to an
+ * ensemble their work runs on one stream.  Every kernel of a timestep is
+ * launched once for all members: each member's launches are recorded in
+ * the order dh_timestep makes them, and the launches of equal position
+ * (the same kernel) become one launch whose blocks are dealt to the members
+ * by a concatenated block table.
+ *   dh_ens_getmindt   every member's CFL time step (getmindt, misc.c:39-84),
+ *                     one reduction over all members' cells, one copy out;
+ *   dh_ens_timestep   one timestep of every member with its own dt[m]
+ *                     (timestep.c:7-45), with the planets each member was
+ *                     given by dh_set_planets; recorded as one HIP graph
+ *                     when dh_ens_enable_graphs is on and the launches are
+ *                     the same as the step before;
+ *   dh_ens_add_diagnostics  add_diagnostics (analysis.c:52-106) of every
+ *                     member: planets[m * DH_MAX_PLANETS * DH_PLANET_STRIDE
+ *                     ...] are member m's npl planets after the step;
+ *   dh_ens_destroy    gives every member its own stream back.
+ * The members' results equal what dh_timestep gives each of them alone. */
+typedef struct dh_ens dh_ens;
+int dh_ens_create(dh_ctx *const *members, int n, dh_ens **out);
+int dh_ens_destroy(dh_ens *ens);
+int dh_ens_getmindt(dh_ens *ens, double *dts);
+int dh_ens_timestep(dh_ens *ens, const double *dts);
+int dh_ens_add_diagnostics(dh_ens *ens, int kind, const double *dts, const double *planets,
+                           int npl);
+int dh_ens_enable_graphs(dh_ens *ens, int on);
+int dh_ens_sync(dh_ens *ens);
+
 /* Launch tuner (no counterpart in the reference, whose loops run in one
  * order on one core).  Every kernel of the step that runs over cells or
  * faces can be launched in several traversal orders: storage order with the

@@ -484,6 +484,17 @@ int dh_c2p_stats(dh_ctx *c, long long *out)
 int dh_enable_timers(dh_ctx *c, int) { return c ? DH_OK : fail(DH_EINVAL, "null context"); }
 int dh_enable_graphs(dh_ctx *c, int) { return c ? DH_OK : fail(DH_EINVAL, "null context"); }
 int dh_autotune(dh_ctx *c, int) { return c ? DH_OK : fail(DH_EINVAL, "null context"); }
+// Ensembles group device launches; the host build has none.
+int dh_ens_create(dh_ctx *const *, int, dh_ens **) { return fail(DH_ENODEV, "ensembles need the GPU library"); }
+int dh_ens_destroy(dh_ens *) { return DH_OK; }
+int dh_ens_getmindt(dh_ens *, double *) { return fail(DH_ENODEV, "ensembles need the GPU library"); }
+int dh_ens_timestep(dh_ens *, const double *) { return fail(DH_ENODEV, "ensembles need the GPU library"); }
+int dh_ens_add_diagnostics(dh_ens *, int, const double *, const double *, int)
+{
+   return fail(DH_ENODEV, "ensembles need the GPU library");
+}
+int dh_ens_enable_graphs(dh_ens *, int) { return fail(DH_ENODEV, "ensembles need the GPU library"); }
+int dh_ens_sync(dh_ens *) { return fail(DH_ENODEV, "ensembles need the GPU library"); }
 // The host build has no device array to work in.
 int dh_c2p_grad_noalloc(const float *, const float *, const float *, long, float *, float *, void *)
 {
