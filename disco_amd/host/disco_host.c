@@ -70,20 +70,55 @@ struct disco_accel {
    int autotune;
    /* possiblyOutput's report counter (domain.c:103, 316-335) */
    int nrpt;
+   /* the domain was set up by the reference's own code (setupDomain,
+    * setupCells) and joined the device on its first getmindt or timestep:
+    * the host mirror is refreshed whenever the reference's possiblyOutput
+    * will read it */
+   int bound;
 };
+
+/* The host layer's bookkeeping of each domain, kept beside it (struct
+ * domain is the reference's, with no room for it): a small table keyed by
+ * the domain's address. */
+#define DISCO_MAX_DOMAINS 256
+static struct {
+   const struct domain *d;
+   struct disco_accel *A;
+} g_accel[DISCO_MAX_DOMAINS];
+
+static struct disco_accel *accel_find(const struct domain *d)
+{
+   for (int i = 0; i < DISCO_MAX_DOMAINS; ++i)
+      if (g_accel[i].d == d) return g_accel[i].A;
+   return NULL;
+}
 
 static struct disco_accel *accel_of(struct domain *d)
 {
-   if (!d->accel) {
-      struct disco_accel *A = calloc(1, sizeof(struct disco_accel));
-      if (A) {
+   struct disco_accel *A = accel_find(d);
+   if (A) return A;
+   for (int i = 0; i < DISCO_MAX_DOMAINS; ++i)
+      if (!g_accel[i].d) {
+         A = calloc(1, sizeof(struct disco_accel));
+         if (!A) return NULL;
          A->graphs = 1;
          A->autotune = DISCO_AUTOTUNE_STEPS;
          A->nrpt = -1;
+         g_accel[i].d = d;
+         g_accel[i].A = A;
+         return A;
       }
-      d->accel = A;
-   }
-   return (struct disco_accel *)d->accel;
+   fprintf(stderr, "disco: more than %d domains\n", DISCO_MAX_DOMAINS);
+   exit(1);
+}
+
+static void accel_forget(const struct domain *d)
+{
+   for (int i = 0; i < DISCO_MAX_DOMAINS; ++i)
+      if (g_accel[i].d == d) {
+         g_accel[i].d = NULL;
+         g_accel[i].A = NULL;
+      }
 }
 
 static int diag_kind_of_module(const char *name)
@@ -402,7 +437,7 @@ static void fill_params(struct domain *d, dh_params *P, const int *bc)
                : !strcmp(METRIC, "schw_ks_rpz")    ? DH_METRIC_SCHW_KS
                : !strcmp(METRIC, "kerr_ks_rpz")    ? DH_METRIC_KERR_KS
                                                    : -1; /* refused at dh_create */
-   const struct disco_accel *A = (const struct disco_accel *)d->accel;
+   const struct disco_accel *A = accel_find(d);
    P->c2p_guess = A ? A->c2p_guess : DH_C2P_GUESS_PREVIOUS;
    P->c2p_stop = A ? A->c2p_stop : DH_C2P_STOP_REFERENCE;
    P->riemann_solver = L->Riemann_Solver;
@@ -462,7 +497,7 @@ static void planets_pack(const struct domain *d, double *out)
  * cell fields from them (B_faces_to_cells, bfields.c:80-81). */
 static int set_B_fields(struct domain *d)
 {
-   struct disco_accel *A = (struct disco_accel *)d->accel;
+   struct disco_accel *A = accel_find(d);
    long nt = A->ntot;
    int Nr = d->Nr, Nz = d->Nz;
    double *Phi = (double *)calloc((size_t)nt * 5, sizeof(double));
@@ -679,7 +714,7 @@ int disco_setup(struct domain *d)
 
 int disco_sync_to_host(struct domain *d)
 {
-   struct disco_accel *A = (struct disco_accel *)d->accel;
+   struct disco_accel *A = accel_find(d);
    if (!A || !A->host_dirty) return 0;
    long nt = A->ntot;
    double *prim = (double *)malloc(sizeof(double) * nt * NUM_Q);
@@ -687,7 +722,8 @@ int disco_sync_to_host(struct domain *d)
    double *piph = (double *)malloc(sizeof(double) * nt);
    double *dphi = (double *)malloc(sizeof(double) * nt);
    double *wiph = (double *)malloc(sizeof(double) * nt);
-   int rc = dh_download_state(A->ctx, prim, cons, piph, dphi, wiph, NULL);
+   double *Phi = NUM_FACES > 0 ? (double *)malloc(sizeof(double) * nt * NUM_FACES) : NULL;
+   int rc = dh_download_state(A->ctx, prim, cons, piph, dphi, wiph, Phi);
    if (rc == 0) {
       for (int a = 0; a < d->Nr * d->Nz; ++a)
          for (int i = 0; i < d->Np[a]; ++i) {
@@ -697,13 +733,14 @@ int disco_sync_to_host(struct domain *d)
                cl->prim[q] = prim[q * nt + c];
                cl->cons[q] = cons[q * nt + c];
             }
+            for (int f = 0; f < NUM_FACES; ++f) cl->Phi[f] = Phi[f * nt + c];
             cl->piph = piph[c];
             cl->dphi = dphi[c];
             cl->wiph = wiph[c];
          }
       A->host_dirty = 0;
    }
-   free(prim); free(cons); free(piph); free(dphi); free(wiph);
+   free(prim); free(cons); free(piph); free(dphi); free(wiph); free(Phi);
    if (rc) return host_fail("dh_download_state: %s", dh_last_error());
    if (d->theTools.Qrz && dh_get_diagnostics(A->ctx, d->theTools.Qrz))
       return host_fail("dh_get_diagnostics: %s", dh_last_error());
@@ -718,7 +755,7 @@ int disco_sync_to_host(struct domain *d)
 
 void zero_diagnostics(struct domain *d)
 {
-   struct disco_accel *A = (struct disco_accel *)d->accel;
+   struct disco_accel *A = accel_find(d);
    if (dh_zero_diagnostics(A->ctx)) {
       host_fail("dh_zero_diagnostics: %s", dh_last_error());
       exit(1);
@@ -729,7 +766,7 @@ void zero_diagnostics(struct domain *d)
 
 void avg_diagnostics(struct domain *d)
 {
-   struct disco_accel *A = (struct disco_accel *)d->accel;
+   struct disco_accel *A = accel_find(d);
    if (dh_avg_diagnostics(A->ctx, d->theTools.t_avg) ||
        dh_get_diagnostics(A->ctx, d->theTools.Qrz)) {
       host_fail("avg_diagnostics: %s", dh_last_error());
@@ -740,7 +777,7 @@ void avg_diagnostics(struct domain *d)
 
 int disco_get_diagnostics(struct domain *d, double *Qrz)
 {
-   struct disco_accel *A = (struct disco_accel *)d->accel;
+   struct disco_accel *A = accel_find(d);
    return dh_get_diagnostics(A->ctx, Qrz) ? host_fail("dh_get_diagnostics: %s", dh_last_error())
                                           : 0;
 }
@@ -749,7 +786,7 @@ int disco_get_diagnostics(struct domain *d, double *Qrz)
  * current state to report.dat, in the reference's format. */
 void disco_report(struct domain *d)
 {
-   struct disco_accel *A = (struct disco_accel *)d->accel;
+   struct disco_accel *A = accel_find(d);
    double pl[DH_MAX_PLANETS * DH_PLANET_STRIDE], v[16];
    planets_pack(d, pl);
    if (dh_report(A->ctx, pl, d->Npl, v)) {
@@ -783,7 +820,7 @@ void disco_possibly_report(struct domain *d, int override)
 
 void add_diagnostics(struct domain *d, double dt)
 {
-   struct disco_accel *A = (struct disco_accel *)d->accel;
+   struct disco_accel *A = accel_find(d);
    double pl[DH_MAX_PLANETS * DH_PLANET_STRIDE];
    planets_pack(d, pl);
    if (dh_add_diagnostics(A->ctx, A->diag_kind, dt, pl, d->Npl)) {
@@ -796,30 +833,127 @@ void add_diagnostics(struct domain *d, double dt)
 int disco_export_state(struct domain *d, double *prim, double *cons, double *piph,
                        double *dphi, double *wiph)
 {
-   struct disco_accel *A = (struct disco_accel *)d->accel;
+   struct disco_accel *A = accel_find(d);
    if (dh_download_state(A->ctx, prim, cons, piph, dphi, wiph, NULL))
       return host_fail("dh_download_state: %s", dh_last_error());
    return 0;
 }
 
-long disco_ntot(const struct domain *d) { return ((struct disco_accel *)d->accel)->ntot; }
+long disco_ntot(const struct domain *d) { return (accel_find(d))->ntot; }
 
-void *disco_device_context(struct domain *d) { return ((struct disco_accel *)d->accel)->ctx; }
+void *disco_device_context(struct domain *d) { return (accel_find(d))->ctx; }
 
 /* ------------------------------------------------------------------------ */
 /* Reference-signature entry points                                          */
 /* ------------------------------------------------------------------------ */
 
-/* getmindt, misc.c:39-84 */
-double getmindt(struct domain *d)
+/* Device state of a domain the reference's own code has set up
+ * (read_par_file, setupGrid, setupDomain, setupCells and the first
+ * boundary_trans of main.c:35-62): its grid, parameters and every cell's
+ * prim, cons, Phi, piph, dphi and wiph go to a new device context as they
+ * are, so the device continues the reference's state exactly. */
+static int attach_reference_domain(struct domain *d)
 {
-   struct disco_accel *A = (struct disco_accel *)d->accel;
+   struct param_list *L = &d->theParList;
+   if (L->RotFrame) return host_fail("RotFrame is not supported");
+   if (L->R_Periodic || L->Z_Periodic) return host_fail("periodic boundaries are not supported");
+   int Nr = d->Nr, Nz = d->Nz;
+   int *Np = d->Np;
+   struct disco_accel *A = accel_of(d);
+   A->bound = 1;
+   A->off = (long *)calloc(Nr * Nz + 1, sizeof(long));
+   for (int a = 0; a < Nr * Nz; ++a) A->off[a + 1] = A->off[a] + Np[a];
+   A->ntot = A->off[Nr * Nz];
+   int sides_ng[4] = {d->NgRa, d->NgRb, d->NgZa, d->NgZb};
+   for (int s = 0; s < 4; ++s) {
+      int kind = bc_kind_of_module(BOUNDARY, s);
+      if (kind < 0) return host_fail("boundary module '%s' has no device implementation", BOUNDARY);
+      A->bc[s] = sides_ng[s] > 0 ? kind : DH_BC_NONE;
+   }
+   A->diag_kind = diag_kind_of_module(ANALYSIS);
+   if (A->diag_kind < 0)
+      return host_fail("analysis module '%s' has no device implementation", ANALYSIS);
+   if (d->Npl > DH_MAX_PLANETS) return host_fail("at most %d planets", DH_MAX_PLANETS);
+   dh_grid g;
+   g.Nr = Nr; g.Nz = Nz;
+   g.NgRa = d->NgRa; g.NgRb = d->NgRb; g.NgZa = d->NgZa; g.NgZb = d->NgZb;
+   g.Np = Np;
+   g.r_jph = d->r_jph - 1;
+   g.z_kph = d->z_kph - 1;
+   g.phi_max = d->phi_max;
+   dh_params P;
+   fill_params(d, &P, A->bc);
+   if (dh_create(&g, &P, &A->ctx)) return host_fail("dh_create: %s", dh_last_error());
+   if (dh_enable_graphs(A->ctx, A->graphs != 0))
+      return host_fail("dh_enable_graphs: %s", dh_last_error());
+   if (dh_autotune(A->ctx, A->autotune > 0 ? A->autotune : 0))
+      return host_fail("dh_autotune: %s", dh_last_error());
+   long nt = A->ntot;
+   double *prim = (double *)malloc(sizeof(double) * nt * NUM_Q);
+   double *cons = (double *)malloc(sizeof(double) * nt * NUM_Q);
+   double *piph = (double *)malloc(sizeof(double) * nt);
+   double *dphi = (double *)malloc(sizeof(double) * nt);
+   double *wiph = (double *)malloc(sizeof(double) * nt);
+   double *Phi = NUM_FACES > 0 ? (double *)malloc(sizeof(double) * nt * NUM_FACES) : NULL;
+   for (int a = 0; a < Nr * Nz; ++a)
+      for (int i = 0; i < Np[a]; ++i) {
+         long c = A->off[a] + i;
+         const struct cell *cl = &d->theCells[a][i];
+         for (int q = 0; q < NUM_Q; ++q) {
+            prim[q * nt + c] = cl->prim[q];
+            cons[q * nt + c] = cl->cons[q];
+         }
+         for (int f = 0; f < NUM_FACES; ++f) Phi[f * nt + c] = cl->Phi[f];
+         piph[c] = cl->piph;
+         dphi[c] = cl->dphi;
+         wiph[c] = cl->wiph;
+      }
+   int rc = dh_upload_state(A->ctx, prim, cons, piph, dphi, wiph, Phi);
+   free(prim); free(cons); free(piph); free(dphi); free(wiph); free(Phi);
+   if (rc) return host_fail("dh_upload_state: %s", dh_last_error());
+   A->host_dirty = 0;
+   return 0;
+}
+
+static struct disco_accel *device_of(struct domain *d)
+{
+   struct disco_accel *A = accel_find(d);
+   if (A && A->ctx) return A;
+   if (attach_reference_domain(d)) exit(1);
+   return accel_find(d);
+}
+
+/* getmindt, misc.c:39-84 */
+static double device_getmindt(struct domain *d)
+{
+   struct disco_accel *A = device_of(d);
    double dt = 0.0;
    if (dh_getmindt(A->ctx, &dt)) {
       host_fail("dh_getmindt: %s", dh_last_error());
       exit(1);
    }
    return dt;
+}
+double getmindt(struct domain *d) { return device_getmindt(d); }
+/* The same under the name a binding that keeps the reference's misc.o (for
+ * its set-up functions) links main.c's call to, with -Wl,--wrap=getmindt. */
+double __wrap_getmindt(struct domain *d) { return device_getmindt(d); }
+
+/* Whether the reference's possiblyOutput (domain.c:316-345), called next
+ * in main.c's loop or after it, will read the cells: a report or checkpoint
+ * is due, or the run ends. */
+static int output_due(const struct domain *d)
+{
+   if (d->final_step || d->check_plz) return 1;
+   double t = d->t, t_min = d->t_init, t_fin = d->t_fin;
+   int LogOut = d->theParList.Out_LogTime;
+   int n0 = (int)(t * d->N_rpt / t_fin);
+   if (LogOut) n0 = (int)(d->N_rpt * log(t / t_min) / log(t_fin / t_min));
+   if (d->nrpt < n0) return 1;
+   n0 = (int)(t * d->N_chk / t_fin);
+   if (LogOut) n0 = (int)(d->N_chk * log(t / t_min) / log(t_fin / t_min));
+   if (d->nchk < n0 && d->N_chk > 0) return 1;
+   return 0;
 }
 
 /* planet_RK_adjust (planet.c:204-212) and the planet part of clean_pi
@@ -853,7 +987,7 @@ static void planets_clean(struct domain *d)
  * device context. */
 int disco_step_begin(struct domain *d, double dt)
 {
-   struct disco_accel *A = (struct disco_accel *)d->accel;
+   struct disco_accel *A = accel_find(d);
    double pl0[DH_MAX_PLANETS * DH_PLANET_STRIDE], pl1[DH_MAX_PLANETS * DH_PLANET_STRIDE];
    for (int p = 0; p < d->Npl; ++p) {   /* planet_RK_copy */
       struct planet *pl = d->thePlanets + p;
@@ -875,7 +1009,7 @@ int disco_step_begin(struct domain *d, double dt)
  * (timestep.c:28-39) and the step count. */
 void disco_step_end(struct domain *d, double dt)
 {
-   struct disco_accel *A = (struct disco_accel *)d->accel;
+   struct disco_accel *A = accel_find(d);
    d->t += dt;
    if (d->theParList.Timestep != 1 && !planet_motion_analytic()) {
       planets_rk_adjust(d, 0.5);
@@ -890,7 +1024,7 @@ void disco_step_end(struct domain *d, double dt)
  * the device's layout, the Analysis module's kind, and dt added to the
  * averaging time as add_diagnostics does. */
 void disco_pack_planets(struct domain *d, double *out) { planets_pack(d, out); }
-int disco_diag_kind(struct domain *d) { return ((struct disco_accel *)d->accel)->diag_kind; }
+int disco_diag_kind(struct domain *d) { return (accel_find(d))->diag_kind; }
 int disco_npl(struct domain *d) { return d->Npl; }
 void disco_add_diag_time(struct domain *d, double dt) { d->theTools.t_avg += dt; }
 
@@ -899,7 +1033,7 @@ void disco_add_diag_time(struct domain *d, double dt) { d->theTools.t_avg += dt;
  * sources), then the whole step runs on the device. */
 void timestep(struct domain *d, double dt)
 {
-   struct disco_accel *A = (struct disco_accel *)d->accel;
+   struct disco_accel *A = device_of(d);
    if (disco_step_begin(d, dt)) exit(1);
    if (dh_timestep(A->ctx, dt)) {
       host_fail("dh_timestep: %s", dh_last_error());
@@ -907,6 +1041,7 @@ void timestep(struct domain *d, double dt)
    }
    disco_step_end(d, dt);
    add_diagnostics(d, dt);
+   if (A->bound && output_due(d) && disco_sync_to_host(d)) exit(1);
 }
 
 /* check_dt, domain.c:284-311, under its own name (serial; the abort/latest files as in the
@@ -925,12 +1060,12 @@ void disco_check_dt(struct domain *d, double *dt)
 
 void disco_free(struct domain *d)
 {
-   struct disco_accel *A = (struct disco_accel *)d->accel;
+   struct disco_accel *A = accel_find(d);
    if (A) {
       if (A->ctx) dh_destroy(A->ctx);
       free(A->off);
       free(A);
-      d->accel = NULL;
+      accel_forget(d);
    }
    if (d->theCells) {
       for (int a = 0; a < d->Nr * d->Nz; ++a) free(d->theCells[a]);
@@ -947,7 +1082,7 @@ void disco_free(struct domain *d)
 /* Writes the device state as a DSTATE01 file (disco_amd/state_io.py). */
 int disco_write_dstate(struct domain *d, const char *path)
 {
-   struct disco_accel *A = (struct disco_accel *)d->accel;
+   struct disco_accel *A = accel_find(d);
    long nt = A->ntot;
    int Nr = d->Nr, Nz = d->Nz;
    int nf = NUM_FACES == 5 ? 5 : 0;
@@ -991,7 +1126,7 @@ int disco_write_dstate(struct domain *d, const char *path)
  * after the reference's restart. */
 int disco_restart_dstate(struct domain *d, const char *path)
 {
-   struct disco_accel *A = (struct disco_accel *)d->accel;
+   struct disco_accel *A = accel_find(d);
    long nt = A->ntot;
    int Nr = d->Nr, Nz = d->Nz;
    int nf = NUM_FACES == 5 ? 5 : 0;

@@ -8,9 +8,12 @@
  * names and meaning.  Compile-time sizes come from -DNUM_C, -DNUM_N and
  * -DCT_MODE exactly as in the reference's Makefile.
  *
+ * The struct layouts are the reference's (with USE_MPI=0), so the host
+ * layer links into a program whose other objects were built against the
+ * reference's own paul.h (tests/test_binding.py checks every offset).
  * What differs is ownership: the state lives on the GPU (libdisco_hip.so);
- * theCells is a host mirror filled by disco_sync_to_host(), and the `accel`
- * member at the end of struct domain holds the device context.
+ * theCells is a host mirror filled by disco_sync_to_host(), and the host
+ * layer keeps each domain's device context in a table of its own.
  */
 #ifndef PAUL
 #define PAUL
@@ -156,9 +159,6 @@ struct domain {
    int N_chk;
    int final_step;
    int check_plz;
-   /* Device context of the MI355X timestep (struct dh_ctx *), the host
-    * layer's bookkeeping (struct disco_accel *).  Not in the reference. */
-   void *accel;
 };
 
 struct cell {
@@ -171,11 +171,13 @@ struct cell {
    double piph;
    double dphi;
    double wiph;
-   double E[NUM_EDGES > 0 ? NUM_EDGES : 1];
-   double B[NUM_EDGES > 0 ? NUM_EDGES : 1];
-   double E_phi[NUM_AZ_EDGES > 0 ? NUM_AZ_EDGES : 1];
-   double Phi[NUM_FACES > 0 ? NUM_FACES : 1];
-   double RK_Phi[NUM_FACES > 0 ? NUM_FACES : 1];
+   /* zero-length without constrained transport (a GNU C extension), as
+    * in the reference, so the layout is the reference's for every CT_MODE */
+   double E[NUM_EDGES];
+   double B[NUM_EDGES];
+   double E_phi[NUM_AZ_EDGES];
+   double Phi[NUM_FACES];
+   double RK_Phi[NUM_FACES];
    double tempDoub;
    int real;
 };

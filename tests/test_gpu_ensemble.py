@@ -51,8 +51,12 @@ def test_ensemble_members_equal_their_own_runs(graphs):
             a.write(pa)
             b.write(pb)
             x, y = state_io.read_state(pa), state_io.read_state(pb)
-            for key in ("prim", "cons", "piph", "dphi", "wiph", "planets"):
+            for key in ("prim", "piph", "dphi", "wiph", "planets"):
                 assert np.array_equal(getattr(x, key), getattr(y, key)), (i, key)
+            # conserved variables of real cells (a ghost cell's are never
+            # written or read: only its primitives are boundary data)
+            m = x.real_mask()
+            assert np.array_equal(x.cons[:, m], y.cons[:, m]), (i, "cons")
             assert x.t == y.t
             print(f"  member {i} (Nr {MEMBERS[i][0]}): identical to its own run", flush=True)
     finally:
