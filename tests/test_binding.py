@@ -67,16 +67,34 @@ def _defs(c, output):
             f"-DNUM_C={c['num_c']}", "-DNUM_N=0", f"-DCT_MODE={c['ct']}"]
 
 
-def _build(cname, d):
-    """(whole reference program, bound program) of one module selection."""
+H5 = "/opt/conda"   # HDF5 (headers and library) of the reference's checkpoints
+
+
+def _build(cname, d, hdf5=False):
+    """(whole reference program, bound program) of one module selection;
+    with hdf5 the reference's own Output/h5out.c and Restart/h5in.c modules,
+    otherwise this test's DSTATE01 Output plugin."""
     c = CASES[cname]
-    cflags = ["gcc", "-O3", "-ffp-contract=off", "-w", f"-I{REF}", f"-I{os.path.join(REPO, 'disco_amd', 'csrc')}",
-              *_defs(c, "dstate_output")]
+    output, restart = ("h5out", "h5in") if hdf5 else ("dstate_output", "no_restart")
+    defs = [f for f in _defs(c, output) if not f.startswith("-DRESTART")]
+    cflags = ["gcc", "-O3", "-ffp-contract=off", "-w", f"-I{REF}",
+              f"-I{os.path.join(REPO, 'disco_amd', 'csrc')}", *defs, f'-DRESTART="{restart}"']
+    libs = ["-lm"]
+    if hdf5:
+        cflags.append(f"-I{H5}/include")
+        # by path, and the system's libraries first for what the GPU runtime
+        # needs (HDF5's directory holds an older libstdc++)
+        libs = [f"{H5}/lib/libhdf5.so", "-Wl,-rpath-link,/usr/lib/x86_64-linux-gnu",
+                f"-Wl,-rpath,{H5}/lib", "-lm"]
+    out_src = (f"{REF}/Output/h5out.c" if hdf5
+               else os.path.join(HERE, "binding", "dstate_output.c"))
     plugins = [c["initial"], c["boundary"], c["planets"], f"{REF}/Diagnostics/{c['analysis']}.c",
                f"{REF}/Hydro/Metric/{c['metric']}.c", f"{REF}/Hydro/{c['hydro']}.c",
-               f"{REF}/{c['hlld']}.c", os.path.join(HERE, "binding", "dstate_output.c"), *c["extra"]]
+               f"{REF}/{c['hlld']}.c", out_src, *c["extra"]]
+    core = [f for f in CORE if f != "Restart/no_restart.c"] + [f"Restart/{restart}.c"
+                                                                if hdf5 else "Restart/no_restart.c"]
     whole = os.path.join(d, "disco_reference")
-    subprocess.run(cflags + [f"{REF}/{f}" for f in CORE] + plugins + ["-lm", "-o", whole],
+    subprocess.run(cflags + [f"{REF}/{f}" for f in core] + plugins + libs + ["-o", whole],
                    check=True)
     # the host layer as the binding links it: disco_host.c alone (the
     # plugins come from the program), on the engine's host build
@@ -91,9 +109,9 @@ def _build(cname, d):
                     f"-L{os.path.dirname(engine)}", "-ldh_cpu", "-lm",
                     f"-Wl,-rpath,{os.path.dirname(engine)}"], check=True)
     bound = os.path.join(d, "disco_bound")
-    kept = [f"{REF}/{f}" for f in CORE if f not in REPLACED]
+    kept = [f"{REF}/{f}" for f in core if f not in REPLACED]
     subprocess.run(cflags + kept + plugins + ["-rdynamic", "-Wl,--wrap=getmindt", f"-L{d}",
-                                              f"-ldisco_host_{cname}", f"-Wl,-rpath,{d}", "-lm",
+                                              f"-ldisco_host_{cname}", f"-Wl,-rpath,{d}", *libs,
                                               "-o", bound], check=True)
     return whole, bound
 
@@ -171,3 +189,69 @@ def test_plugin_abi_structs_have_the_reference_layout(sizes):
     ours = _layout(os.path.join(REPO, "include", "disco"), flags)
     ref = _layout(REF, ["-DUSE_MPI=0", *flags])
     assert ours == ref
+
+
+def _cells(h5, d):
+    """(T, Grid/Np[Nz][Nr], Data/Cells) of a checkpoint in the reference's
+    layout (cells annulus by annulus, k outer, j inner)."""
+    reader = os.path.join(d, "h5cells")
+    if not os.path.exists(reader):
+        subprocess.run(["gcc", "-O2", f"-I{H5}/include", os.path.join(HERE, "binding", "h5cells.c"),
+                        f"-L{H5}/lib", "-lhdf5", f"-Wl,-rpath,{H5}/lib", "-o", reader], check=True)
+    raw = h5 + ".bin"
+    subprocess.run([reader, h5, raw], check=True)
+    buf = open(raw, "rb").read()
+    T = np.frombuffer(buf, np.float64, 1, 0)[0]
+    nn = np.frombuffer(buf, np.int64, 2, 8)
+    o = 24
+    Np = np.frombuffer(buf, np.int64, int(nn[0] * nn[1]), o).reshape(nn)
+    o += Np.nbytes
+    n = np.frombuffer(buf, np.int64, 2, o)
+    return T, Np, np.frombuffer(buf, np.float64, int(n[0] * n[1]), o + 16).reshape(n)
+
+
+@pytest.mark.skipif(not os.path.exists(f"{H5}/include/hdf5.h"), reason="no HDF5 headers")
+@pytest.mark.parametrize("cname,over,n1,n2", [
+    ("euler_yee", {"Num_R": 24}, 4, 4),
+    ("mhd_kep", {"Num_R": 12, "Num_Z": 6}, 3, 3),
+])
+def test_checkpoints_in_the_reference_layout_restart_across(cname, over, n1, n2):
+    """Checkpoints in the reference's HDF5 layout (Output/h5out.c) and
+    restarts from them (Restart/h5in.c), across the two programs: the drop-in
+    continues a run from the reference's checkpoint, and the reference
+    continues one from the drop-in's, each ending where the reference's own
+    split run ends, bit for bit."""
+    cfg = configs.get(cname)
+    d = tempfile.mkdtemp(prefix="bindh5_")
+    progs = dict(zip(("whole", "bound"), _build(cname, d, hdf5=True)))
+    h = 1e-3
+    first = cfg.par_dict(**over, Max_DT=h, T_End=n1 * h * (1 - 1e-9), Num_Reports=0,
+                         Num_Checkpoints=0)
+    second = dict(first, T_End=(n1 + n2) * h * (1 - 1e-9), Restart=1)
+    ends = {}
+    for a in ("whole", "bound"):           # who writes the checkpoint
+        wa = os.path.join(d, f"{a}_first")
+        state_io.write_par(wa, first)
+        r = subprocess.run([progs[a]], cwd=wa, capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, (a, r.stderr[-2000:])
+        for b in ("whole", "bound"):       # who continues from it
+            wb = os.path.join(d, f"{a}_then_{b}")
+            state_io.write_par(wb, second)
+            os.link(os.path.join(wa, "output.h5"), os.path.join(wb, "input.h5"))
+            r = subprocess.run([progs[b]], cwd=wb, capture_output=True, text=True, timeout=600)
+            assert r.returncode == 0, (a, b, r.stderr[-2000:])
+            ends[(a, b)] = _cells(os.path.join(wb, "output.h5"), d)
+    T0, Np, ref = ends[("whole", "whole")]
+    assert abs(T0 - (n1 + n2) * h) < 1e-9 and ref.shape[1] == cfg.nq + cfg.nf + 1
+    # real cells: the ghost annuli (two on every side with a boundary) carry
+    # boundary data; the reference's own restart leaves parts of its ghost
+    # cells' edge and flux records unset, so they are compared on real cells
+    nz, nr = Np.shape
+    gr = [0 if first.get(k) else 2 for k in ("NoBC_Rmin", "NoBC_Rmax")]
+    gz = [0 if first.get(k) or nz == 1 else 2 for k in ("NoBC_Zmin", "NoBC_Zmax")]
+    real = np.zeros((nz, nr), bool)
+    real[gz[0]:nz - gz[1], gr[0]:nr - gr[1]] = True
+    m = np.repeat(real.ravel(), Np.ravel())
+    for k, (T, Np_k, cells) in ends.items():
+        assert T == T0 and np.array_equal(Np_k, Np), k
+        assert np.array_equal(cells[m], ref[m]), (k, np.abs(cells[m] - ref[m]).max())
