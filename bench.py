@@ -142,10 +142,10 @@ def kernel_bytes(name: str, nq: int, mhd: bool, nt: int, nr: int, nfr: int, nfz:
                 + (3 * d * nf if mhd else 0))
     if name == "edges_collect":
         return d * 21 * nt + 2 * steps + i * nt + 3 * d * nfa
-    if name in ("face_emf_r", "face_emf_z"):
-        return nf * (2 * i + 1 + 2 * d) + d * 18 * nt + i * nt
     if name == "edges_final":
-        return d * 25 * nt + 2 * steps + i * nt + 3 * d * nfa
+        # Epre (16), wiph, piph and dphi in, Eo and Eown (4 + 4) out per cell;
+        # per owned face its cells and orientation and its E_phi in
+        return d * 27 * nt + 2 * steps + i * nt + nfa * (d + 2 * i + 1)
     if name == "update_B":
         # Phi in and out, rkPhi on the second substep only (mean 2.5 per cell)
         return d * (12.5 + 4) * nt + 2 * steps + i * nt + nfa * (2 * i + 1)
@@ -612,7 +612,7 @@ def main(argv=None):
                     help="grmhd2 cons2prim's stopping rule")
     ap.add_argument("--graphs", choices=["on", "off"], default="on",
                     help="record the RK2 timestep once as a HIP graph and replay it")
-    ap.add_argument("--autotune", type=int, default=8,
+    ap.add_argument("--autotune", type=int, default=12,
                     help="steps of launch tuning before the warmup (0: untuned launches)")
     ap.add_argument("--ensemble-mode", choices=["packed", "streams"], default="packed",
                     help="disk_ensemble: one launch per kernel for all members, or a "

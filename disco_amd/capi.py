@@ -162,6 +162,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "dh_ens_enable_graphs": ([vp, C.c_int], C.c_int),
         "dh_ens_sync": ([vp], C.c_int),
         "dh_read_tuning": ([vp, C.c_int, C.c_void_p, _ip, _ip, _dp, _dp, _ip], C.c_int),
+        "dh_read_tuning_sweep": ([vp, C.c_char_p, C.c_int, _ip, _dp, _ip], C.c_int),
         "dh_last_error": ([], C.c_char_p),
     }
     for name, (args, res) in sig.items():
@@ -177,7 +178,7 @@ EXPORTED_SYMBOLS = ["dh_create", "dh_destroy", "dh_sizes", "dh_upload_state",
                     "dh_read_timers", "dh_add_diagnostics", "dh_get_diagnostics",
                     "dh_zero_diagnostics", "dh_avg_diagnostics", "dh_report", "dh_set_c2p_model",
                     "dh_c2p_harvest", "dh_c2p_predict", "dh_c2p_stats", "dh_c2p_train",
-                    "dh_c2p_grad", "dh_c2p_grad_noalloc", "dh_autotune", "dh_read_tuning",
+                    "dh_c2p_grad", "dh_c2p_grad_noalloc", "dh_autotune", "dh_read_tuning", "dh_read_tuning_sweep",
                     "dh_ens_create", "dh_ens_destroy", "dh_ens_getmindt", "dh_ens_timestep",
                     "dh_ens_add_diagnostics", "dh_ens_enable_graphs", "dh_ens_sync",
                     "dh_last_error"]
@@ -186,8 +187,8 @@ VARIANT_NAMES = {0: "storage order, round-robin XCDs", 1: "storage order, XCD-co
 
 
 def read_tuning(lib, ctx):
-    """{kernel: {variant, strip, ms_chosen, ms_default}} of a context's tuner
-    (dh_read_tuning)."""
+    """{kernel: {variant, order, ms_chosen, ms_default, ms_variants}} of a
+    context's tuner (dh_read_tuning, dh_read_tuning_sweep)."""
     cap = 64
     names = (C.c_char * 48 * cap)()
     var = (C.c_int * cap)()
@@ -198,12 +199,20 @@ def read_tuning(lib, ctx):
     if lib.dh_read_tuning(ctx, cap, names, var, strip, msc, msd, C.byref(n)):
         raise DhError(f"dh_read_tuning: {lib.dh_last_error().decode()}")
     out = {}
+    sw_strip = (C.c_int * 16)()
+    sw_ms = (C.c_double * 16)()
+    nvar = C.c_int()
     for i in range(n.value):
         v = var[i]
-        out[bytes(names[i]).split(b"\0")[0].decode()] = {
+        name = bytes(names[i]).split(b"\0")[0].decode()
+        if lib.dh_read_tuning_sweep(ctx, name.encode(), 16, sw_strip, sw_ms, C.byref(nvar)):
+            raise DhError(f"dh_read_tuning_sweep: {lib.dh_last_error().decode()}")
+        sweep = {VARIANT_NAMES.get(u, f"strips of {sw_strip[u]} layers"): sw_ms[u]
+                 for u in range(min(nvar.value, 16))}
+        out[name] = {
             "variant": v,
             "order": VARIANT_NAMES.get(v, f"strips of {strip[i]} layers"),
-            "ms_chosen": msc[i], "ms_default": msd[i]}
+            "ms_chosen": msc[i], "ms_default": msd[i], "ms_variants": sweep}
     return out
 
 

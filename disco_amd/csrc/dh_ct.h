@@ -14,9 +14,10 @@
 // The reference updates these with sequential scatter loops over faces and
 // edges.  Here every output slot is computed by its owner (dh_kernels.h):
 //   KEdgesCollect  edge EMFs after the Riemann solves     (riemann.c:66-211)
-//   KFaceEmf       face-interpolated edge EMFs, with the phi-neighbour
-//                  averages formed where they are read    (bfields.c:400-531)
-//   KEdgesFinal    owner edges + advective subtraction    (bfields.c:533-690)
+//   KEdgesFinal    owner edges: the phi-neighbour averages and each owned
+//                  face's interpolated edge EMF, formed where they are read
+//                  (bfields.c:400-531), then the advective subtraction
+//                  (bfields.c:533-690)
 //   KUpdateB       Phi += E dl dt of the face edges       (bfields.c:313-382)
 //   KLineSteps, KEdgeAdjust  E_phi edges: each face's merge steps on its two
 //                  edge lines, then its segments; a merge of two face lists
@@ -94,6 +95,94 @@ DHD CellRanges load_ranges(const Mesh &m, int dim, const int *rng, const int *fI
                        &r.out_t0, &r.out_t1);
    }
    return r;
+}
+
+// A cell's faces in both directions as one sequence in the reference's
+// order: r inner head, r inner tail, r outer head, r outer tail, then the
+// same four for z (ranges of absolute face indices, empty ones lo > hi).
+// Position v of the sequence maps to its range and face through compares
+// unrolled over the eight ranges, so that the ranges stay in registers.
+// A kernel walks the sequence one face ahead of the face it adds: the next
+// face's loads are in flight while the current one is summed, and the sums
+// keep their order.
+struct FaceSeq {
+   int lo[8], end[8];  // range k holds positions end[k-1] .. end[k]-1
+   int n;
+   DHD int face(int v, int *k) const
+   {
+      int f = 0, b = 0;
+      *k = 0;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+         if (v >= b && v < end[r]) {
+            f = lo[r] + (v - b);
+            *k = r;
+         }
+         b = end[r];
+      }
+      return f;
+   }
+};
+
+// Ranges r0 .. r0+3 of a sequence from one direction's CellRanges (the
+// inner pair's when `in`, the outer pair's when `out`, else empty); b
+// counts the positions so far.
+DHD void seq_put(FaceSeq &s, int r0, const CellRanges &cr, bool in, bool out, int &b)
+{
+   int lo[4] = {1, 1, 1, 1}, hi[4] = {0, 0, 0, 0};
+   if (in) {
+      lo[0] = cr.base_in + cr.in_h0;
+      hi[0] = cr.base_in + cr.in_h1;
+      lo[1] = cr.base_in + cr.in_t0;
+      hi[1] = cr.base_in + cr.in_t1;
+   }
+   if (out) {
+      lo[2] = cr.base_out + cr.out_h0;
+      hi[2] = cr.base_out + cr.out_h1;
+      lo[3] = cr.base_out + cr.out_t0;
+      hi[3] = cr.base_out + cr.out_t1;
+   }
+#pragma unroll
+   for (int r = 0; r < 4; ++r) {
+      int cnt = hi[r] - lo[r] + 1;
+      b += cnt > 0 ? cnt : 0;
+      s.lo[r0 + r] = lo[r];
+      s.end[r0 + r] = b;
+   }
+}
+
+// One direction's faces (ranges 0-3; 4-7 empty).
+DHD FaceSeq pair_seq(const CellRanges &cr)
+{
+   FaceSeq s;
+   CellRanges none;
+   none.in = none.out = false;
+   int b = 0;
+   seq_put(s, 0, cr, cr.in, cr.out, b);
+   seq_put(s, 4, none, false, false, b);
+   s.n = b;
+   return s;
+}
+
+// Walks a face sequence one face ahead: ld(f, k, d) loads face f (of range
+// k) into d and use(d, k) consumes it, in the sequence's order; the next
+// face's loads are issued before the current face is used, so that two
+// faces' loads are in flight where a plain loop would wait for each.
+template <class D, class L, class U>
+DHD void walk_ahead(const FaceSeq &s, L ld, U use)
+{
+   if (s.n == 0) return;
+   int k;
+   D cur, nxt;
+   ld(s.face(0, &k), k, cur);
+   for (int v = 1;; ++v) {
+      int kc = k;
+      bool more = v < s.n;
+      if (more) ld(s.face(v, &k), k, nxt);
+      use(cur, kc);
+      if (!more) break;
+      cur = nxt;
+   }
 }
 
 // The track loops of update_B_fluxes and check_flipped/flip_fluxes

@@ -487,7 +487,7 @@ struct TimerRow {
 // (strips of annuli, XCD-contiguous).  All give the same results.
 struct TuneRow {
    std::string name;
-   int nvar = 0, next = 0, chosen = -1, dflt = 0;
+   int nvar = 0, chosen = -1, dflt = 0;
    double ms[2 + MAX_ORDERS] = {0};
    int n[2 + MAX_ORDERS] = {0};
    int strip[MAX_ORDERS] = {0};
@@ -500,10 +500,13 @@ struct TunePending {
 struct HipExec {
    hipStream_t stream = nullptr;
    bool timers = false;
-   // Tuner (dh_autotune): while tune_left timesteps remain, every launch of
-   // a tunable kernel takes its next variant in turn and is timed; then
+   // Tuner (dh_autotune): while tune_left timesteps remain, a tunable
+   // kernel runs in variant (tune_step mod its variant count) for all its
+   // launches of the step, so that every variant is timed over whole steps
+   // (the two substeps' launches of a kernel differ in cost, and variants
+   // dealt out launch by launch would each meet only one of them); then
    // each kernel keeps its fastest variant.
-   int tune_left = 0;
+   int tune_left = 0, tune_step = 0;
    std::vector<TuneRow> tunes;
    std::vector<TunePending> tpend;
    // Recorded timestep (dh_enable_graphs): replayed while the key it was
@@ -581,7 +584,7 @@ struct HipExec {
       TuneRow &t = tunes[row];
       int v = t.chosen >= 0 ? t.chosen : t.dflt;
       bool timing = tune_left > 0;
-      if (timing) v = t.next++ % t.nvar;
+      if (timing) v = tune_step % t.nvar;
       hipEvent_t e0{}, e1{};
       if (timing) {
          HIPCHK(hipEventCreate(&e0));
@@ -619,6 +622,7 @@ struct HipExec {
          (void)hipEventDestroy(p.e1);
       }
       tpend.clear();
+      ++tune_step;
       if (--tune_left == 0)
          for (auto &t : tunes) {
             int best = -1;
@@ -1716,9 +1720,9 @@ int dh_autotune(dh_ctx *c, int steps)
    if (steps < 0) return fail(DH_EINVAL, "negative step count");
    c->x.drop_graph();
    c->x.tune_left = steps;
+   c->x.tune_step = 0;
    if (steps > 0)
       for (auto &t : c->x.tunes) {
-         t.next = 0;
          for (int v = 0; v < t.nvar; ++v) t.ms[v] = 0.0, t.n[v] = 0;
       }
    return DH_OK;
@@ -1742,6 +1746,23 @@ int dh_read_tuning(dh_ctx *c, int cap, char (*names)[48], int *variant, int *str
       ++n;
    }
    *nrows = n;
+   return DH_OK;
+}
+
+int dh_read_tuning_sweep(dh_ctx *c, const char *name, int cap, int *strip, double *ms,
+                         int *nvar)
+{
+   if (!c || !name || cap < 0 || !nvar) return fail(DH_EINVAL, "bad arguments");
+   *nvar = 0;
+   for (auto &t : c->x.tunes) {
+      if (t.name != name) continue;
+      *nvar = t.nvar;
+      for (int v = 0; v < t.nvar && v < cap; ++v) {
+         if (strip) strip[v] = v >= 2 ? t.strip[v - 2] : 0;
+         if (ms) ms[v] = t.n[v] ? t.ms[v] / t.n[v] : -1.0;
+      }
+      break;
+   }
    return DH_OK;
 }
 

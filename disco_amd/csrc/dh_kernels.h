@@ -249,6 +249,33 @@ struct KWcellAvg {
 // Generalised-minmod slope along phi (plm.c:6-51)
 // ---------------------------------------------------------------------------
 
+// The limited phi slopes of cell c from its two phi neighbours.
+template <int NQ>
+DHD void plm_phi_cell(const Mesh &m, double PLM, const double *prim, const double *dphi, long c,
+                      double *g)
+{
+   size_t nt = (size_t)m.ntot;
+   int a = m.cjk[c];
+   int c0 = m.off[a], np = m.Np[a];
+   int i = (int)(c - c0);
+   long cL = i == 0 ? c0 + np - 1 : c - 1;
+   long cR = i == np - 1 ? c0 : c + 1;
+   double dpL = dphi[cL], dpC = dphi[c], dpR = dphi[cR];
+#pragma unroll
+   for (int q = 0; q < NQ; ++q) {
+      double pL = prim[q * nt + cL];
+      double pC = prim[q * nt + c];
+      double pR = prim[q * nt + cR];
+      double sL = pC - pL;
+      sL /= .5 * (dpC + dpL);
+      double sR = pR - pC;
+      sR /= .5 * (dpR + dpC);
+      double sC = pR - pL;
+      sC /= .5 * (dpL + dpR) + dpC;
+      g[q] = minmod(PLM * sL, sC, PLM * sR);
+   }
+}
+
 template <int NQ>
 struct KPlmPhi {
    Mesh m;
@@ -258,25 +285,10 @@ struct KPlmPhi {
    DHD void operator()(long c) const
    {
       size_t nt = (size_t)m.ntot;
-      int a = m.cjk[c];
-      int c0 = m.off[a], np = m.Np[a];
-      int i = (int)(c - c0);
-      long cL = i == 0 ? c0 + np - 1 : c - 1;
-      long cR = i == np - 1 ? c0 : c + 1;
-      double dpL = dphi[cL], dpC = dphi[c], dpR = dphi[cR];
+      double g[NQ];
+      plm_phi_cell<NQ>(m, PLM, prim, dphi, c, g);
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-         double pL = prim[q * nt + cL];
-         double pC = prim[q * nt + c];
-         double pR = prim[q * nt + cR];
-         double sL = pC - pL;
-         sL /= .5 * (dpC + dpL);
-         double sR = pR - pC;
-         sR /= .5 * (dpR + dpC);
-         double sC = pR - pL;
-         sC /= .5 * (dpL + dpR) + dpC;
-         gradp[q * nt + c] = minmod(PLM * sL, sC, PLM * sR);
-      }
+      for (int q = 0; q < NQ; ++q) gradp[q * nt + c] = g[q];
    }
 };
 
@@ -445,27 +457,43 @@ struct KGrad {
       double sum[NQ], best[NQ];
       unsigned pos = 0, neg = 0;
    };
-   // Face f of a pair whose distances from the centroids are dxL, dxR; the
-   // cell is its L (own_left) or R, with state (pc, gc).  The one-sided slope
-   // is S = (W_R - W_L)/(dx_L + dx_R) of the phi-reconstructed states at the
-   // face's phi centre (plm.c:84-110).
-   DHD void add_face(int f, bool own_left, long c, const double *pc, const double *gc, double dxL,
-                     double dxR, Acc &A) const
+   // What one face brings from the other cell o: the face's phi centre and
+   // area, o's phi extent and its state (prim, gradp).
+   struct FaceIn {
+      long o;
+      double phic, dA, po, dpo, p[NQ], g[NQ];
+   };
+   DHD void load_face(int f, bool own_left, FaceIn &d) const
    {
       size_t nt = (size_t)m.ntot;
-      long o = own_left ? F.R[f] : F.L[f];
-      long cL = own_left ? c : o, cR = own_left ? o : c;
-      double phic = F.phic[f];
-      double dA = F.dA[f];
-      double dpL = get_sdp(phic, piph[cL] - .5 * dphi[cL], m.Pmax);
-      double dpR = get_sdp(piph[cR] - .5 * dphi[cR], phic, m.Pmax);
+      d.o = own_left ? F.R[f] : F.L[f];
+      d.phic = F.phic[f];
+      d.dA = F.dA[f];
+      d.po = piph[d.o];
+      d.dpo = dphi[d.o];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-         double po = prim[q * nt + o], go = gradp[q * nt + o];
-         double WL = own_left ? pc[q] + dpL * gc[q] : po + dpL * go;
-         double WR = own_left ? po - dpR * go : pc[q] - dpR * gc[q];
+         d.p[q] = prim[q * nt + d.o];
+         d.g[q] = gradp[q * nt + d.o];
+      }
+   }
+   // A face of a pair whose distances from the centroids are dxL, dxR; the
+   // cell is its L (own_left) or R, with state (pc, gc) and phi extent
+   // (pcc, dpc).  The one-sided slope is S = (W_R - W_L)/(dx_L + dx_R) of
+   // the phi-reconstructed states at the face's phi centre (plm.c:84-110).
+   DHD void add_face(const FaceIn &d, bool own_left, double pcc, double dpc, const double *pc,
+                     const double *gc, double dxL, double dxR, Acc &A) const
+   {
+      double pL = own_left ? pcc : d.po, dL = own_left ? dpc : d.dpo;
+      double pR = own_left ? d.po : pcc, dR = own_left ? d.dpo : dpc;
+      double dpL = get_sdp(d.phic, pL - .5 * dL, m.Pmax);
+      double dpR = get_sdp(pR - .5 * dR, d.phic, m.Pmax);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+         double WL = own_left ? pc[q] + dpL * gc[q] : d.p[q] + dpL * d.g[q];
+         double WR = own_left ? d.p[q] - dpR * d.g[q] : pc[q] - dpR * gc[q];
          double S = (WR - WL) / (dxR + dxL);
-         A.sum[q] += S * dA;
+         A.sum[q] += S * d.dA;
          double v = PLM * S;
          if (fabs(v) < fabs(A.best[q])) A.best[q] = v;
          if (S > 0.0) A.pos |= 1u << q;
@@ -484,6 +512,7 @@ struct KGrad {
       double pc[NQ], gc[NQ];
       load_q<NQ>(prim, nt, c, pc);
       load_q<NQ>(gradp, nt, c, gc);
+      double pcc = piph[c], dpc = dphi[c];
       Acc A;
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
@@ -491,19 +520,16 @@ struct KGrad {
          A.best[q] = HUGE_VAL;
       }
       // the reference's order: the inner pair (the cell is R), then the outer
-      // pair (the cell is L), each head before tail
-      if (cr.in) {
-         for (int f = cr.base_in + cr.in_h0; f <= cr.base_in + cr.in_h1; ++f)
-            add_face(f, false, c, pc, gc, dxLi, dxRi, A);
-         for (int f = cr.base_in + cr.in_t0; f <= cr.base_in + cr.in_t1; ++f)
-            add_face(f, false, c, pc, gc, dxLi, dxRi, A);
-      }
-      if (cr.out) {
-         for (int f = cr.base_out + cr.out_h0; f <= cr.base_out + cr.out_h1; ++f)
-            add_face(f, true, c, pc, gc, dxLo, dxRo, A);
-         for (int f = cr.base_out + cr.out_t0; f <= cr.base_out + cr.out_t1; ++f)
-            add_face(f, true, c, pc, gc, dxLo, dxRo, A);
-      }
+      // pair (the cell is L), each head before tail; one face ahead (the
+      // kernel waits on its gathers: two faces in flight per thread beat
+      // the occupancy the extra registers cost)
+      walk_ahead<FaceIn>(
+          pair_seq(cr), [&](int f, int r, FaceIn &d) { load_face(f, r >= 2, d); },
+          [&](const FaceIn &d, int r) {
+             bool own_left = r >= 2;
+             add_face(d, own_left, pcc, dpc, pc, gc, own_left ? dxLo : dxLi,
+                      own_left ? dxRo : dxRi, A);
+          });
       double phip = piph[c];
       double phim = phip - dphi[c];
       double dp = get_dp(phip, phim, m.Pmax);
@@ -581,15 +607,26 @@ struct KFluxPhi {
    const double *dtp;
    const double *prim, *gradp, *gradr, *gradz, *piph, *dphi, *wiph;
    double *dFp, *emf_phi;
+   DHD bool in_range(int j, int k) const { return phi_in_range(m, IsMHD<H>::value && P.ct, j, k); }
    DHD void operator()(long c) const
    {
-      constexpr bool mhd = IsMHD<H>::value;
       int a = m.cjk[c];
       int j = a % m.Nr, k = a / m.Nr;
-      if (!phi_in_range(m, mhd && P.ct, j, k)) return;
+      if (!in_range(j, k)) return;
       size_t nt = (size_t)m.ntot;
       int c0 = m.off[a], np = m.Np[a];
       long cR = (c - c0 == np - 1) ? c0 : c + 1;
+      double gC[NQ], gR[NQ];
+      load_q<NQ>(gradp, nt, c, gC);
+      load_q<NQ>(gradp, nt, cR, gR);
+      flux(c, cR, j, k, gC, gR);
+   }
+   // The face between c and its upper phi neighbour cR, whose phi slopes are
+   // gC and gR.
+   DHD void flux(long c, long cR, int j, int k, const double *gC, const double *gR) const
+   {
+      constexpr bool mhd = IsMHD<H>::value;
+      size_t nt = (size_t)m.ntot;
       double rp = m.r_jph[j], rm = m.r_jph[j - 1];
       double zp = m.z_kph[k], zm = m.z_kph[k - 1];
       double r = centroid_r(rp, rm);
@@ -598,8 +635,8 @@ struct KFluxPhi {
       double pL[NQ], pR[NQ];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
-         pL[q] = prim[q * nt + c] + .5 * gradp[q * nt + c] * dL;
-         pR[q] = prim[q * nt + cR] - .5 * gradp[q * nt + cR] * dR;
+         pL[q] = prim[q * nt + c] + .5 * gC[q] * dL;
+         pR[q] = prim[q * nt + cR] - .5 * gR[q] * dR;
       }
       if (mhd) {
          double Bp = .5 * (pL[BPP] + pR[BPP]);
@@ -1012,33 +1049,28 @@ DHD double edge_avg(const Mesh &m, const double *Epre, int h, int s, long c)
    return .5 * (X[(base + e - 2) * nt + cm] + X[(base + e) * nt + c]);
 }
 
-// avg_Efields, second part (bfields.c:443-531): the edge at a face's upper
-// end is the mean of its owner's value and the other cell's two corner
-// values interpolated in phi.
-template <int DIM>
-struct KFaceEmf {
-   Mesh m;
-   static constexpr int dim = DIM;
-   Faces F;
-   const double *piph, *dphi, *Epre;
-   double *fE, *fB;
-   DHD void operator()(long f) const
-   {
-      int L = F.L[f], R = F.R[f];
-      int LR = F.LR[f];
-      int c1 = LR == 0 ? L : R, c2 = LR == 0 ? R : L;
-      double dp1 = get_dp(piph[c2], piph[c1], m.Pmax);
-      double dp2 = dphi[c2] - dp1;
-      int eL = dim == 1 ? 1 : 5, eR = dim == 1 ? 0 : 4;
-      // LRtype 0: the face ends at L's upper edge; R's corners are interpolated
-      int eo = LR == 0 ? eL : eR, ec = LR == 0 ? eR : eL;
-      int co = LR == 0 ? L : R;
-      double Ea = (dp2 * edge_avg(m, Epre, 0, ec, c2) + dp1 * edge_avg(m, Epre, 0, ec + 2, c2)) / (dp1 + dp2);
-      double Ba = (dp2 * edge_avg(m, Epre, 1, ec, c2) + dp1 * edge_avg(m, Epre, 1, ec + 2, c2)) / (dp1 + dp2);
-      fE[f] = .5 * (edge_avg(m, Epre, 0, eo, co) + Ea);
-      fB[f] = .5 * (edge_avg(m, Epre, 1, eo, co) + Ba);
-   }
-};
+// avg_Efields, second part (bfields.c:443-531): the edge at the upper end of
+// face f (of direction dim) is the mean of its owner's value and the other
+// cell's two corner values interpolated in phi; E in *fE, B in *fB.  Each
+// face's edge is read by one cell, the face's owner (edges_post_face), which
+// forms it where it reads it.
+DHD void face_emf(const Mesh &m, int dim, const Faces &F, const double *piph,
+                  const double *dphi, const double *Epre, int f, double *fE, double *fB)
+{
+   int L = F.L[f], R = F.R[f];
+   int LR = F.LR[f];
+   int c1 = LR == 0 ? L : R, c2 = LR == 0 ? R : L;
+   double dp1 = get_dp(piph[c2], piph[c1], m.Pmax);
+   double dp2 = dphi[c2] - dp1;
+   int eL = dim == 1 ? 1 : 5, eR = dim == 1 ? 0 : 4;
+   // LRtype 0: the face ends at L's upper edge; R's corners are interpolated
+   int eo = LR == 0 ? eL : eR, ec = LR == 0 ? eR : eL;
+   int co = LR == 0 ? L : R;
+   double Ea = (dp2 * edge_avg(m, Epre, 0, ec, c2) + dp1 * edge_avg(m, Epre, 0, ec + 2, c2)) / (dp1 + dp2);
+   double Ba = (dp2 * edge_avg(m, Epre, 1, ec, c2) + dp1 * edge_avg(m, Epre, 1, ec + 2, c2)) / (dp1 + dp2);
+   *fE = .5 * (edge_avg(m, Epre, 0, eo, co) + Ea);
+   *fB = .5 * (edge_avg(m, Epre, 1, eo, co) + Ba);
+}
 
 // The four phi+ edges of a cell after avg_Efields, before the advective
 // subtraction: {E0, E1, E4, E5} then {B0, B1, B4, B5}.  An edge bounding
@@ -1047,7 +1079,7 @@ struct EdgeArgs {
    Mesh m;
    Faces Fr, Fz;
    const int *rngr, *rngz;
-   const double *Epre, *fEr, *fBr, *fEz, *fBz;
+   const double *Epre, *piph, *dphi;
 };
 
 DHD void edges_post_face(const EdgeArgs &g, long c, double *out)
@@ -1063,13 +1095,13 @@ DHD void edges_post_face(const EdgeArgs &g, long c, double *out)
    }
    if (m.Nr > 1) {
       CellRanges cr = load_ranges(m, 1, g.rngr, g.Fr.fI, c, a);
-      if (cr.in) { int f = cr.base_in + cr.in_h1; out[0] = g.fEr[f]; out[4] = g.fBr[f]; }
-      if (cr.out) { int f = cr.base_out + cr.out_h1; out[1] = g.fEr[f]; out[5] = g.fBr[f]; }
+      if (cr.in) face_emf(m, 1, g.Fr, g.piph, g.dphi, g.Epre, cr.base_in + cr.in_h1, &out[0], &out[4]);
+      if (cr.out) face_emf(m, 1, g.Fr, g.piph, g.dphi, g.Epre, cr.base_out + cr.out_h1, &out[1], &out[5]);
    }
    if (m.Nz > 1) {
       CellRanges cr = load_ranges(m, 2, g.rngz, g.Fz.fI, c, a);
-      if (cr.in) { int f = cr.base_in + cr.in_h1; out[2] = g.fEz[f]; out[6] = g.fBz[f]; }
-      if (cr.out) { int f = cr.base_out + cr.out_h1; out[3] = g.fEz[f]; out[7] = g.fBz[f]; }
+      if (cr.in) face_emf(m, 2, g.Fz, g.piph, g.dphi, g.Epre, cr.base_in + cr.in_h1, &out[2], &out[6]);
+      if (cr.out) face_emf(m, 2, g.Fz, g.piph, g.dphi, g.Epre, cr.base_out + cr.out_h1, &out[3], &out[7]);
    }
 }
 

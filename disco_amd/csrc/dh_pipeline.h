@@ -78,7 +78,7 @@ struct Buffers {
    int *I0 = nullptr;
    // constrained transport
    double *Phi = nullptr, *rkPhi = nullptr, *emf_phi = nullptr;
-   double *emf_f[2] = {nullptr, nullptr}, *fE[2] = {nullptr, nullptr}, *fB[2] = {nullptr, nullptr};
+   double *emf_f[2] = {nullptr, nullptr};
    double *Epre = nullptr, *Eo = nullptr, *Eown = nullptr, *Ecarry = nullptr;
    signed char *flag[2] = {nullptr, nullptr};
    int *estep[2] = {nullptr, nullptr};  // merge steps of each face on its two edge lines
@@ -378,8 +378,6 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
       DH_TRY(alloc_n(x, &b.rng[d], sizeof(int) * 2 * (size_t)ntot));
       if (b.mhd) {
          DH_TRY(alloc_n(x, &b.emf_f[d], sizeof(double) * 3 * n));
-         DH_TRY(alloc_n(x, &b.fE[d], sizeof(double) * n));
-         DH_TRY(alloc_n(x, &b.fB[d], sizeof(double) * n));
          DH_TRY(alloc_n(x, &b.flag[d], sizeof(signed char) * n));
          DH_TRY(alloc_n(x, &b.estep[d], sizeof(int) * 2 * n));
          // edge lines: the line whose first list is pair P holds the merge
@@ -581,19 +579,7 @@ struct Engine {
          KEdgesCollect kc{m, ct, b.dphi, b.emf_phi, b.F[0], b.F[1], b.rng[0], b.rng[1],
                           b.emf_f[0], b.emf_f[1], b.Ecarry, b.Epre};
          DH_TRY(run_ord(x, "edges_collect", kc, m.ntot, b.ord[ORD_CELLS]));
-         for (int dim = 1; dim <= 2; ++dim) {
-            int d = dim - 1;
-            if (!dim_active(m, dim)) continue;
-            if (dim == 1)
-               DH_TRY(run_ord(x, "face_emf_r",
-                              KFaceEmf<1>{m, b.F[d], b.piph, b.dphi, b.Epre, b.fE[d], b.fB[d]},
-                              b.F[d].n, b.ord[ORD_FACES_R]));
-            else
-               DH_TRY(run_ord(x, "face_emf_z",
-                              KFaceEmf<2>{m, b.F[d], b.piph, b.dphi, b.Epre, b.fE[d], b.fB[d]},
-                              b.F[d].n, b.ord[ORD_FACES_Z]));
-         }
-         EdgeArgs ea{m, b.F[0], b.F[1], b.rng[0], b.rng[1], b.Epre, b.fE[0], b.fB[0], b.fE[1], b.fB[1]};
+         EdgeArgs ea{m, b.F[0], b.F[1], b.rng[0], b.rng[1], b.Epre, b.piph, b.dphi};
          KEdgesFinal kef{ea, ct, b.wiph, b.emf_f[0], b.emf_f[1], b.Eo, b.Eown, b.Ecarry};
          DH_TRY(run_ord(x, "edges_final", kef, m.ntot, b.ord[ORD_CELLS]));
          KUpdateB kb{m, RK, dt, b.F[0], b.F[1], b.rng[0], b.rng[1], b.Eo, b.Phi, b.rkPhi};

@@ -46,10 +46,10 @@ int planet_motion_analytic(void);
 void initializePlanets(struct planet *);
 void movePlanets(struct planet *, double, double);
 
-/* Timesteps of launch tuning at the start of a run: enough for every
- * kernel (launched once or twice a step) to run each of its up to six
- * variants at least once. */
-#define DISCO_AUTOTUNE_STEPS 8
+/* Timesteps of launch tuning at the start of a run: each tuning step
+ * times one variant of every kernel over the whole step, so 12 steps time
+ * each of up to six variants twice. */
+#define DISCO_AUTOTUNE_STEPS 12
 
 struct disco_accel {
    dh_ctx *ctx;

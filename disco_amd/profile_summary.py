@@ -31,7 +31,6 @@ _NAMES = [
     (r"KFluxPhi<", "flux_phi"),
     (r"KFluxTrans<.*, 1>", "flux_r"), (r"KFluxTrans<.*, 2>", "flux_z"),
     (r"KEdgesCollect", "edges_collect"),
-    (r"KFaceEmf<1>", "face_emf_r"), (r"KFaceEmf<2>", "face_emf_z"),
     (r"KEdgesFinal", "edges_final"),
     (r"KUpdateB", "update_B"),
     (r"KLineSteps<1>", "line_steps_r"), (r"KLineSteps<2>", "line_steps_z"),

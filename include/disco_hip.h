@@ -324,17 +324,26 @@ int dh_ens_sync(dh_ens *ens);
  * eighths (1), or whole annuli (annulus pairs) taken in strips of S layers
  * of z, the strips one after another, r after r within a strip and z
  * fastest (2, 3, ...; S = 4, 8, 16 or all layers).  Results are the same
- * in every order.  dh_autotune makes the next `steps` timesteps launch each
- * such kernel in its variants in turn, timed with events (eagerly, never
- * recorded), after which each kernel keeps its fastest; steps = 0 keeps
- * the choices made so far.  Until a kernel is tuned it runs in variant 0,
- * or 1 for the kernels that gather from the next annulus in r.
+ * in every order.  dh_autotune makes the next `steps` timesteps (eager,
+ * never recorded) tuning steps: in tuning step s every such kernel runs
+ * all its launches of the step in variant s mod (its variant count), timed
+ * with events, so that each variant is timed over whole steps; then each
+ * kernel keeps its fastest (steps at least the largest variant count time
+ * them all; steps = 0 keeps the choices made so far).  Until a kernel is
+ * tuned it runs in variant 0, or 1 for the kernels that gather from the
+ * next annulus in r.
  * dh_read_tuning reports up to `cap` kernels: the chosen variant, its
  * strip height (0 for variants 0 and 1), and the mean launch time of the
- * chosen and of the untuned variant over the tuning steps (-1 if none). */
+ * chosen and of the untuned variant over the tuning steps (-1 if none).
+ * dh_read_tuning_sweep reports every variant of the kernel `name`: its
+ * strip height and mean launch time over the tuning steps (-1 if it was
+ * not timed); *nvar is the kernel's variant count (0 for a name that has
+ * not run), of which the first `cap` are written. */
 int dh_autotune(dh_ctx *ctx, int steps);
 int dh_read_tuning(dh_ctx *ctx, int cap, char (*names)[48], int *variant, int *strip,
                    double *ms_chosen, double *ms_default, int *nrows);
+int dh_read_tuning_sweep(dh_ctx *ctx, const char *name, int cap, int *strip, double *ms,
+                         int *nvar);
 
 const char *dh_last_error(void);
 

@@ -506,6 +506,11 @@ int dh_read_tuning(dh_ctx *c, int, char (*)[48], int *, int *, double *, double 
    *nrows = 0;
    return DH_OK;
 }
+int dh_read_tuning_sweep(dh_ctx *, const char *, int, int *, double *, int *nvar)
+{
+   if (nvar) *nvar = 0;
+   return DH_OK;
+}
 
 int dh_report(dh_ctx *c, const double *planets, int npl, double *out)
 {
