@@ -301,34 +301,43 @@ struct FaceGeo {
    double phic, dp, dA;
 };
 
-// addFace (faces_alt.c:10-34, 135-175) for the face between cells cL (inner
-// or lower annulus) and cR of pair (j, k).
-DHD FaceGeo face_geometry(const Mesh &m, int dim, int j, int k, int cL, int cR,
-                          const double *piph, const double *dphi)
+// addFace (faces_alt.c:10-34, 135-175) for the face of pair (j, k) between
+// cells of upper phi edges pL, pR and extents dL, dR (L in the inner or lower
+// annulus): its orientation, phi extent, phi centre and area.  The slope
+// and flux kernels read the stored centre and area (formed there again
+// from the positions, the r-face fluxes ran 1.6 times slower).
+DHD void face_shape(const Mesh &m, int dim, int j, int k, double pL, double dL, double pR,
+                    double dR, int *LR, double *dp, double *phic, double *dA)
 {
    double Pmax = m.Pmax;
-   FaceGeo g;
-   g.cL = cL;
-   g.cR = cR;
-   double pL = piph[cL], pR = piph[cR];
-   double dL = dphi[cL], dR = dphi[cR];
    double w = wrap_half(pL - pR, Pmax);
-   g.LR = w > 0. ? 1 : 0;
+   *LR = w > 0. ? 1 : 0;
    w = wrap_half(pL - dL - pR + dR, Pmax);
    int LRb = w < 0. ? 1 : 0;
-   double xp1 = g.LR == 0 ? pL : pR;
+   double xp1 = *LR == 0 ? pL : pR;
    double xm1 = LRb == 0 ? pL - dL : pR - dR;
-   g.dp = get_dp(xp1, xm1, Pmax);
-   g.phic = get_dp(xp1, .5 * g.dp, Pmax);
+   *dp = get_dp(xp1, xm1, Pmax);
+   *phic = get_dp(xp1, .5 * *dp, Pmax);
    if (dim == 1) {
       double r = .5 * (m.r_jph[j] + m.r_jph[j]);
       double dz = m.z_kph[k] - m.z_kph[k - 1];
-      g.dA = r * g.dp * dz;
+      *dA = r * *dp * dz;
    } else {
       double r = .5 * (m.r_jph[j] + m.r_jph[j - 1]);
       double dr = m.r_jph[j] - m.r_jph[j - 1];
-      g.dA = r * dr * g.dp;
+      *dA = r * dr * *dp;
    }
+}
+
+// The same for the face between cells cL and cR of pair (j, k).
+DHD FaceGeo face_geometry(const Mesh &m, int dim, int j, int k, int cL, int cR,
+                          const double *piph, const double *dphi)
+{
+   FaceGeo g;
+   g.cL = cL;
+   g.cR = cR;
+   face_shape(m, dim, j, k, piph[cL], dphi[cL], piph[cR], dphi[cR], &g.LR, &g.dp, &g.phic,
+              &g.dA);
    return g;
 }
 
@@ -457,8 +466,8 @@ struct KGrad {
       double sum[NQ], best[NQ];
       unsigned pos = 0, neg = 0;
    };
-   // What one face brings from the other cell o: the face's phi centre and
-   // area, o's phi extent and its state (prim, gradp).
+   // What one face brings from the other cell o: o's phi edge and extent
+   // and its state (prim, gradp).
    struct FaceIn {
       long o;
       double phic, dA, po, dpo, p[NQ], g[NQ];
@@ -477,23 +486,25 @@ struct KGrad {
          d.g[q] = gradp[q * nt + d.o];
       }
    }
-   // A face of a pair whose distances from the centroids are dxL, dxR; the
-   // cell is its L (own_left) or R, with state (pc, gc) and phi extent
-   // (pcc, dpc).  The one-sided slope is S = (W_R - W_L)/(dx_L + dx_R) of
-   // the phi-reconstructed states at the face's phi centre (plm.c:84-110).
-   DHD void add_face(const FaceIn &d, bool own_left, double pcc, double dpc, const double *pc,
-                     const double *gc, double dxL, double dxR, Acc &A) const
+   // A face of pair (pj, pk) whose distances from the centroids are dxL,
+   // dxR; the cell is its L (own_left) or R, with state (pc, gc) and phi
+   // edge and extent (pcc, dpc).  The one-sided slope is
+   // S = (W_R - W_L)/(dx_L + dx_R) of the phi-reconstructed states at the
+   // face's phi centre (plm.c:84-110), weighted by the face's area.
+   DHD void add_face(const FaceIn &d, bool own_left, int pj, int pk, double pcc, double dpc,
+                     const double *pc, const double *gc, double dxL, double dxR, Acc &A) const
    {
       double pL = own_left ? pcc : d.po, dL = own_left ? dpc : d.dpo;
       double pR = own_left ? d.po : pcc, dR = own_left ? d.dpo : dpc;
-      double dpL = get_sdp(d.phic, pL - .5 * dL, m.Pmax);
-      double dpR = get_sdp(pR - .5 * dR, d.phic, m.Pmax);
+      double phic = d.phic, dA = d.dA;
+      double dpL = get_sdp(phic, pL - .5 * dL, m.Pmax);
+      double dpR = get_sdp(pR - .5 * dR, phic, m.Pmax);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
          double WL = own_left ? pc[q] + dpL * gc[q] : d.p[q] + dpL * d.g[q];
          double WR = own_left ? d.p[q] - dpR * d.g[q] : pc[q] - dpR * gc[q];
          double S = (WR - WL) / (dxR + dxL);
-         A.sum[q] += S * d.dA;
+         A.sum[q] += S * dA;
          double v = PLM * S;
          if (fabs(v) < fabs(A.best[q])) A.best[q] = v;
          if (S > 0.0) A.pos |= 1u << q;
@@ -527,7 +538,9 @@ struct KGrad {
           pair_seq(cr), [&](int f, int r, FaceIn &d) { load_face(f, r >= 2, d); },
           [&](const FaceIn &d, int r) {
              bool own_left = r >= 2;
-             add_face(d, own_left, pcc, dpc, pc, gc, own_left ? dxLo : dxLi,
+             // the outer pair is (j, k); the inner one (j-1, k) or (j, k-1)
+             int pj = own_left || dim == 2 ? j : j - 1, pk = own_left || dim == 1 ? k : k - 1;
+             add_face(d, own_left, pj, pk, pcc, dpc, pc, gc, own_left ? dxLo : dxLi,
                       own_left ? dxRo : dxRi, A);
           });
       double phip = piph[c];
@@ -852,7 +865,11 @@ struct KUpdate {
    int first_step;
    const PlanetSet *pls;
    const double *prim_in;
-   double *prim, *cons;
+   double *prim;
+   // the substep's conserved state in, the RK blend's other state, and the
+   // updated state out (cons_out may be cons)
+   const double *cons;
+   double *cons_out;
    const double *rkcons, *gradr, *gradp, *gradz, *piph, *dphi, *wiph;
    double *piph_out, *dphi_out;
    const double *dFp, *dFr, *dFz;
@@ -929,7 +946,7 @@ struct KUpdate {
          H::visc_source(P, pr, gr, gp, gz, cs, centroid_r(rp, rm), dVdt);
       }
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) cons[q * nt + c] = cs[q];
+      for (int q = 0; q < NQ; ++q) cons_out[q * nt + c] = cs[q];
       if constexpr (!mhd) {
          // cons2prim at the moved position.
          double r = centroid_r(rp, rm);
@@ -1184,8 +1201,11 @@ struct KUpdateB {
    Faces Fr, Fz;
    const int *rngr, *rngz;
    const double *Eo;
-   double *Phi;
+   // the substep's fluxes in, the RK blend's other fluxes, the updated
+   // fluxes out (Phi_out may be Phi)
+   const double *Phi;
    const double *rkPhi;
+   double *Phi_out;
    DHD void operator()(long c) const
    {
       size_t nt = (size_t)m.ntot;
@@ -1237,7 +1257,7 @@ struct KUpdateB {
          }
       }
 #pragma unroll
-      for (int s = 0; s < 5; ++s) Phi[s * nt + c] = ph[s];
+      for (int s = 0; s < 5; ++s) Phi_out[s * nt + c] = ph[s];
    }
 };
 

@@ -404,6 +404,7 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
    DH_TRY(alloc_n(x, &b.prim, qb));
    DH_TRY(alloc_n(x, &b.cons, qb));
    DH_TRY(alloc_n(x, &b.rkcons, qb));
+   DH_TRY(x.zero(b.rkcons, qb));
    DH_TRY(alloc_n(x, &b.gradp, qb));
    DH_TRY(alloc_n(x, &b.gradr, 2 * qb));  // r and z slopes in one block
    b.gradz = b.gradr + (size_t)nq * ntot;
@@ -454,6 +455,7 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
       b.Eown = b.gradp + 4 * (size_t)ntot;
       DH_TRY(alloc_n(x, &b.Ecarry, 16 * cb));
       DH_TRY(x.zero(b.Phi, 5 * cb));
+      DH_TRY(x.zero(b.rkPhi, 5 * cb));
       DH_TRY(x.zero(b.emf_phi, 4 * cb));
       DH_TRY(x.zero(b.Ecarry, 16 * cb));
    }
@@ -541,11 +543,20 @@ struct Engine {
       return DH_OK;
    }
 
+   // The conserved state and the magnetic fluxes of a substep: in, the RK
+   // blend's other state, and out (the buffer that holds the substep's
+   // result, which the kernels after the update read); pflip is the copy of
+   // the fluxes that the face flips hand over alongside out (the reference's
+   // RK_Phi), a buffer other than pout.
+   struct SubState {
+      double *cin, *crk, *cout, *pin, *prk, *pout, *pflip;
+   };
+
    // One substep (onestep.c:84-228); its dt and planets are b.step's
    // entries sub.  reuse: the faces of the current positions are already
    // built (buildfaces would rebuild the same ones).
    template <int NQ, class H>
-   int substep(double RK, int first_step, int sub, bool reuse)
+   int substep(double RK, int first_step, int sub, bool reuse, const SubState &st)
    {
       const double *dt = &b.step->dt[sub];
       constexpr bool mhd = IsMHD<H>::value;
@@ -582,7 +593,7 @@ struct Engine {
          EdgeArgs ea{m, b.F[0], b.F[1], b.rng[0], b.rng[1], b.Epre, b.piph, b.dphi};
          KEdgesFinal kef{ea, ct, b.wiph, b.emf_f[0], b.emf_f[1], b.Eo, b.Eown, b.Ecarry};
          DH_TRY(run_ord(x, "edges_final", kef, m.ntot, b.ord[ORD_CELLS]));
-         KUpdateB kb{m, RK, dt, b.F[0], b.F[1], b.rng[0], b.rng[1], b.Eo, b.Phi, b.rkPhi};
+         KUpdateB kb{m, RK, dt, b.F[0], b.F[1], b.rng[0], b.rng[1], b.Eo, st.pin, st.prk, st.pout};
          DH_TRY(run_ord(x, "update_B", kb, m.ntot, b.ord[ORD_CELLS]));
          if (dim_active(m, 2)) {
             if (dim_active(m, 1) && !reuse)
@@ -595,12 +606,12 @@ struct Engine {
                              b.piph, b.dphi, b.Eown, b.seg[0], b.seg[1]};
             DH_TRY(run_ord(x, "edge_segments", ks, (long)b.F[0].n + b.F[1].n, b.ord[ORD_EDGES]));
             KEdgeAdjust ke{m, b.F[0], b.F[1], b.estep[0], b.estep[1], b.lbase[0], b.lbase[1],
-                           b.seg[0], b.seg[1], b.Phi};
+                           b.seg[0], b.seg[1], st.pout};
             DH_TRY(run_ord(x, "edge_adjust", ke, (long)b.F[0].n + b.F[1].n, b.ord[ORD_EDGES]));
          }
       }
-      KUpdate<NQ, H> ku{m, P, RK, dt, first_step, &b.step->pl[sub], b.prim, b.prim, b.cons,
-                        b.rkcons, b.gradr, b.gradp, b.gradz, b.piph, b.dphi, b.wiph,
+      KUpdate<NQ, H> ku{m, P, RK, dt, first_step, &b.step->pl[sub], b.prim, b.prim, st.cin,
+                        st.cout, st.crk, b.gradr, b.gradp, b.gradz, b.piph, b.dphi, b.wiph,
                         b.piph2, b.dphi2, b.dFp, b.dF[0], b.dF[1], b.F[0].fI, b.F[1].fI,
                         b.rng[0], b.rng[1], b.F[0].n, b.F[1].n};
       DH_TRY(run_ord(x, "update", ku, m.ntot, b.ord[ORD_CELLS]));
@@ -617,20 +628,20 @@ struct Engine {
                   DH_TRY(run_ord(x, "flip_flags_z",
                                  KFlipFlags<2>{m, b.F[d], b.piph, b.wiph, dt, b.flag[d]},
                                  b.F[d].n, b.ord[ORD_FACES_Z]));
-               DH_TRY(x.flips(m, dim, b.F[d], b.flag[d], b.Phi, b.rkPhi));
+               DH_TRY(x.flips(m, dim, b.F[d], b.flag[d], st.pout, st.pflip));
             }
          }
          // with the learned guess cons2prim waits for the model's guesses,
          // which need the cells' conserved field first
          bool learned = IsGR<H>::value && P.c2p_guess == DH_C2P_GUESS_LEARNED;
          KBfacesC2p<NQ, H> kbc{m, P, b.F[0], b.F[1], b.rng[0], b.rng[1], b.piph2, b.dphi2,
-                               b.Phi, b.cons, b.prim, 1, learned ? 0 : 1, b.c2p_info};
+                               st.pout, st.cout, b.prim, 1, learned ? 0 : 1, b.c2p_info};
          DH_TRY(run_ord(x, "bfaces_c2p", kbc, m.ntot, b.ord[ORD_CELLS]));
          if constexpr (IsGR<H>::value) {
             if (learned) {
-               DH_TRY(x.template c2p_guess<NQ>(m, P, b.cons, b.piph2, b.dphi2, b.c2p_prm,
+               DH_TRY(x.template c2p_guess<NQ>(m, P, st.cout, b.piph2, b.dphi2, b.c2p_prm,
                                                b.c2p_frag, b.c2p_guess));
-               KC2p<NQ> kc{m, P, b.piph2, b.dphi2, b.cons, b.prim, b.c2p_guess, b.c2p_info};
+               KC2p<NQ> kc{m, P, b.piph2, b.dphi2, st.cout, b.prim, b.c2p_guess, b.c2p_info};
                DH_TRY(run_ord(x, "c2p", kc, m.ntot, b.ord[ORD_CELLS]));
             }
          }
@@ -665,10 +676,22 @@ struct Engine {
       bool reuse = b.topo_valid;
       b.topo_valid = false;
       DH_TRY(x.replayable(rk2 && reuse ? (const void *)b.piph : nullptr, [&]() -> int {
-         DH_TRY(x.copy(b.rkcons, b.cons, sizeof(double) * NQ * b.m.ntot));
-         if (b.mhd) DH_TRY(x.copy(b.rkPhi, b.Phi, sizeof(double) * 5 * b.m.ntot));
-         DH_TRY((substep<NQ, H>(0.0, 1, 0, reuse)));
-         if (rk2) DH_TRY((substep<NQ, H>(0.5, 0, 1, false)));
+         // RK2 without copies: the first substep reads the step's start
+         // state from cons (Phi) and writes its result to rkcons (rkPhi);
+         // its face flips hand over both, the start state standing for the
+         // reference's RK copy; the second substep blends its input with the
+         // start state and writes back to cons (Phi).  Forward Euler updates
+         // in place (its RK copy is never read: the flips' second array is
+         // the spare rkPhi).
+         if (rk2) {
+            DH_TRY((substep<NQ, H>(0.0, 1, 0, reuse,
+                                   {b.cons, b.cons, b.rkcons, b.Phi, b.Phi, b.rkPhi, b.Phi})));
+            DH_TRY((substep<NQ, H>(0.5, 0, 1, false,
+                                   {b.rkcons, b.cons, b.cons, b.rkPhi, b.Phi, b.Phi, nullptr})));
+         } else {
+            DH_TRY((substep<NQ, H>(0.0, 1, 0, reuse,
+                                   {b.cons, b.cons, b.cons, b.Phi, b.Phi, b.Phi, b.rkPhi})));
+         }
          return DH_OK;
       }));
       b.topo_valid = rk2;
