@@ -147,6 +147,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "dh_c2p_harvest": ([vp, C.POINTER(C.c_float), C.POINTER(C.c_float)], C.c_int),
         "dh_c2p_predict": ([vp, C.POINTER(C.c_float)], C.c_int),
         "dh_c2p_stats": ([vp, C.POINTER(C.c_longlong)], C.c_int),
+        "dh_c2p_info": ([vp, vp], C.c_int),
         "dh_c2p_train": ([C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_long, C.c_int,
                           C.c_int, C.c_float, C.c_ulonglong, C.POINTER(C.c_float),
                           C.POINTER(C.c_float)], C.c_int),
@@ -177,7 +178,7 @@ EXPORTED_SYMBOLS = ["dh_create", "dh_destroy", "dh_sizes", "dh_upload_state",
                     "dh_face_count", "dh_get_faces", "dh_sync", "dh_enable_timers", "dh_enable_graphs",
                     "dh_read_timers", "dh_add_diagnostics", "dh_get_diagnostics",
                     "dh_zero_diagnostics", "dh_avg_diagnostics", "dh_report", "dh_set_c2p_model",
-                    "dh_c2p_harvest", "dh_c2p_predict", "dh_c2p_stats", "dh_c2p_train",
+                    "dh_c2p_harvest", "dh_c2p_predict", "dh_c2p_stats", "dh_c2p_info", "dh_c2p_train",
                     "dh_c2p_grad", "dh_c2p_grad_noalloc", "dh_autotune", "dh_read_tuning", "dh_read_tuning_sweep",
                     "dh_ens_create", "dh_ens_destroy", "dh_ens_getmindt", "dh_ens_timestep",
                     "dh_ens_add_diagnostics", "dh_ens_enable_graphs", "dh_ens_sync",
@@ -342,6 +343,14 @@ class DiscoHip:
         out = (C.c_longlong * 4)()
         self._check(self.lib.dh_c2p_stats(self.ctx, out))
         return dict(zip(("cells", "iterations", "learned", "fallbacks"), list(out)))
+
+    def c2p_info(self) -> np.ndarray:
+        """The last cons2prim's record of every cell (dh_c2p_info): bits 0-5
+        Newton iterations (63: the stopping rule was not met), 64 learned
+        guess, 128 fall-back; 0 outside the real cells."""
+        out = np.zeros(self.ntot, dtype=np.uint8)
+        self._check(self.lib.dh_c2p_info(self.ctx, out.ctypes.data_as(C.c_void_p)))
+        return out
 
     def init_cells(self):
         self._check(self.lib.dh_init_cells(self.ctx))

@@ -1690,6 +1690,12 @@ int dh_c2p_stats(dh_ctx *c, long long *out)
    return c2p_stats(c->x, c->b, out);
 }
 
+int dh_c2p_info(dh_ctx *c, unsigned char *info)
+{
+   if (!c) return fail(DH_EINVAL, "null context");
+   return c2p_info(c->x, c->b, info);
+}
+
 int dh_enable_timers(dh_ctx *c, int on)
 {
    if (!c) return fail(DH_EINVAL, "null context");

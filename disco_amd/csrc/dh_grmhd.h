@@ -325,8 +325,10 @@ DHD void c2p_invariants(const GRMetric &g, double r, const double *cons, C2pInv 
    v.e = (v.tau / D + Us + 1.0) / (g.al * U[0]);
 }
 
-// The record cons2prim leaves per cell: Newton iterations (saturating at
-// C2P_INFO_ITERS) and whether the learned guess was used or fell back.
+// The record cons2prim leaves per cell: Newton iterations (at most 60: 30
+// from the learned guess and 30 after a fall-back), C2P_INFO_ITERS itself
+// when the last solve did not meet its stopping rule, and whether the
+// learned guess was used or fell back.
 enum { C2P_INFO_ITERS = 63, C2P_INFO_LEARNED = 64, C2P_INFO_FALLBACK = 128 };
 // Relative residual of the residual stop rule (DH_C2P_STOP_RESIDUAL).
 constexpr double C2P_RES_TOL = 1.0e-10;
@@ -798,12 +800,12 @@ struct GRMHD {
          int it2 = 0;
          v2 = v2p;
          eta = etap;
-         noble2d(v, n, P.c2p_stop, v2, eta, it2);
+         ok = noble2d(v, n, P.c2p_stop, v2, eta, it2);
          it += it2;
          fell_back = true;
       }
       if (info) {
-         int k = it < C2P_INFO_ITERS ? it : C2P_INFO_ITERS;
+         int k = ok && it < C2P_INFO_ITERS ? it : C2P_INFO_ITERS;
          *info = (unsigned char)(k | (learned ? C2P_INFO_LEARNED : 0) |
                                  (fell_back || (guess && !learned) ? C2P_INFO_FALLBACK : 0));
       }

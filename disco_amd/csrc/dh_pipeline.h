@@ -1064,6 +1064,22 @@ int c2p_stats(X &x, Buffers &b, long long *out)
    return DH_OK;
 }
 
+// The last cons2prim's record of every cell (C2P_INFO_*; 0 outside the
+// real cells).
+template <class X>
+int c2p_info(X &x, Buffers &b, unsigned char *out)
+{
+   if (b.P.hydro != DH_HYDRO_GRMHD) return fail(DH_EINVAL, "c2p records need grmhd2");
+   if (!out) return fail(DH_EINVAL, "null argument");
+   std::vector<unsigned char> info((size_t)b.m.ntot);
+   DH_TRY(x.get(info.data(), b.c2p_info, info.size()));
+   for (int a = 0; a < b.m.nann; ++a) {
+      bool real = is_real(b.m, a % b.m.Nr, a / b.m.Nr);
+      for (int c = b.h_off[a]; c < b.h_off[a + 1]; ++c) out[c] = real ? info[c] : 0;
+   }
+   return DH_OK;
+}
+
 // splitmix64 (Steele, Lea & Flood 2014): the seeded stream of the training
 // set-up (initial weights, batch order).
 struct SplitMix {

@@ -236,11 +236,17 @@ int dh_report(dh_ctx *ctx, const double *planets, int npl, double *out);
  * iterations, cells solved from the learned guess, cells that fell back to
  * the previous primitives}.  dh_c2p_predict writes the model's guesses
  * guess[2 * ntot] = (v^2, eta) per cell for the current state (NaN outside
- * the real cells). */
+ * the real cells).  dh_c2p_info writes the last cons2prim's record of every
+ * cell, info[ntot] (0 outside the real cells): bits 0-5 the Newton
+ * iterations (63: the last solve did not meet its stopping rule), 64 the
+ * learned guess was used, 128 the solve fell back to the previous
+ * primitives; dh_c2p_stats counts a solve that did not meet its rule as 63
+ * iterations. */
 int dh_set_c2p_model(dh_ctx *ctx, const float *params, long n);
 int dh_c2p_harvest(dh_ctx *ctx, float *feat, float *targ);
 int dh_c2p_predict(dh_ctx *ctx, float *guess);
 int dh_c2p_stats(dh_ctx *ctx, long long *out);
+int dh_c2p_info(dh_ctx *ctx, unsigned char *info);
 
 /* Training of the learned guess on the device (no context): float32
  * forward and backward passes, deterministic per-parameter gradient sums

@@ -19,7 +19,11 @@
  *      reference's report() line of the final state, report.c:7-202)
  *
  * Usage: disco_ref_<cfg> <outdir> <nsteps> [dump_every]
+ *        disco_ref_<cfg> p2c <state.bin> <cons.bin>
  *   in.par is read from the current directory (readpar.c:60).
+ *   p2c: the reference's prim2cons (as setupCells applies it, domain.c:
+ *   174-207) of every real cell of a DSTATE01 file's primitives at its cell
+ *   positions; writes double cons[NQ][Ntot] (zero on ghost cells).
  */
 #include "paul.h"
 #include "profiler.h"
@@ -105,8 +109,87 @@ static double wall(void)
    return tv.tv_sec + 1e-6 * tv.tv_usec;
 }
 
+#include "geometry.h"
+#include "hydro.h"
+
+/* p2c mode: read a DSTATE01 file into the set-up domain's cells and write
+ * prim2cons of its real cells. */
+static int prim2cons_file(struct domain *d, const char *in, const char *out)
+{
+   FILE *f = fopen(in, "rb");
+   if (!f) { perror(in); return 2; }
+   char magic[8];
+   int hdr[16];
+   double dhdr[8];
+   if (fread(magic, 1, 8, f) != 8 || memcmp(magic, "DSTATE01", 8) ||
+       fread(hdr, sizeof(int), 16, f) != 16 || fread(dhdr, sizeof(double), 8, f) != 8) {
+      fprintf(stderr, "%s: not a DSTATE01 file\n", in);
+      return 2;
+   }
+   int Nr = d->Nr, Nz = d->Nz, jk, i, q;
+   long Ntot = 0;
+   for (jk = 0; jk < Nr * Nz; ++jk) Ntot += d->Np[jk];
+   if (hdr[0] != NUM_Q || hdr[2] != Nr || hdr[3] != Nz || hdr[8] != Ntot) {
+      fprintf(stderr, "%s: mesh or NUM_Q differs from in.par's\n", in);
+      return 2;
+   }
+   int *Np = (int *)malloc(sizeof(int) * Nr * Nz);
+   size_t nskip = (size_t)(Nr + 1) + (size_t)(Nz + 1);
+   double *skip = (double *)malloc(sizeof(double) * nskip);
+   double *prim = (double *)malloc(sizeof(double) * NUM_Q * Ntot);
+   double *tmp = (double *)malloc(sizeof(double) * NUM_Q * Ntot);
+   double *piph = (double *)malloc(sizeof(double) * Ntot);
+   double *dphi = (double *)malloc(sizeof(double) * Ntot);
+   int ok = fread(Np, sizeof(int), Nr * Nz, f) == (size_t)(Nr * Nz) &&
+            fread(skip, sizeof(double), nskip, f) == nskip &&
+            fread(prim, sizeof(double), NUM_Q * Ntot, f) == (size_t)(NUM_Q * Ntot) &&
+            fread(tmp, sizeof(double), NUM_Q * Ntot, f) == (size_t)(NUM_Q * Ntot) &&
+            fread(piph, sizeof(double), Ntot, f) == (size_t)Ntot &&
+            fread(dphi, sizeof(double), Ntot, f) == (size_t)Ntot;
+   fclose(f);
+   for (jk = 0; ok && jk < Nr * Nz; ++jk) ok = Np[jk] == d->Np[jk];
+   if (!ok) {
+      fprintf(stderr, "%s: short file or different annuli\n", in);
+      return 2;
+   }
+   memset(tmp, 0, sizeof(double) * NUM_Q * Ntot);
+   double *r_jph = d->r_jph, *z_kph = d->z_kph;
+   long n = 0;
+   int j, k;
+   for (jk = 0; jk < Nr * Nz; ++jk) {
+      j = jk % Nr;
+      k = jk / Nr;
+      int real = j >= d->NgRa && j < Nr - d->NgRb && k >= d->NgZa && k < Nz - d->NgZb;
+      for (i = 0; i < Np[jk]; ++i, ++n) {
+         if (!real) continue;
+         double phip = piph[n], phim = phip - dphi[n];
+         double xp[3] = {r_jph[j], phip, z_kph[k]};
+         double xm[3] = {r_jph[j - 1], phim, z_kph[k - 1]};
+         double dV = get_dV(xp, xm);
+         double x[3] = {get_centroid(r_jph[j], r_jph[j - 1], 1), phip - .5 * dphi[n],
+                        get_centroid(z_kph[k], z_kph[k - 1], 2)};
+         double pc[NUM_Q], cc[NUM_Q];
+         for (q = 0; q < NUM_Q; ++q) pc[q] = prim[q * Ntot + n];
+         prim2cons(pc, cc, x, dV);
+         for (q = 0; q < NUM_Q; ++q) tmp[q * Ntot + n] = cc[q];
+      }
+   }
+   FILE *g = fopen(out, "wb");
+   if (!g) { perror(out); return 2; }
+   fwrite(tmp, sizeof(double), NUM_Q * Ntot, g);
+   fclose(g);
+   free(Np);
+   free(skip);
+   free(prim);
+   free(tmp);
+   free(piph);
+   free(dphi);
+   return 0;
+}
+
 int main(int argc, char *argv[])
 {
+   int p2c = argc == 4 && !strcmp(argv[1], "p2c");
    if (argc < 3) {
       fprintf(stderr, "usage: %s outdir nsteps [dump_every]\n", argv[0]);
       return 1;
@@ -124,6 +207,7 @@ int main(int argc, char *argv[])
    setupGrid(&theDomain);
    setupDomain(&theDomain);
    setupCells(&theDomain);
+   if (p2c) return prim2cons_file(&theDomain, argv[2], argv[3]);
 
    if (theDomain.Nr > 1) {
       exchangeData(&theDomain, 0);
