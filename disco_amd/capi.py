@@ -44,7 +44,7 @@ class DhParams(C.Structure):
     _fields_ = ([(n, C.c_int) for n in _PARAM_INTS]
                 + [(n, C.c_double) for n in _PARAM_DOUBLES]
                 + [("init_par", C.c_double * 9), ("phi_order", C.c_double),
-                   ("metric_par3", C.c_double)])
+                   ("metric_par3", C.c_double), ("nozzle_par", C.c_double * 4)])
 
 
 PLANET_STRIDE = 8
@@ -85,6 +85,8 @@ def make_params(cfg: "cfgmod.Config", par: Dict[str, object]) -> DhParams:
         setattr(p, f, float(full[k]))
     for i in range(9):
         p.init_par[i] = float(full[f"Init_Par{i}"])
+    for i in range(4):
+        p.nozzle_par[i] = float(full.get(f"Nozzle_Par{i + 1}", 0.0))
     p.bc_rmin, p.bc_rmax, p.bc_zmin, p.bc_zmax = cfg.dh_bc(full)
     p.phi_order = 2.0  # PHI_ORDER, planet.c:4
     # grmhd2 cons2prim's guess and stopping rule (DH_C2P_*; not reference

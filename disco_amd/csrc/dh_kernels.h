@@ -820,6 +820,71 @@ DHD void planet_src(const dh_params &P, const double *pl, const double *prim, do
    cons[TAU] += rho * (hr * F0 * vr + hz * F2 * vz + hp * F1 * omega) * dVdt;
 }
 
+// omega_src (rotframe.c:32-82) in cylindrical coordinates: the centrifugal
+// and Coriolis forces of a frame rotating at RotOmega about an axis at
+// distance RotD, at the cell's centroid (get_centroid_arr).  The vector
+// conversions of the reference are kept step by step (to the orthonormal
+// basis and back), so that the sums round as there.
+DHD void omega_src(const dh_params &P, const double *prim, double *cons, double rp, double rm,
+                   double phip, double phim, double zp, double zm, double dVdt)
+{
+   double r = centroid_r(rp, rm);
+   double phi = 0.5 * (phip + phim);
+   double rho = prim[RHO];
+   // get_vec_covariant (orthonormal), then get_vec_contravariant
+   double v[3] = {prim[URR], prim[UPP] * r, prim[UZZ]};
+   double vrpz[3] = {v[0] / 1.0, v[1] / r, v[2] / 1.0};
+   // omegaForce (rotframe.c:19-30); cos and sin of one angle from one sincos
+   // call, as the reference's optimised build evaluates them
+   double O0 = P.rot_omega, d = P.rot_d;
+   double sn, cs;
+   sincos(phi, &sn, &cs);
+   double Rr = r - d * cs;
+   double Rp = d * sn;
+   double f[3] = {O0 * O0 * Rr + 2. * O0 * r * vrpz[1], O0 * O0 * Rp - 2. * O0 * vrpz[0], 0.0};
+   // the energy-frame velocity, still orthonormal
+   v[1] -= r * get_om(P, r);
+   double vf = v[0] * f[0] + v[1] * f[1] + v[2] * f[2];
+   // back to covariant components
+   f[0] = f[0] * 1.0;
+   f[1] = f[1] * r;
+   f[2] = f[2] * 1.0;
+   cons[SRR] += rho * f[0] * dVdt;
+   cons[LLL] += rho * f[1] * dVdt;
+   cons[SZZ] += rho * f[2] * dVdt;
+   cons[TAU] += rho * vf * dVdt;
+}
+
+// sink_src (sink.c:35-70), Nozzle_Type 1: mass injected at a uniform rate
+// (Mdot = 1) into a sphere of radius dR about (R0, 0, 0), or the disk of its
+// cross-section in 2-D, with azimuthal speed v and sound speed v / Mach;
+// cell midpoints (not centroids), phi taken into (-pi, pi].
+template <int NQ, int NC>
+DHD void sink_src(const dh_params &P, bool twoD, double *cons, double rp, double rm, double phip,
+                  double phim, double zp, double zm, double dVdt)
+{
+   if (P.nozzle_type != 1) return;
+   double R0 = P.nozzle_par[0], dR = P.nozzle_par[1], v = P.nozzle_par[2],
+          mach = P.nozzle_par[3];
+   double r = 0.5 * (rp + rm);
+   double phi = 0.5 * (phip + phim);
+   double z = 0.5 * (zp + zm);
+   if (phi > M_PI) phi -= 2 * M_PI;
+   if ((r - R0) * (r - R0) + r * r * phi * phi + z * z > dR * dR) return;
+   double Mdot = 1.0;
+   double rhodot = Mdot * 3.0 / (4.0 * M_PI * dR * dR * dR);
+   if (twoD) rhodot *= 2 * sqrt(dR * dR - (r - R0) * (r - R0) - r * r * phi * phi);
+   if (rhodot * dVdt > cons[DDD]) rhodot = cons[DDD] / dVdt;
+   double om = get_om(P, r);
+   double cs2 = v * v / (mach * mach);
+   double gl = P.gamma_law;
+   double eps = cs2 / (gl * (gl - 1));
+   cons[DDD] += rhodot * dVdt;
+   cons[LLL] += rhodot * r * v * dVdt;
+   cons[TAU] += (0.5 * rhodot * (v - r * om) * (v - r * om) + rhodot * eps) * dVdt;
+   if constexpr (NQ > NC) cons[NC] += rhodot * dVdt;
+}
+
 // ---------------------------------------------------------------------------
 // Cell update: RK blend (misc.c:226-244), flux gather in the reference's
 // order (phi, then r, then z faces; each cell's inner pair before its outer
@@ -945,6 +1010,8 @@ struct KUpdate {
          load_q<NQ>(gradz, nt, c, gz);
          H::visc_source(P, pr, gr, gp, gz, cs, centroid_r(rp, rm), dVdt);
       }
+      if (P.rot_frame) omega_src(P, pr, cs, rp, rm, phip, phim, zp, zm, dVdt);
+      sink_src<NQ, H::NC>(P, m.Nz == 1, cs, rp, rm, phip, phim, zp, zm, dVdt);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) cons_out[q * nt + c] = cs[q];
       if constexpr (!mhd) {

@@ -163,8 +163,14 @@ inline int validate(const dh_grid *g, const dh_params &P, long *ntot_out)
    if (P.hydro != DH_HYDRO_GRMHD &&
        (P.c2p_guess != DH_C2P_GUESS_PREVIOUS || P.c2p_stop != DH_C2P_STOP_REFERENCE))
       return fail(DH_EINVAL, "c2p_guess and c2p_stop apply to grmhd2 only");
-   if (P.rot_frame) return fail(DH_EINVAL, "RotFrame is not supported");
-   if (P.nozzle_type) return fail(DH_EINVAL, "Nozzle_Type is not supported");
+   // the rotating frame's and the nozzle's sources read the Newtonian
+   // primitives (omega, v); grmhd2's are covariant four-velocities
+   if ((P.rot_frame || P.nozzle_type) && P.hydro == DH_HYDRO_GRMHD)
+      return fail(DH_EINVAL, "RotFrame and Nozzle_Type apply to the Newtonian modules");
+   if (P.nozzle_type != 0 && P.nozzle_type != 1)
+      return fail(DH_EINVAL, "Nozzle_Type %d is not a nozzle of sink.c (0 or 1)", P.nozzle_type);
+   if (P.nozzle_type == 1 && !(P.nozzle_par[1] > 0.0))
+      return fail(DH_EINVAL, "Nozzle_Par2 (the nozzle's radius) must be positive");
    if (P.r_periodic || P.z_periodic) return fail(DH_EINVAL, "periodic boundaries are not supported");
    if (!(P.phi_order > 0.0)) return fail(DH_EINVAL, "phi_order must be positive (PHI_ORDER is 2)");
    if (g->Nr < 1 || g->Nz < 1) return fail(DH_EINVAL, "empty grid");

@@ -382,6 +382,12 @@ static int bc_kind_of_module(const char *mod, int side)
    return -1;
 }
 
+/* subtract_omega (rotframe.c:15-17): initial values in the rotating frame. */
+static void subtract_rot_omega(struct domain *d, double *prim)
+{
+   if (d->theParList.RotFrame) prim[UPP] -= d->theParList.RotOmega;
+}
+
 /* Value of initial() that the boundary module writes into ghost cell c of
  * annulus (j,k) (Boundary/polar.c:26-75: centroids; Boundary/fixed.c:24-60
  * and boundary_functions.c:21-31: midpoints / centroid array). */
@@ -399,6 +405,7 @@ static void ghost_initial(struct domain *d, struct cell *c, int j, int k, double
    }
    x[1] = phi;
    initial(prim, x);
+   subtract_rot_omega(d, prim);
 }
 
 static double noise_u(void) { return 2 * ((double)rand() / (double)(RAND_MAX)) - 1; }
@@ -479,6 +486,10 @@ static void fill_params(struct domain *d, dh_params *P, const int *bc)
                    L->initPar5, L->initPar6, L->initPar7, L->initPar8};
    memcpy(P->init_par, ip, sizeof ip);
    P->phi_order = 2.0; /* PHI_ORDER, planet.c:4 */
+   P->nozzle_par[0] = L->nozzlePar1;
+   P->nozzle_par[1] = L->nozzlePar2;
+   P->nozzle_par[2] = L->nozzlePar3;
+   P->nozzle_par[3] = L->nozzlePar4;
 }
 
 static void planets_pack(const struct domain *d, double *out)
@@ -565,7 +576,6 @@ int disco_setup(struct domain *d)
 {
    struct param_list *L = &d->theParList;
    if (L->restart_flag) return host_fail("Restart is not supported by this host layer");
-   if (L->RotFrame) return host_fail("RotFrame is not supported");
    if (L->include_atmos) return host_fail("Include_Atmos is not supported");
    if (L->R_Periodic || L->Z_Periodic) return host_fail("periodic boundaries are not supported");
    disco_setup_grid(d);
@@ -648,6 +658,7 @@ int disco_setup(struct domain *d)
             struct cell *c = &d->theCells[jk][i];
             double x[3] = {r, c->piph - .5 * c->dphi, z};
             initial(c->prim, x);
+            subtract_rot_omega(d, c->prim);
             if (L->noiseType != 0) add_noise(d, c->prim, x);
             c->real = 1;
          }
@@ -855,7 +866,6 @@ void *disco_device_context(struct domain *d) { return (accel_find(d))->ctx; }
 static int attach_reference_domain(struct domain *d)
 {
    struct param_list *L = &d->theParList;
-   if (L->RotFrame) return host_fail("RotFrame is not supported");
    if (L->R_Periodic || L->Z_Periodic) return host_fail("periodic boundaries are not supported");
    int Nr = d->Nr, Nz = d->Nz;
    int *Np = d->Np;

@@ -118,7 +118,8 @@ typedef struct dh_params {
    int nobc_rmin, nobc_rmax, nobc_zmin, nobc_zmax;
    int r_periodic, z_periodic;
    int bc_rmin, bc_rmax, bc_zmin, bc_zmax;  /* DH_BC_* per side */
-   int nozzle_type;             /* Nozzle_Type (only 0 is supported) */
+   int nozzle_type;             /* Nozzle_Type: 0 none, 1 the spherical nozzle
+                                   of sink.c:35-70 */
    int metric;                  /* DH_METRIC_* (the METRIC module) */
    int c2p_guess;               /* DH_C2P_GUESS_*: grmhd2 cons2prim's initial guess */
    int c2p_stop;                /* DH_C2P_STOP_*: grmhd2 cons2prim's stopping rule */
@@ -136,6 +137,8 @@ typedef struct dh_params {
                                    2.0 (planet.c:4); a run-time value as there,
                                    so pow() is evaluated as the reference does */
    double metric_par3;          /* Metric_Par3 (the spin a of DH_METRIC_KERR_KS) */
+   double nozzle_par[4];        /* Nozzle_Par1..4: centre radius, radius, speed,
+                                   Mach number of the nozzle (sink.c:35-70) */
 } dh_params;
 
 /* Planet record passed to dh_set_planets: 8 doubles per planet,

@@ -34,6 +34,16 @@ import build as cpu_build  # noqa: E402
     # (viscous fluxes and sources) and a planet on a circular orbit
     ("disk_planet", {"Num_R": 32}, 6),
     ("disk_planet", {"Num_R": 64, "Mass_Ratio": 1e-5, "Riemann_Solver": 0}, 4),
+    # the rotating frame (rotframe.c): corotating with the planet, and about
+    # an axis off the origin (RotD)
+    ("disk_planet", {"Num_R": 32, "RotFrame": 1, "RotOmega": 1.0}, 6),
+    ("disk_planet", {"Num_R": 32, "RotFrame": 1, "RotOmega": 0.5, "RotD": 0.3}, 4),
+    # the nozzle (sink.c): a 2-D jet of gas and passive scalar into the
+    # vortex, and a 3-D one into the magnetised disk
+    ("euler_yee_n1", {"Num_R": 24, "Nozzle_Type": 1, "Nozzle_Par1": 0.5, "Nozzle_Par2": 0.2,
+                      "Nozzle_Par3": 0.5, "Nozzle_Par4": 5.0}, 6),
+    ("mhd_kep", {"Num_R": 12, "Num_Z": 6, "Max_DT": 0.02, "Nozzle_Type": 1, "Nozzle_Par1": 1.2,
+                 "Nozzle_Par2": 0.25, "Nozzle_Par3": 0.8, "Nozzle_Par4": 10.0}, 4),
     # GRMHD on the equator of a Schwarzschild black hole: HLLC, HLL, and mesh
     # motion from the fluid's angular velocity (get_omega)
     ("gr_disk", {"Num_R": 24, "Num_Z": 6}, 8),

@@ -33,6 +33,14 @@ HOST = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
     ("mhd_kep", {**configs.C5_PAR, "Num_R": 32, "Num_Z": 10}, 10),
     # a disk-planet ensemble member (alpha viscosity, a planet on its orbit)
     ("disk_planet", {"Num_R": 128}, 20),
+    # the same in a frame rotating about an axis off the origin (rotframe.c),
+    # and a nozzle (sink.c) feeding gas and passive scalar into the vortex
+    # and into the magnetised disk
+    ("disk_planet", {"Num_R": 128, "RotFrame": 1, "RotOmega": 0.5, "RotD": 0.3}, 20),
+    ("euler_yee_n1", {"Num_R": 64, "Nozzle_Type": 1, "Nozzle_Par1": 0.5, "Nozzle_Par2": 0.2,
+                      "Nozzle_Par3": 0.5, "Nozzle_Par4": 5.0}, 10),
+    ("mhd_kep", {"Num_R": 32, "Num_Z": 8, "Nozzle_Type": 1, "Nozzle_Par1": 1.2,
+                 "Nozzle_Par2": 0.25, "Nozzle_Par3": 0.8, "Nozzle_Par4": 10.0}, 6),
     # GRMHD trajectories with HLL and HLLD over the steps a one-ulp libm change
     # moves by less than 1e-11, and GR HLLC on the 2-D disk falling in on a
     # static mesh, whose contacts stay away from the faces' speeds
