@@ -392,6 +392,18 @@ DHD bool noble2d(const C2pInv &v, double n, int stop, double &v2_io, double &eta
    return met;
 }
 
+// The residual rule's measure of a Newton start (v2, eta):
+// max(|f_a| / (eta + Q)^2, |f_b| / eta), the functions of noble2d.
+DHD double noble2d_residual(const C2pInv &v, double n, double v2, double eta)
+{
+   double Q = v.Q, psi = v.psi;
+   double w = 1.0 / sqrt(1.0 - v2);
+   double fa = (eta + Q) * (eta + Q) * v2 - psi * (2 * eta + Q) / (eta * eta) - v.s2;
+   double fb = eta - n * (1 - v2) * (eta - w) + 0.5 * Q * (1 + v2) - 0.5 * psi / (eta * eta) - v.e;
+   double ra = fabs(fa) / ((eta + Q) * (eta + Q)), rb = fabs(fb) / eta;
+   return ra > rb ? ra : rb;
+}
+
 template <int NQ>
 struct GRMHD {
    static constexpr int NC = 8;
@@ -790,7 +802,13 @@ struct GRMHD {
       double v2p = u2 / (1.0 + u2);
       double etap = w * (1.0 + gl / (gl - 1.0) * prim[PPP] / prim[RHO]);
 
-      bool learned = guess && P.c2p_guess == DH_C2P_GUESS_LEARNED && c2p_guess_ok(guess);
+      // the model's guess is taken where it is physical; with
+      // DH_C2P_GUESS_BETTER only where it also starts closer to the
+      // solution than the previous primitives (a smaller residual)
+      bool learned = guess && P.c2p_guess != DH_C2P_GUESS_PREVIOUS && c2p_guess_ok(guess) &&
+                     (P.c2p_guess != DH_C2P_GUESS_BETTER ||
+                      noble2d_residual(v, n, (double)guess[0], (double)guess[1]) <
+                          noble2d_residual(v, n, v2p, etap));
       double v2 = learned ? (double)guess[0] : v2p;
       double eta = learned ? (double)guess[1] : etap;
       int it = 0;

@@ -156,7 +156,8 @@ inline int validate(const dh_grid *g, const dh_params &P, long *ntot_out)
    } else {
       return fail(DH_EINVAL, "hydro module %d not supported", P.hydro);
    }
-   if (P.c2p_guess != DH_C2P_GUESS_PREVIOUS && P.c2p_guess != DH_C2P_GUESS_LEARNED)
+   if (P.c2p_guess != DH_C2P_GUESS_PREVIOUS && P.c2p_guess != DH_C2P_GUESS_LEARNED &&
+       P.c2p_guess != DH_C2P_GUESS_BETTER)
       return fail(DH_EINVAL, "bad c2p_guess %d", P.c2p_guess);
    if (P.c2p_stop != DH_C2P_STOP_REFERENCE && P.c2p_stop != DH_C2P_STOP_RESIDUAL)
       return fail(DH_EINVAL, "bad c2p_stop %d", P.c2p_stop);
@@ -468,7 +469,7 @@ int create_buffers(X &x, Buffers &b, const dh_grid *g, const dh_params &P)
    if (P.hydro == DH_HYDRO_GRMHD) {
       DH_TRY(alloc_n(x, &b.c2p_info, (size_t)ntot));
       DH_TRY(x.zero(b.c2p_info, (size_t)ntot));
-      if (P.c2p_guess == DH_C2P_GUESS_LEARNED) {
+      if (P.c2p_guess != DH_C2P_GUESS_PREVIOUS) {
          DH_TRY(alloc_n(x, &b.c2p_prm, sizeof(float) * C2PNET_NPARAM));
          DH_TRY(alloc_n(x, &b.c2p_frag, sizeof(unsigned short) * C2PNET_NFRAG));
          DH_TRY(alloc_n(x, &b.c2p_guess, sizeof(float) * 2 * (size_t)ntot));
@@ -639,7 +640,7 @@ struct Engine {
          }
          // with the learned guess cons2prim waits for the model's guesses,
          // which need the cells' conserved field first
-         bool learned = IsGR<H>::value && P.c2p_guess == DH_C2P_GUESS_LEARNED;
+         bool learned = IsGR<H>::value && P.c2p_guess != DH_C2P_GUESS_PREVIOUS;
          KBfacesC2p<NQ, H> kbc{m, P, b.F[0], b.F[1], b.rng[0], b.rng[1], b.piph2, b.dphi2,
                                st.pout, st.cout, b.prim, 1, learned ? 0 : 1, b.c2p_info};
          DH_TRY(run_ord(x, "bfaces_c2p", kbc, m.ntot, b.ord[ORD_CELLS]));
@@ -669,7 +670,7 @@ struct Engine {
    template <int NQ, class H>
    int timestep(double dt)
    {
-      if (b.P.c2p_guess == DH_C2P_GUESS_LEARNED && !b.c2p_model)
+      if (b.P.c2p_guess != DH_C2P_GUESS_PREVIOUS && !b.c2p_model)
          return fail(DH_EINVAL, "the learned cons2prim guess has no model (dh_set_c2p_model)");
       StepArgs sa;
       memset(&sa, 0, sizeof sa);
@@ -991,8 +992,8 @@ int export_faces(X &x, Buffers &b, int dim, int *L, int *R, int *LRtype, double 
 template <class X>
 int set_c2p_model(X &x, Buffers &b, const float *prm, long n)
 {
-   if (b.P.hydro != DH_HYDRO_GRMHD || b.P.c2p_guess != DH_C2P_GUESS_LEARNED)
-      return fail(DH_EINVAL, "the context was not created with c2p_guess = DH_C2P_GUESS_LEARNED");
+   if (b.P.hydro != DH_HYDRO_GRMHD || b.P.c2p_guess == DH_C2P_GUESS_PREVIOUS)
+      return fail(DH_EINVAL, "the context was not created with a learned c2p_guess");
    if (!prm || n != C2PNET_NPARAM)
       return fail(DH_EINVAL, "a c2p model has %d parameters (got %ld)", (int)C2PNET_NPARAM, n);
    for (long i = 0; i < n; ++i)

@@ -559,7 +559,7 @@ static int set_B_fields(struct domain *d)
 static int load_c2p_model(struct disco_accel *A)
 {
    enum { NPARAM = 4886 };
-   if (!A->c2p_model[0]) return host_fail("C2P_Guess 1 needs C2P_Model (a model file)");
+   if (!A->c2p_model[0]) return host_fail("C2P_Guess 1 or 2 needs C2P_Model (a model file)");
    FILE *f = fopen(A->c2p_model, "rb");
    if (!f) return host_fail("cannot open C2P_Model %s", A->c2p_model);
    float *w = (float *)malloc(sizeof(float) * (NPARAM + 1));
@@ -698,7 +698,7 @@ int disco_setup(struct domain *d)
    dh_params P;
    fill_params(d, &P, A->bc);
    if (dh_create(&g, &P, &A->ctx)) return host_fail("dh_create: %s", dh_last_error());
-   if (P.c2p_guess == DH_C2P_GUESS_LEARNED && load_c2p_model(A)) return 1;
+   if (P.c2p_guess != DH_C2P_GUESS_PREVIOUS && load_c2p_model(A)) return 1;
    if (dh_enable_graphs(A->ctx, A->graphs != 0))
       return host_fail("dh_enable_graphs: %s", dh_last_error());
    if (dh_autotune(A->ctx, A->autotune > 0 ? A->autotune : 0))

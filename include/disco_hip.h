@@ -73,9 +73,11 @@ enum {
  * by 10 more iterations.  DH_C2P_GUESS_LEARNED takes (v^2, eta) from the
  * model given to dh_set_c2p_model (no counterpart in the reference), falling
  * back to the previous primitives where its output is unphysical or the solve
- * from it does not converge; DH_C2P_STOP_RESIDUAL stops when both residuals
- * are below 1e-10 relative, without the extra iterations. */
-enum { DH_C2P_GUESS_PREVIOUS = 0, DH_C2P_GUESS_LEARNED = 1 };
+ * from it does not converge; DH_C2P_GUESS_BETTER takes the model's guess only
+ * where, besides, its residual is below the previous primitives' (the better
+ * of the two starts, each cell choosing); DH_C2P_STOP_RESIDUAL stops when
+ * both residuals are below 1e-10 relative, without the extra iterations. */
+enum { DH_C2P_GUESS_PREVIOUS = 0, DH_C2P_GUESS_LEARNED = 1, DH_C2P_GUESS_BETTER = 2 };
 enum { DH_C2P_STOP_REFERENCE = 0, DH_C2P_STOP_RESIDUAL = 1 };
 
 /* Boundary treatment of one side of the mesh, applied to ghost annuli after
@@ -231,7 +233,7 @@ int dh_report(dh_ctx *ctx, const double *planets, int npl, double *out);
  * reference: an MLP 8 -> 64 -> 64 -> 2 run on MFMA).
  * dh_set_c2p_model loads its n = 4886 float parameters (layout in
  * disco_amd/csrc/dh_c2pnet.h) into a context created with
- * c2p_guess = DH_C2P_GUESS_LEARNED; dh_timestep refuses to run such a
+ * c2p_guess = DH_C2P_GUESS_LEARNED or _BETTER; dh_timestep refuses to run such a
  * context before.  dh_c2p_harvest writes the model's features
  * feat[8 * ntot] and targets targ[2 * ntot] of the current state (NaN
  * outside the real cells), the training data.  dh_c2p_stats sums the last

@@ -154,6 +154,44 @@ def test_learned_guess_and_residual_rule_match_reference_rule(host, trained, gue
     g.close()
 
 
+def test_better_start_takes_the_smaller_residual(host, trained):
+    """C2P_Guess 2 starts each cell from the model's guess or the previous
+    primitives, whichever has the smaller residual.  From a state whose
+    primitives are a poor start (velocities off by 30%) it takes the model's
+    guess at a tenth of the cells or more and spends no more Newton iterations than either
+    start alone; from the run's own state it keeps the previous primitives
+    more often than C2P_Guess 1 does; every start reaches the same
+    solution."""
+    cfg, par, a = _initial()
+    g = capi.DiscoHip(cfg, par, a, library=host)
+    for _ in range(2):
+        g.timestep(0.05)
+    s = g.download(a)
+    g.close()
+    bad = s.copy()
+    bad.prim[2:5] *= 1.3
+    for state, poor in ((bad, True), (s, False)):
+        outs, stats = {}, {}
+        for mode in (0, 1, 2):
+            h = capi.DiscoHip(cfg, {**par, "C2P_Guess": mode, "C2P_Stop": 1}, state, library=host)
+            if mode:
+                h.set_c2p_model(trained[0])
+            h.timestep(0.05)
+            outs[mode], stats[mode] = h.download(a), h.c2p_stats()
+            h.close()
+        for mode in (1, 2):
+            worst = max(field_errors(outs[mode], outs[0]).values())
+            assert worst < 1e-12, (mode, worst)
+        st = stats[2]
+        assert st["learned"] + st["fallbacks"] == st["cells"]
+        if poor:
+            assert st["learned"] > 0.1 * st["cells"], st
+            assert st["iterations"] <= 1.02 * min(stats[0]["iterations"],
+                                                  stats[1]["iterations"]), stats
+        else:
+            assert st["learned"] < stats[1]["learned"], stats
+
+
 def test_model_is_validated(host, trained):
     p = trained[0]
     cfg, par, a = _initial({"C2P_Guess": 1})
