@@ -120,6 +120,7 @@ def kernel_bytes(name: str, nq: int, mhd: bool, nt: int, nr: int, nfr: int, nfz:
     (double); every cell has two merge steps per direction (int) and its
     annulus index cjk (int)."""
     d, i = 8, 4
+    nql = nq - 3 if mhd else nq         # components the fluxes update
     nf = {"r": nfr, "z": nfz}.get(name[-1:], 0)
     nfa = nfr + nfz
     steps = 2 * i * nt                  # the two merge steps of one direction
@@ -136,9 +137,9 @@ def kernel_bytes(name: str, nq: int, mhd: bool, nt: int, nr: int, nfr: int, nfz:
     if name in ("grad_r", "grad_z"):
         return steps + i * nt + nf * (2 * i + 2 * d) + d * nt * (2 + 3 * nq)
     if name == "flux_phi":
-        return d * (3 * nq + 2) * nt + i * nt + (d * 4 * nt if mhd else 0)
+        return d * (2 * nq + nql + 2) * nt + i * nt + (d * 4 * nt if mhd else 0)
     if name in ("flux_r", "flux_z"):
-        return (nf * (2 * i + 2 * d) + d * nt * (2 + 3 * nq) + d * nq * nf
+        return (nf * (2 * i + 2 * d) + d * nt * (2 + 3 * nq) + d * nql * nf
                 + (3 * d * nf if mhd else 0))
     if name == "edges_collect":
         return d * 21 * nt + 2 * steps + i * nt + 3 * d * nfa
@@ -161,8 +162,9 @@ def kernel_bytes(name: str, nq: int, mhd: bool, nt: int, nr: int, nfr: int, nfz:
     if name == "update":
         # cons in/out, rkcons on the second substep only, prim in for the
         # sources (+ prim out for hydro), phi/r/z face deltas, geometry in
-        # and moved geometry out
-        return (d * nq * nr * (2.5 + 1 + (0 if mhd else 1)) + d * nq * (nt + nfa)
+        # and moved geometry out; with constrained transport the field
+        # components of cons and of the face deltas are not touched
+        return (d * nr * (2.5 * nql + nq + (0 if mhd else nq)) + d * nql * (nt + nfa)
                 + 2 * steps + i * nt + d * 5 * nt)
     if name == "c2p_guess":            # the invariants' inputs in, (v^2, eta) out
         return d * 8 * nr + d * 2 * nt + i * nt + 4 * 2 * nt

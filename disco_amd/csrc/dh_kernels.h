@@ -48,6 +48,16 @@ struct IsMHD {
    static constexpr bool value = H::NC == 8;
 };
 
+// The conserved components a cell's fluxes update.  With constrained
+// transport (the MHD modules) the cell's field components are set from its
+// faces' magnetic fluxes after the update (KBfacesC2p, bfields.c:88-309),
+// so their face fluxes are neither stored nor gathered.
+template <class H>
+DHD constexpr bool flux_updates(int q)
+{
+   return !(H::NC == 8 && q >= BRR && q <= BZZ);
+}
+
 template <class H>
 struct IsGR {
    static constexpr bool value = false;
@@ -672,7 +682,8 @@ struct KFluxPhi {
          add_visc<NQ, H>(P, pL, pR, gr, gp, gz, r, n, Flux);
       }
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) dFp[q * nt + c] = (Flux[q] - w * Ustr[q]) * dAdt;
+      for (int q = 0; q < NQ; ++q)
+         if (flux_updates<H>(q)) dFp[q * nt + c] = (Flux[q] - w * Ustr[q]) * dAdt;
       if constexpr (mhd) {
          double e[4];
          H::flux_to_E(Flux, Ustr, r, 0, e);
@@ -748,7 +759,8 @@ struct KFluxTrans {
       }
       double dAdt = F.dA[f] * *dtp;
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) dF[q * nfs + f] = (Flux[q] - 0.0 * Ustr[q]) * dAdt;
+      for (int q = 0; q < NQ; ++q)
+         if (flux_updates<H>(q)) dF[q * nfs + f] = (Flux[q] - 0.0 * Ustr[q]) * dAdt;
       if constexpr (mhd) {
          double e[4];
          H::flux_to_E(Flux, Ustr, r, dim, e);
@@ -892,32 +904,33 @@ DHD void sink_src(const dh_params &P, bool twoD, double *cons, double rp, double
 // hydro, cons2prim (misc.c:290-325).  Ghost cells only move.
 // ---------------------------------------------------------------------------
 
-template <int NQ>
+template <int NQ, class H>
 DHD void gather_range(const double *dF, size_t nfs, int base, int lo, int hi, bool add,
                       double *cons)
 {
    for (int f = base + lo; f <= base + hi; ++f) {
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
+         if (!flux_updates<H>(q)) continue;
          if (add) cons[q] += dF[q * nfs + f];
          else cons[q] -= dF[q * nfs + f];
       }
    }
 }
 
-template <int NQ>
+template <int NQ, class H>
 DHD void gather_pairs(const Mesh &m, int dim, long c, int a, const int *fI, const int *rng,
                       int nf, const double *dF, double *cons)
 {
    CellRanges cr = load_ranges(m, dim, rng, fI, c, a);
    size_t nfs = (size_t)nf;
    if (cr.in) {
-      gather_range<NQ>(dF, nfs, cr.base_in, cr.in_h0, cr.in_h1, true, cons);
-      gather_range<NQ>(dF, nfs, cr.base_in, cr.in_t0, cr.in_t1, true, cons);
+      gather_range<NQ, H>(dF, nfs, cr.base_in, cr.in_h0, cr.in_h1, true, cons);
+      gather_range<NQ, H>(dF, nfs, cr.base_in, cr.in_t0, cr.in_t1, true, cons);
    }
    if (cr.out) {
-      gather_range<NQ>(dF, nfs, cr.base_out, cr.out_h0, cr.out_h1, false, cons);
-      gather_range<NQ>(dF, nfs, cr.base_out, cr.out_t0, cr.out_t1, false, cons);
+      gather_range<NQ, H>(dF, nfs, cr.base_out, cr.out_h0, cr.out_h1, false, cons);
+      gather_range<NQ, H>(dF, nfs, cr.base_out, cr.out_t0, cr.out_t1, false, cons);
    }
 }
 
@@ -966,9 +979,13 @@ struct KUpdate {
       dphi_out[c] = dp_new;
       if (!is_real(m, j, k)) return;
 
+      // the field components (constrained transport) are left to
+      // KBfacesC2p, which sets them from the faces' magnetic fluxes
       double cs[NQ];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
+         cs[q] = 0.0;
+         if (!flux_updates<H>(q)) continue;
          double v = cons[q * nt + c];
          if (RK != 0.0) v = (1. - RK) * v + RK * rkcons[q * nt + c];
          cs[q] = v;
@@ -977,17 +994,21 @@ struct KUpdate {
       // reaches face i-1 first except for i = 0, whose R face is the last one.
       if (i == 0) {
 #pragma unroll
-         for (int q = 0; q < NQ; ++q) cs[q] -= dFp[q * nt + c];
+         for (int q = 0; q < NQ; ++q)
+            if (flux_updates<H>(q)) cs[q] -= dFp[q * nt + c];
 #pragma unroll
-         for (int q = 0; q < NQ; ++q) cs[q] += dFp[q * nt + cm];
+         for (int q = 0; q < NQ; ++q)
+            if (flux_updates<H>(q)) cs[q] += dFp[q * nt + cm];
       } else {
 #pragma unroll
-         for (int q = 0; q < NQ; ++q) cs[q] += dFp[q * nt + cm];
+         for (int q = 0; q < NQ; ++q)
+            if (flux_updates<H>(q)) cs[q] += dFp[q * nt + cm];
 #pragma unroll
-         for (int q = 0; q < NQ; ++q) cs[q] -= dFp[q * nt + c];
+         for (int q = 0; q < NQ; ++q)
+            if (flux_updates<H>(q)) cs[q] -= dFp[q * nt + c];
       }
-      if (m.Nr > 1) gather_pairs<NQ>(m, 1, c, a, fIr, rngr, nfr, dFr, cs);
-      if (m.Nz > 1) gather_pairs<NQ>(m, 2, c, a, fIz, rngz, nfz, dFz, cs);
+      if (m.Nr > 1) gather_pairs<NQ, H>(m, 1, c, a, fIr, rngr, nfr, dFr, cs);
+      if (m.Nz > 1) gather_pairs<NQ, H>(m, 2, c, a, fIz, rngz, nfz, dFz, cs);
 
       // Sources at the cell's position before the move.
       double rp = m.r_jph[j], rm = m.r_jph[j - 1];
@@ -1013,7 +1034,8 @@ struct KUpdate {
       if (P.rot_frame) omega_src(P, pr, cs, rp, rm, phip, phim, zp, zm, dVdt);
       sink_src<NQ, H::NC>(P, m.Nz == 1, cs, rp, rm, phip, phim, zp, zm, dVdt);
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) cons_out[q * nt + c] = cs[q];
+      for (int q = 0; q < NQ; ++q)
+         if (flux_updates<H>(q)) cons_out[q * nt + c] = cs[q];
       if constexpr (!mhd) {
          // cons2prim at the moved position.
          double r = centroid_r(rp, rm);
