@@ -388,129 +388,6 @@ def run_ensemble(args, w, cfg):
     return out
 
 
-def dominant_roofline(kernels, ba, ncell, workload):
-    """Roofline of the kernel with the most device time per step.  Its
-    algorithmic bytes per launch are the survey's per-zone-update figure
-    B_alg taken over one substep (B_alg / 2: the RK2 step reads and writes
-    the state {cons, Phi, piph} twice) times the real cells a launch covers;
-    the kernel's own byte model (every array it touches once) is reported
-    beside it as kernel_model_GBps."""
-    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
-    row = kernels[dom]
-    mean_s = row["mean_launch_ms"] * 1e-3
-    alg = ba / 2 * ncell
-    ach = alg / mean_s / 1e9 if mean_s > 0 else 0.0
-    return {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS,
-            "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
-            "traffic": measured_traffic(dom, workload),
-            "algorithmic_bytes_per_launch": alg,
-            "algorithmic_bytes_rule": f"B_alg/2 = {ba / 2:g} B per real cell per substep x "
-                                      f"{ncell} cells",
-            "mean_launch_ms": row["mean_launch_ms"],
-            "kernel_model_bytes_per_launch": row["model_bytes_per_launch"],
-            "kernel_model_GBps": row["model_GBps"],
-            "share_of_step": row["ms_per_step"] / sum(v["ms_per_step"] for v in kernels.values())}
-
-
-def measured_traffic(kernel: str, workload: str):
-    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
-    summary (profiles/traffic_<workload>.json), or None."""
-    path = os.path.join(REPO, "profiles", f"traffic_{workload}.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        table = json.load(f)
-    row = table.get("kernels", {}).get(kernel)
-    return row.get("hbm_bytes_per_launch") if row else None
-
-
-def ensemble_members(w):
-    """(Num_R, Mass_Ratio) of every member: Nr uniform in {64, 128, 256, 512}
-    and q log-uniform in [1e-5, 1e-3], from the workload's seed."""
-    import numpy as np
-    rng = np.random.default_rng(w["seed"])
-    return [(int(rng.choice([64, 128, 256, 512])), float(10.0 ** rng.uniform(-5.0, -3.0)))
-            for _ in range(w["members"])]
-
-
-def run_ensemble(args, w, cfg):
-    """Independent runs stepped together: each step takes every member's
-    CFL time step (a device round trip per member), then launches every
-    member's RK2 step on its own stream, so that the members' kernels
-    overlap on the GPU.  value = all members' zone-updates per second."""
-    import ctypes as C
-    from disco_amd import capi
-    from disco_amd.host_lib import HostRun
-    steps = args.steps if args.steps is not None else w["steps"]
-    warmup = args.warmup if args.warmup is not None else w["warmup"]
-    members = ensemble_members(w)
-    tmp = tempfile.mkdtemp(prefix="bench_ens_")
-    t0 = time.perf_counter()
-    runs = []
-    for i, (nr, q) in enumerate(members):
-        par = cfg.par_dict(Num_R=nr, Mass_Ratio=q, Autotune=args.autotune)
-        runs.append(HostRun(w["config"], state_io.write_par(os.path.join(tmp, str(i)), par)))
-    t_setup = time.perf_counter() - t0
-    ncell = sum(r.real_cells for r in runs)
-    print(f"bench {args.workload}: set-up of {len(runs)} members, {ncell} cells in "
-          f"{t_setup:.1f} s", file=sys.stderr, flush=True)
-    lib = capi.lib()
-    for r in runs:
-        if lib.dh_enable_graphs(C.c_void_p(r.ctx), 1 if args.graphs == "on" else 0):
-            raise RuntimeError(lib.dh_last_error().decode())
-
-    def step_all():
-        dts = [r.getmindt() for r in runs]
-        for r, dt in zip(runs, dts):
-            r.timestep(dt)
-
-    def sync_all():
-        for r in runs:
-            if lib.dh_sync(C.c_void_p(r.ctx)):
-                raise RuntimeError(lib.dh_last_error().decode())
-
-    for _ in range(args.autotune):  # the members' launch tuning (set-up, untimed)
-        step_all()
-    for _ in range(warmup):
-        step_all()
-    sync_all()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        step_all()
-    sync_all()
-    secs = time.perf_counter() - t0
-    print(f"bench {args.workload}: {steps} timed steps in {secs:.2f} s", file=sys.stderr, flush=True)
-    for r in runs:
-        r.close()
-    value = ncell * steps / secs
-    counts = {n: sum(1 for m in members if m[0] == n) for n in (64, 128, 256, 512)}
-    out = {
-        "metric": "zone-updates/s on 1 MI355X (%HBM roofline)",
-        "value": value,
-        "unit": "zone-updates/s",
-        "n_gpus": 1,
-        "steps": steps,
-        "warmup": warmup,
-        "ms_per_step": secs / steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (initial conditions from the workload's Initial plugin)",
-        "config": {"workload": w["desc"], "name": args.workload, "graphs": args.graphs == "on", "members": len(members),
-                   "members_by_Nr": counts, "real_cells": ncell, "setup_s": t_setup},
-        "roofline": None,
-        "step_roofline": {"b_alg_bytes_per_zone_update": b_alg(cfg.nq, cfg.nf),
-                          "achieved": b_alg(cfg.nq, cfg.nf) * value / 1e9,
-                          "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                          "frac": b_alg(cfg.nq, cfg.nf) * value / 1e9 / PEAK_HBM_GBS},
-        "cpu_baseline": None,
-    }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg, w["cpu_over"], w["cpu_steps"])
-    return out
-
-
 # The GRMHD workload of the cons2prim comparison: its first `harvest` steps
 # (plain guess) give the training data, then `steps` are timed.
 C2P_COMPARE = dict(workload="kerr_torus", harvest=50, every=5, rows=20000, epochs=30,
