@@ -253,7 +253,7 @@ def cpu_baseline(cfg, over, steps, cores=None):
 
 def dominant_roofline(kernels, ba, ncell, workload):
     """Roofline of the kernel with the most device time per step.  Its
-    algorithmic bytes per launch are the survey's per-zone-update figure
+    algorithmic bytes per launch are the compulsory per-zone-update figure
     B_alg taken over one substep (B_alg / 2: the RK2 step reads and writes
     the state {cons, Phi, piph} twice) times the real cells a launch covers;
     the kernel's own byte model (every array it touches once) is reported

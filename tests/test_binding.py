@@ -1,4 +1,4 @@
-"""The documented binding built and run: the
+"""The documented binding of the drop-in, built and run: the
 reference's own program -- its main.c, readpar.c, gridsetup.c, domain.c,
 report.c, misc.c (set-up), the Hydro/HLLD/Boundary/Initial/Planets modules --
 compiled from /root/reference without timestep.c, onestep.c and analysis.c,
@@ -38,7 +38,8 @@ CORE = ["readpar.c", "timestep.c", "onestep.c", "riemann.c", "mpisetup.c", "grid
         "planet.c", "omega.c", "analysis.c", "bfields.c", "rotframe.c",
         "boundary_functions.c", "geometry_functions.c", "noise.c", "sink.c",
         "Restart/no_restart.c", "Geometry/cylindrical.c", "Hydro/Frame/normal.c", "main.c"]
-# the objects the binding leaves out
+# the objects the binding leaves out: the step's own files, whose entry
+# points the host layer supplies
 REPLACED = {"timestep.c", "onestep.c", "analysis.c"}
 
 CASES = {
