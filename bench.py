@@ -21,14 +21,16 @@ is exactly K steps between device synchronisations.  Prints one JSON line:
                  written once) over the same time
   step_roofline  B_alg x zone-updates/s against 8 TB/s, with
                  B_alg = 40 (NQ + NF + 1) bytes per zone-update (the state
-                 {cons, Phi, piph} read and written over the RK2 step)
+                 {cons, Phi, piph} read and written over the RK2 step); also
+                 against the bandwidth a plain device copy measures on this
+                 GPU (measured_copy), as roofline is
+  grmhd_c2p      the GRMHD workload stepped with the plain (previous
+                 primitives) guess, the learned one and the better of the
+                 two per cell, all with the residual stopping rule (both
+                 residuals below 1e-10): the ratios of their zone-updates/s
   cpu_baseline   the reference build (oracle/_ref) on this host's cores: one
                  serial process per core, each stepping the same bounded
                  sample of the workload; value = their summed zone-updates/s
-  grmhd_c2p      the GRMHD workload stepped with the plain (previous
-                 primitives) and the learned cons2prim guess, both with the
-                 residual stopping rule (both residuals below 1e-10): the
-                 ratio of their zone-updates/s
 """
 from __future__ import annotations
 
@@ -249,6 +251,36 @@ def cpu_baseline(cfg, over, steps, cores=None):
                       f"same workload at {size} ({results[0]['cells']} cells each) with the "
                       f"reference timestep built from its sources (gcc -O3, USE_MPI=0); "
                       f"{secs:.1f} s of stepping; value is their summed zone-updates/s"}
+
+
+def copy_bandwidth(gib: int = 4):
+    """The HBM bandwidth a plain device-to-device copy reaches on this GPU:
+    a gib-GiB float64 buffer copied 10 times after one untimed copy, timed
+    with events on torch's stream; bytes read and written over the time.
+    None without a visible GPU."""
+    try:
+        import torch
+    except ImportError:
+        return None
+    if not torch.cuda.is_available():
+        return None
+    n = gib << 27
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 10
+    e0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record()
+    e1.synchronize()
+    secs = e0.elapsed_time(e1) * 1e-3 / reps
+    del x, y
+    torch.cuda.empty_cache()
+    return {"GBps": 2 * n * 8 / secs / 1e9, "bytes_per_copy": 2 * n * 8,
+            "how": f"torch device-to-device copy of {gib} GiB, mean of {reps}"}
 
 
 def dominant_roofline(kernels, ba, ncell, workload):
@@ -650,6 +682,13 @@ def main(argv=None):
                           "iterations_per_cell": st[1] / st[0],
                           "learned_fraction": st[2] / st[0], "fallback_fraction": st[3] / st[0]}
     run.close()
+    # the measured streaming bandwidth beside the published 8 TB/s
+    cb = copy_bandwidth()
+    if cb:
+        out["step_roofline"]["measured_copy"] = cb
+        out["step_roofline"]["frac_of_measured_copy"] = step_ach / cb["GBps"]
+        if roofline:
+            roofline["frac_of_measured_copy"] = roofline["achieved"] / cb["GBps"]
     if not args.no_c2p_compare:
         out["grmhd_c2p"] = grmhd_c2p_compare(args)
     if not args.no_cpu_baseline:
