@@ -253,15 +253,11 @@ def cpu_baseline(cfg, over, steps, cores=None):
                       f"{secs:.1f} s of stepping; value is their summed zone-updates/s"}
 
 
-def copy_bandwidth(gib: int = 4):
-    """The HBM bandwidth a plain device-to-device copy reaches on this GPU:
-    a gib-GiB float64 buffer copied 10 times after one untimed copy, timed
-    with events on torch's stream; bytes read and written over the time.
-    None without a visible GPU."""
-    try:
-        import torch
-    except ImportError:
-        return None
+def _copy_bandwidth_here(gib: int = 4):
+    """A gib-GiB float64 buffer copied device to device 10 times after one
+    untimed copy, timed with events on torch's stream: bytes read and
+    written over the time."""
+    import torch
     if not torch.cuda.is_available():
         return None
     n = gib << 27
@@ -277,10 +273,22 @@ def copy_bandwidth(gib: int = 4):
     e1.record()
     e1.synchronize()
     secs = e0.elapsed_time(e1) * 1e-3 / reps
-    del x, y
-    torch.cuda.empty_cache()
     return {"GBps": 2 * n * 8 / secs / 1e9, "bytes_per_copy": 2 * n * 8,
-            "how": f"torch device-to-device copy of {gib} GiB, mean of {reps}"}
+            "how": f"torch device-to-device copy of {gib} GiB, mean of {reps}, own process"}
+
+
+def copy_bandwidth():
+    """The HBM bandwidth a plain device-to-device copy reaches on this GPU,
+    measured in a child process (torch's runtime set up beside this
+    process's library would not see the device); None if it cannot be."""
+    code = ("import json, sys; sys.path.insert(0, %r); import bench; "
+            "print(json.dumps(bench._copy_bandwidth_here()))" % REPO)
+    try:
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                           timeout=300)
+        return json.loads(r.stdout.strip().splitlines()[-1]) if r.returncode == 0 else None
+    except (subprocess.TimeoutExpired, ValueError, IndexError):
+        return None
 
 
 def dominant_roofline(kernels, ba, ncell, workload):
