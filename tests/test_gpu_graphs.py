@@ -36,8 +36,10 @@ def test_recorded_timestep_matches_eager(cname, over, steps):
     cfg = configs.get(cname)
     print(f"case {cname} {over}: {steps} steps", flush=True)
     d = tempfile.mkdtemp(prefix="graphs_")
-    # in.par Graphs: 0 eager, 1 recorded (the driver's default)
-    runs = [HostRun(cname, state_io.write_par(os.path.join(d, k), cfg.par_dict(**over, Graphs=g)))
+    # in.par Graphs: 0 eager, 1 recorded (the driver's default); no tuner
+    # steps, which run eagerly
+    runs = [HostRun(cname, state_io.write_par(os.path.join(d, k),
+                                              cfg.par_dict(**over, Graphs=g, Autotune=0)))
             for k, g in (("eager", 0), ("graph", 1))]
     lib = capi.lib()
     try:
@@ -81,3 +83,4 @@ def test_face_reuse_equals_rebuild_gpu(name):
     finally:
         g.close()
         h.close()
+
