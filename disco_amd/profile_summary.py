@@ -63,36 +63,81 @@ def _rows(db: str, sql: str):
         con.close()
 
 
-def kernel_stats(db: str):
-    """{kernel: (calls, total_ns, mean_ns)} of a --kernel-trace database."""
-    out = {}
-    for name, calls, total in _rows(
-            db, "select name, count(*), sum(duration) from kernels group by name"):
-        out[name] = (calls, total, total / calls)
-    return out
+def functor_of(kernel: str) -> str:
+    """The functor a k_run / k_run_xcd launch runs, whatever its launch
+    variant (storage order, XCD-contiguous, or wrapped in a group order);
+    other kernels are their own."""
+    m = re.match(r"void k_run(?:_xcd)?<(.*)>\((.*), long\)$", kernel)
+    if not m:
+        return kernel
+    f = m.group(2)
+    g = re.match(r"dh::Grouped<(.*)>$", f)
+    return g.group(1).strip() if g else f.strip()
 
 
-def counter_means(db: str, counter: str):
-    """{kernel: (dispatches, mean value)} of one --pmc counter."""
+def after_tuning(rows):
+    """The rows (kernel, dispatch_id, value) of each functor's last launch
+    variant from the dispatch after the last one of any other variant of the
+    same functor: the launches that follow the launch tuner's sweep."""
+    last = defaultdict(lambda: -1)
+    for name, did, _ in rows:
+        last[name] = max(last[name], did)
+    by_functor = defaultdict(list)
+    for name in last:
+        by_functor[functor_of(name)].append(name)
+    keep = {}
+    for names in by_functor.values():
+        final = max(names, key=lambda n: last[n])
+        keep[final] = max([last[n] for n in names if n != final], default=-1)
+    return [r for r in rows if r[0] in keep and r[1] > keep[r[0]]]
+
+
+def kernel_stats(db: str, tuned: bool = False):
+    """{kernel: (calls, total_ns, mean_ns)} of a --kernel-trace database
+    (tuned: only the launches after the tuner's sweep)."""
+    rows = _rows(db, "select name, dispatch_id, duration from kernels")
+    if tuned:
+        rows = after_tuning(rows)
     acc = defaultdict(list)
-    for name, value in _rows(db, "select kernel_name, value from counters_collection "
-                                 f"where counter_name = '{counter}'"):
+    for name, _, dur in rows:
+        acc[name].append(dur)
+    return {k: (len(v), sum(v), sum(v) / len(v)) for k, v in acc.items()}
+
+
+def counter_means(db: str, counter: str, tuned: bool = False):
+    """{kernel: (dispatches, mean value)} of one --pmc counter (tuned: only
+    the launches after the tuner's sweep)."""
+    rows = _rows(db, "select kernel_name, dispatch_id, value from counters_collection "
+                     f"where counter_name = '{counter}'")
+    if tuned:
+        rows = after_tuning(rows)
+    acc = defaultdict(list)
+    for name, _, value in rows:
         acc[name].append(value)
     return {k: (len(v), sum(v) / len(v)) for k, v in acc.items()}
 
 
-def cmd_kernels(args):
-    stats = kernel_stats(args.db)
+def _kernel_table(stats):
     total = sum(v[1] for v in stats.values())
-    lines = [f"# Kernel trace: {args.title}", ""]
-    if args.cmd:
-        lines += ["Command:", "", "```", args.cmd, "```", ""]
-    lines += ["| kernel | timer name | calls | mean (ms) | total (ms) | share |",
-              "|---|---|---|---|---|---|"]
+    lines = ["| kernel | timer name | calls | mean (ms) | total (ms) | share |",
+             "|---|---|---|---|---|---|"]
     for name, (calls, tot, mean) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
         short = name if len(name) < 90 else name[:87] + "..."
         lines.append(f"| `{short}` | {timer_name(name)} | {calls} | {mean / 1e6:.4f} | "
                      f"{tot / 1e6:.3f} | {100 * tot / total:.1f}% |")
+    return lines
+
+
+def cmd_kernels(args):
+    lines = [f"# Kernel trace: {args.title}", ""]
+    if args.cmd:
+        lines += ["Command:", "", "```", args.cmd, "```", ""]
+    lines += ["## After the launch tuner's sweep", "",
+              "Each kernel in the launch variant the tuner kept, from its first launch after "
+              "the last launch of any other variant (the warmup and timed steps).", ""]
+    lines += _kernel_table(kernel_stats(args.db, tuned=True))
+    lines += ["", "## Every dispatch of the run", ""]
+    lines += _kernel_table(kernel_stats(args.db))
     with open(args.out, "w") as f:
         f.write("\n".join(lines) + "\n")
     print(f"wrote {args.out}")
@@ -111,8 +156,8 @@ def _copy_factor(fetch_db: str, write_db: str):
 
 def cmd_traffic(args):
     rf, wf, probe = _copy_factor(args.copy_fetch, args.copy_write)
-    fetch = counter_means(args.fetch, "FETCH_SIZE")
-    write = counter_means(args.write, "WRITE_SIZE")
+    fetch = counter_means(args.fetch, "FETCH_SIZE", tuned=args.tuned)
+    write = counter_means(args.write, "WRITE_SIZE", tuned=args.tuned)
     kernels = {}
     for name in sorted(set(fetch) | set(write)):
         tn = timer_name(name)
@@ -131,6 +176,8 @@ def cmd_traffic(args):
         row["hbm_bytes_per_launch"] = row["read_bytes_per_launch"] + row["write_bytes_per_launch"]
     out = {"workload": args.workload,
            "counters": "FETCH_SIZE and WRITE_SIZE (KiB per dispatch), one rocprofv3 --pmc run each",
+           "launches": ("each kernel in the launch variant the tuner kept, after its sweep"
+                        if args.tuned else "every dispatch"),
            "scale": {"read": rf, "write": wf,
                      "probe": f"1 GiB device-to-device copy ({probe.split('(')[0]})"},
            "command": args.cmd, "kernels": kernels}
@@ -158,14 +205,22 @@ def cmd_fp64(args):
     """FP64 rate of every kernel: wavefront FP64 instructions per dispatch
     (one --pmc run with FP64_COUNTERS) times 64 lanes, an FMA counting two
     operations, over the kernel's mean duration in a --kernel-trace run of
-    the same command."""
-    stats = kernel_stats(args.trace)
-    per = {c: counter_means(args.pmc, c) for c in FP64_COUNTERS}
+    the same workload (the launches after the tuner's sweep, in the variant
+    it kept; every launch variant of a kernel runs the same instructions)."""
+    stats = kernel_stats(args.trace, tuned=True)
+    # instruction counts per functor: every launch variant runs the same code
+    per = {}
+    for c in FP64_COUNTERS:
+        acc = defaultdict(list)
+        for name, (n, mean) in counter_means(args.pmc, c).items():
+            acc[functor_of(name)] += [mean] * n
+        per[c] = {f: sum(v) / len(v) for f, v in acc.items()}
     rows = []
     for name, (calls, tot, mean_ns) in stats.items():
-        if name not in per[FP64_COUNTERS[0]]:
+        fn = functor_of(name)
+        if fn not in per[FP64_COUNTERS[0]]:
             continue
-        add, mul, fma, trans = (per[c].get(name, (0, 0.0))[1] for c in FP64_COUNTERS)
+        add, mul, fma, trans = (per[c].get(fn, 0.0) for c in FP64_COUNTERS)
         flops = 64.0 * (add + mul + trans + 2.0 * fma)
         tf = flops / mean_ns / 1e3
         rows.append((name, mean_ns / 1e6, flops, tf, tf / PEAK_FP64_TFLOPS))
@@ -200,6 +255,8 @@ def main(argv=None):
     for a in ("--fetch", "--write", "--copy-fetch", "--copy-write", "--workload", "--out"):
         t.add_argument(a, required=True)
     t.add_argument("--cmd", default="")
+    t.add_argument("--tuned", action="store_true",
+                   help="only the launches after the launch tuner's sweep")
     sub.add_parser("copy-probe")
     f = sub.add_parser("fp64")
     for a in ("--pmc", "--trace", "--out"):
