@@ -36,6 +36,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
@@ -100,7 +101,7 @@ WORKLOADS = {
              "aspect 2.5 (Np up to 512), torus inner edge r=6, pressure maximum r=12, "
              "gamma 4/3, poloidal loops on density contours at beta=100 plus a uniform "
              "B^z=1e-4, 1% pressure noise, atmosphere rho 1e-2, P 1e-5; relativistic HLLD, "
-             "PLM 1.5, constrained transport, mesh moving with the gas, RK2, CFL 0.2",
+             "PLM 1.5, constrained transport, mesh moving with the gas, RK2, CFL 0.05",
         cpu_over={"Num_R": 48, "Num_Z": 24}, cpu_steps=4),
 }
 DEFAULT_WORKLOAD = "mhd_kep"
@@ -327,6 +328,24 @@ def measured_traffic(kernel: str, workload: str):
     return row.get("hbm_bytes_per_launch") if row else None
 
 
+def c2p_health(lib, ctx, ntot):
+    """The last cons2prim's unconverged share (solves that did not meet the
+    stopping rule within 30 Newton iterations, recorded 63 by dh_c2p_info)
+    and whether the state is finite (the report's total mass, a sum over
+    every real cell, is finite)."""
+    import ctypes as C
+    import numpy as np
+    info = np.zeros(ntot, dtype=np.uint8)
+    if lib.dh_c2p_info(ctx, info.ctypes.data_as(C.c_void_p)):
+        raise RuntimeError(lib.dh_last_error().decode())
+    solved = int((info != 0).sum())
+    rep = (C.c_double * 16)()
+    if lib.dh_report(ctx, None, 0, rep):
+        raise RuntimeError(lib.dh_last_error().decode())
+    return {"unconverged_fraction": float(((info & 63) == 63).sum()) / max(solved, 1),
+            "state_finite": bool(math.isfinite(rep[7]))}
+
+
 def ensemble_members(w):
     """(Num_R, Mass_Ratio) of every member: Nr uniform in {64, 128, 256, 512}
     and q log-uniform in [1e-5, 1e-3], from the workload's seed."""
@@ -469,7 +488,8 @@ def grmhd_c2p_compare(args):
         lib.dh_c2p_stats(ctx, st)
         stats = {"iterations_per_cell": st[1] / max(st[0], 1),
                  "learned_fraction": st[2] / max(st[0], 1),
-                 "fallback_fraction": st[3] / max(st[0], 1)}
+                 "fallback_fraction": st[3] / max(st[0], 1),
+                 **c2p_health(lib, ctx, run.ntot)}
         return run.real_cells * cc["steps"] / secs, secs, stats
 
     run = HostRun(w["config"], state_io.write_par(os.path.join(tmp, "plain"), base))
@@ -687,6 +707,7 @@ def main(argv=None):
         st = (C.c_longlong * 4)()
         if lib.dh_c2p_stats(ctx, st) == 0 and st[0]:
             out["c2p"] = {"guess": args.c2p_guess, "stop": args.c2p_stop,
+                          **c2p_health(lib, ctx, ntot),
                           "iterations_per_cell": st[1] / st[0],
                           "learned_fraction": st[2] / st[0], "fallback_fraction": st[3] / st[0]}
     run.close()
