@@ -61,9 +61,19 @@ DHD void range_from_steps(int h1, int hp, bool start, int nf, int *h0, int *t0, 
 DHD CellRanges load_ranges(const Mesh &m, int dim, const int *rng, const int *fI, long c, int a)
 {
    size_t nt = (size_t)m.ntot;
+   // The ring predecessor's steps: those of the storage predecessor c - 1,
+   // read together with the cell's own (before the annulus offset is
+   // known), except for the annulus's first cell, whose predecessor is its
+   // last.
+   long cs = c > 0 ? c - 1 : c;
+   int hp_in = rng[cs], hp_out = rng[nt + cs];
+   int h_in = rng[c], h_out = rng[nt + c];
    int c0 = m.off[a], np = m.Np[a];
    int i = (int)(c - c0);
-   long cp = i == 0 ? c0 + np - 1 : c - 1;
+   if (i == 0) {
+      hp_in = rng[c0 + np - 1];
+      hp_out = rng[nt + c0 + np - 1];
+   }
    bool start = i == m.I0[a];
    int j = a % m.Nr, k = a / m.Nr;
    CellRanges r;
@@ -84,14 +94,14 @@ DHD CellRanges load_ranges(const Mesh &m, int dim, const int *rng, const int *fI
    r.in_h1 = r.in_t1 = r.out_h1 = r.out_t1 = 0;
    if (r.in) {
       r.base_in = fI[Jin];
-      r.in_h1 = rng[c];
-      range_from_steps(r.in_h1, rng[cp], start, fI[Jin + 1] - r.base_in, &r.in_h0, &r.in_t0,
+      r.in_h1 = h_in;
+      range_from_steps(r.in_h1, hp_in, start, fI[Jin + 1] - r.base_in, &r.in_h0, &r.in_t0,
                        &r.in_t1);
    }
    if (r.out) {
       r.base_out = fI[Jout];
-      r.out_h1 = rng[nt + c];
-      range_from_steps(r.out_h1, rng[nt + cp], start, fI[Jout + 1] - r.base_out, &r.out_h0,
+      r.out_h1 = h_out;
+      range_from_steps(r.out_h1, hp_out, start, fI[Jout + 1] - r.base_out, &r.out_h0,
                        &r.out_t0, &r.out_t1);
    }
    return r;
