@@ -23,7 +23,8 @@ INCLUDE = os.path.join(REPO, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 REFERENCE = "/root/reference"
 
-# -freciprocal-math: divisions by one divisor may share its reciprocal
+# -freciprocal-math, for the device code only (the library's host code keeps
+# IEEE division): divisions by one divisor may share its reciprocal
 # (one correctly rounded 1/y, then multiplications, instead of a full
 # division sequence per quotient); the Riemann solvers divide many
 # quantities by the same density, enthalpy or wave-speed differences, and
@@ -34,7 +35,7 @@ REFERENCE = "/root/reference"
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-freciprocal-math",
+    "-Xarch_device", "-freciprocal-math",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
