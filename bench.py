@@ -122,6 +122,9 @@ def kernel_bytes(name: str, nq: int, mhd: bool, nt: int, nr: int, nfr: int, nfz:
     Per face the topology holds L, R (int), LR (byte) and up, dA, phic, dphi
     (double); every cell has two merge steps per direction (int) and its
     annulus index cjk (int)."""
+    merged = {"wcell_plm_phi": ("wcell", "plm_phi"), "flux_phi_r": ("flux_phi", "flux_r")}
+    if name in merged:                  # two launches run as one on small meshes
+        return sum(kernel_bytes(k, nq, mhd, nt, nr, nfr, nfz, zlayer) for k in merged[name])
     d, i = 8, 4
     nql = nq - 3 if mhd else nq         # components the fluxes update
     nf = {"r": nfr, "z": nfz}.get(name[-1:], 0)

@@ -836,6 +836,7 @@ struct HipExec {
       HIPCHK(hipStreamSynchronize(stream));
       return DH_OK;
    }
+   static constexpr bool merges_launches() { return true; }
 };
 
 struct dh_ctx {
@@ -1085,6 +1086,8 @@ struct GroupExec {
    {
       return body();
    }
+   // Every member must record the same launch sequence.
+   static constexpr bool merges_launches() { return false; }
 };
 
 // One launch of an ensemble: the ops of every mesh in device memory and

@@ -98,6 +98,20 @@ struct Grouped {
    DHD void operator()(long t) const { k(o.elem(t)); }
 };
 
+// Two kernels that share no data in one launch: threads [0, n1) run the
+// first over its elements, the rest the second over its own.
+template <class K1, class K2>
+struct KBoth {
+   K1 k1;
+   K2 k2;
+   long n1;
+   DHD void operator()(long t) const
+   {
+      if (t < n1) k1(t);
+      else k2(t - n1);
+   }
+};
+
 // One direction's face list: faces of annulus pair P are fI[P] .. fI[P+1]-1.
 struct Faces {
    int n, npairs;

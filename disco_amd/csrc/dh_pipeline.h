@@ -60,6 +60,10 @@ struct OrderSet {
    int n = 0;
 };
 
+// Meshes below this many cells (ghosts included) run some independent
+// launches of a substep as one.
+constexpr long SMALL_MESH_CELLS = 1L << 20;
+
 struct Buffers {
    dh_params P;
    int nq = 0, nf = 0;
@@ -87,6 +91,7 @@ struct Buffers {
    int *bcstart = nullptr;          // where each ghost cell's overlap sum starts
    // Traversal orders the executor may launch a kernel in (OrderSet)
    OrderSet ord[ORD_KINDS];
+   OrderSet none;  // storage order only (launches whose elements are not one kind)
    int bc_maxnp[4][8] = {{0}};      // largest Np of each side's ghost layers
    PlanetSet pl[2];
    StepArgs *step = nullptr;  // device copy of {dt, planets} of the current timestep
@@ -569,23 +574,39 @@ struct Engine {
       constexpr bool mhd = IsMHD<H>::value;
       const Mesh &m = b.m;
       const dh_params &P = b.P;
-      if (first_step) {
-         DH_TRY(run_ord(x, "wcell", KWcell<NQ, H>{m, P, b.prim, b.wiph}, m.ntot, b.ord[ORD_CELLS]));
-         if (P.mesh_motion == 3) DH_TRY(x.run("wcell", KWcellAvg{m, b.wiph}, m.nann));
+      // On small meshes, where every launch waits mostly on its own
+      // latency, launches that share no data go as one: the mesh velocity
+      // (wiph) with the phi slopes (gradp), the phi with the r fluxes.
+      const bool merge = x.merges_launches() && m.ntot < SMALL_MESH_CELLS;
+      KWcell<NQ, H> kw{m, P, b.prim, b.wiph};
+      KPlmPhi<NQ> kpl{m, P.plm, b.prim, b.dphi, b.gradp};
+      if (first_step && merge && P.mesh_motion != 3) {
+         DH_TRY(run_ord(x, "wcell_plm_phi", KBoth<KWcell<NQ, H>, KPlmPhi<NQ>>{kw, kpl, m.ntot},
+                        2 * m.ntot, b.none));
+      } else {
+         if (first_step) {
+            DH_TRY(run_ord(x, "wcell", kw, m.ntot, b.ord[ORD_CELLS]));
+            if (P.mesh_motion == 3) DH_TRY(x.run("wcell", KWcellAvg{m, b.wiph}, m.nann));
+         }
+         DH_TRY(run_ord(x, "plm_phi", kpl, m.ntot, b.ord[ORD_CELLS]));
       }
-      DH_TRY(run_ord(x, "plm_phi", KPlmPhi<NQ>{m, P.plm, b.prim, b.dphi, b.gradp}, m.ntot,
-                     b.ord[ORD_CELLS]));
       DH_TRY(topology<NQ>(true, reuse));
       KFluxPhi<NQ, H> kp{m, P, dt, b.prim, b.gradp, b.gradr, b.gradz, b.piph, b.dphi, b.wiph,
                          b.dFp, b.emf_phi};
-      DH_TRY(run_ord(x, "flux_phi", kp, m.ntot, b.ord[ORD_CELLS]));
+      const bool merge_flux = merge && dim_active(m, 1);
+      if (!merge_flux) DH_TRY(run_ord(x, "flux_phi", kp, m.ntot, b.ord[ORD_CELLS]));
       for (int dim = 1; dim <= 2; ++dim) {
          int d = dim - 1;
          if (!dim_active(m, dim)) continue;
          if (dim == 1) {
             KFluxTrans<NQ, H, 1> kt{m, P, dt, b.F[d], b.prim, b.gradp, b.gradr, b.gradz,
                                     b.piph, b.dphi, b.dF[d], b.emf_f[d]};
-            DH_TRY(run_ord(x, "flux_r", kt, b.F[d].n, b.ord[ORD_FACES_R]));
+            if (merge_flux)
+               DH_TRY(run_ord(x, "flux_phi_r",
+                              KBoth<KFluxPhi<NQ, H>, KFluxTrans<NQ, H, 1>>{kp, kt, m.ntot},
+                              m.ntot + b.F[d].n, b.none));
+            else
+               DH_TRY(run_ord(x, "flux_r", kt, b.F[d].n, b.ord[ORD_FACES_R]));
          } else {
             KFluxTrans<NQ, H, 2> kt{m, P, dt, b.F[d], b.prim, b.gradp, b.gradr, b.gradz,
                                     b.piph, b.dphi, b.dF[d], b.emf_f[d]};

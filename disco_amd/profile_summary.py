@@ -23,6 +23,7 @@ from collections import defaultdict
 
 # kernel symbol (as rocprofv3 prints it) -> the engine's timer name
 _NAMES = [
+    (r"KBoth<dh::KWcell<", "wcell_plm_phi"), (r"KBoth<dh::KFluxPhi<", "flux_phi_r"),
     (r"KPlmPhi<", "plm_phi"),
     (r"k_ring_start", "ring_start"),
     (r"KCellRanges<1>", "ranges_r"), (r"KCellRanges<2>", "ranges_z"),

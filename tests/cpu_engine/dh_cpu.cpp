@@ -187,6 +187,7 @@ struct HostExec {
    {
       return body();
    }
+   static constexpr bool merges_launches() { return true; }
 };
 
 struct dh_ctx {
