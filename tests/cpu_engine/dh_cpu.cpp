@@ -187,7 +187,14 @@ struct HostExec {
    {
       return body();
    }
-   static constexpr bool merges_launches() { return true; }
+   // Whether small meshes run launches that share no data as one (the GPU
+   // executor's rule); DH_CPU_MERGE=0 keeps them apart, to check that both
+   // sequences give the same state.
+   bool merges_launches() const
+   {
+      const char *e = getenv("DH_CPU_MERGE");
+      return !(e && atoi(e) == 0);
+   }
 };
 
 struct dh_ctx {

@@ -249,3 +249,25 @@ def test_strip_orders_are_permutations_and_change_nothing(cpu_lib, monkeypatch, 
         o.timestep(dt)
     _assert_same(g.download(a), o.get_state(), f"order {order}")
     g.close()
+
+
+@pytest.mark.parametrize("cname,over", [
+    ("euler_yee", {"Num_R": 24}),
+    ("mhd_kep", {"Num_R": 12, "Num_Z": 6, "Max_DT": 0.02}),
+])
+def test_separate_launches_equal_oracle(cpu_lib, monkeypatch, cname, over):
+    """Small meshes run the mesh velocity with the phi slopes and the phi
+    with the r fluxes as one launch each (every test mesh here is small);
+    with them kept apart, as on large meshes (DH_CPU_MERGE=0), the steps
+    are bit-identical with the oracle as well."""
+    cfg, par, a = _ref_initial(cname, over)
+    monkeypatch.setenv("DH_CPU_MERGE", "0")
+    g = capi.DiscoHip(cfg, par, a, library=cpu_lib)
+    o = Oracle(cfg, par, a)
+    for n in range(4):
+        dt = o.getmindt()
+        assert g.getmindt() == dt, f"step {n}: CFL dt"
+        g.timestep(dt)
+        o.timestep(dt)
+    _assert_same(g.download(a), o.get_state(), f"{cname} separate launches")
+    g.close()
